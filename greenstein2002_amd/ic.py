@@ -1,0 +1,99 @@
+"""Initial conditions in the reference's formats.
+
+``read_ic_dir`` parses the reference's ``ic/<name>/`` text files the way
+``initialize_state.c:43-133`` does (one header count, then whitespace-separated
+numbers; records are reused cyclically every min(NFRU, 1250) FRUs,
+``initialize_state.c:38-41,62-65``).  ``load_stable`` returns the same data for
+the shipped ``ic/stable`` set, packaged as ``data/ic_stable.npz``.
+``total_ca`` is the cell-Ca bookkeeping of ``initialize_state.c:233-246`` and
+``errweight`` the RK error weights of ``initialize_state.c:274-310``.
+"""
+from __future__ import annotations
+
+import os
+import re
+
+import numpy as np
+
+from . import constants as K
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+STABLE_NPZ = os.path.join(HERE, "data", "ic_stable.npz")
+REWIND = 1250
+
+_NUM = re.compile(rb"[0-9eE+\-.]+")
+
+
+def _tokens(path: str):
+    with open(path, "rb") as f:
+        return _NUM.findall(f.read())
+
+
+def read_ic_dir(path: str, nfru: int, names=None) -> dict:
+    """Read ic_{states,FRU,LCh,RyR,Ito2}_*.txt from ``path`` for ``nfru`` FRUs."""
+    names = names or dict(states="ic_states_stable.txt", fru="ic_FRU_stable.txt",
+                          lch="ic_LCh_stable.txt", ryr="ic_RyR_stable.txt",
+                          ito2="ic_Ito2_stable.txt")
+    rew = min(nfru, REWIND)
+
+    def records(name, per, conv):
+        tok = _tokens(os.path.join(path, name))
+        nrec = int(tok[0])
+        vals = np.array([conv(t) for t in tok[1:1 + per * min(nrec, rew)]])
+        vals = vals.reshape(-1, per)
+        idx = np.arange(nfru) % rew
+        return vals[idx]
+
+    out = dict(
+        ryr=records(names["ryr"], 20, int).reshape(nfru, 4, 5).astype(np.int32),
+        ltype=records(names["lch"], 8, int).reshape(nfru, 4, 2).astype(np.int32),
+        ito2=records(names["ito2"], 4, int).reshape(nfru, 4).astype(np.int32),
+        fru=records(names["fru"], 5, float).reshape(nfru, 5).astype(np.float64),
+    )
+    tok = _tokens(os.path.join(path, names["states"]))
+    out["state"] = np.array([float(t) for t in tok[:37]])
+    return out
+
+
+def load_stable(nfru: int, scale: float | None = None) -> dict:
+    """ic/stable replicated to ``nfru`` FRUs, with state[CaTOT] recomputed for
+    that FRU count and scale as initialize_state.c:233-246 does."""
+    d = np.load(STABLE_NPZ)
+    idx = np.arange(nfru) % min(nfru, REWIND)
+    out = dict(fru=d["fru"][idx].copy(), ltype=d["ltype"][idx].copy(), ryr=d["ryr"][idx].copy(),
+               ito2=d["ito2"][idx].copy(), state=d["state"].copy())
+    if scale is None:
+        scale = 12500.0 / nfru
+    out["state"][31] = total_ca(out["fru"], out["state"], scale)
+    return out
+
+
+def total_ca(fru: np.ndarray, state: np.ndarray, scale: float) -> float:
+    """initialize_state.c:233-246, sequential FRU order."""
+    a1 = 0.0
+    a2 = 0.0
+    for k in range(len(fru)):
+        cj = float(fru[k, 0])
+        a1 = a1 + cj * (1.0 + K.CSQNtot / (K.KmCSQN + cj))
+        for c in range(4):
+            x = float(fru[k, c + 1])
+            a2 = a2 + x * (1.0 + K.BSRtot / (K.KBSR + x) + K.BSLtot / (K.KBSL + x))
+    Cai = float(state[6])
+    return 1.e6 * ((a1 * K.VJSR + a2 * K.VSS) * scale + float(state[7]) * K.VNSR +
+                   Cai * K.Vmyo * (1.0 + K.CMDNtot / (K.KmCMDN + Cai) + K.EGTAtot / (K.KmEGTA + Cai)) +
+                   (float(state[9]) * K.LTRPNtot + float(state[10]) * K.HTRPNtot) * K.Vmyo)
+
+
+def errweight() -> np.ndarray:
+    """initialize_state.c:274-310."""
+    w = np.ones(37)
+    w[0] = 0.0
+    w[4] = 0.1
+    w[5] = 1 / 140.0
+    w[6] = 1.e3
+    w[7] = 2.5
+    w[20] = 0.0
+    w[30] = 0.0
+    w[31] = 0.1
+    w[36] = 0.0
+    return w
