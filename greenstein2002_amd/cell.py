@@ -1,0 +1,179 @@
+"""Whole-cell runs: the global model integrated around the device-resident FRUs.
+
+``Protocol`` is the runtime form of the reference's compile-time protocol
+switches (parameters.h:92-163).  ``Cell`` couples a ``FruCell`` with the
+37-state global state and runs the caf.c output loop (caf.c:150-251) through
+``grw_run_cell`` / ``grw_rk54pd`` (include/grw_cell.h), which follow
+rk_pd54m.c:29-345 on the host and call the FRU library for every RK stage.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+from .fru import FruCell
+from .ic import errweight, load_stable
+
+NGLOBAL = 37
+NCUR = 24
+NOTHER = 12
+
+
+class GrwProtocol(C.Structure):
+    _fields_ = [
+        ("pulse_duration", C.c_double),
+        ("pulse_amplitude", C.c_double),
+        ("period", C.c_double),
+        ("shift", C.c_double),
+        ("vclamp", C.c_int32),
+        ("chf", C.c_int32),
+        ("vclamp_duration", C.c_double),
+        ("vclamp_set", C.c_double),
+        ("vclamp_shift", C.c_double),
+        ("vclamp_hold", C.c_double),
+        ("vclamp_period", C.c_double),
+        ("serca_scale", C.c_double),
+        ("time_step", C.c_double),
+        ("step_min", C.c_double),
+        ("step_max", C.c_double),
+        ("tolrk", C.c_double),
+    ]
+
+
+class GrwRkStats(C.Structure):
+    _fields_ = [("attempts", C.c_int64), ("rejects", C.c_int64), ("forced", C.c_int64)]
+
+
+def _declare(L):
+    if getattr(L, "_grw_cell_declared", False):
+        return
+    vp = C.c_void_p
+    L.grw_protocol_default.argtypes = [C.POINTER(GrwProtocol)]
+    L.grw_protocol_default.restype = None
+    L.grw_fcn.argtypes = [C.POINTER(GrwProtocol), C.c_double, vp, vp, vp, C.c_int, vp]
+    L.grw_fcn.restype = C.c_int
+    L.grw_rk54pd.argtypes = [vp, C.POINTER(GrwProtocol), vp, C.c_double, C.c_double, C.c_double, vp,
+                             vp, C.POINTER(C.c_double), vp, C.c_int, C.POINTER(C.c_int),
+                             C.POINTER(GrwRkStats)]
+    L.grw_rk54pd.restype = C.c_int
+    L.grw_run_cell.argtypes = [vp, C.POINTER(GrwProtocol), vp, C.c_double, C.c_int, vp,
+                               C.POINTER(C.c_double), vp, vp, vp, vp, C.c_int, C.POINTER(C.c_int),
+                               C.POINTER(GrwRkStats)]
+    L.grw_run_cell.restype = C.c_int
+    L._grw_cell_declared = True
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class Protocol:
+    """parameters.h:92-163 as runtime values (defaults = the reference's)."""
+
+    pulse_duration: float = 1.0
+    pulse_amplitude: float = -100.0
+    period: float = 1000.0
+    shift: float = 10.0
+    vclamp: bool = False
+    chf: bool = False
+    vclamp_duration: float = 200.0
+    vclamp_set: float = 0.0
+    vclamp_shift: float = 300.0
+    vclamp_hold: float = -80.0
+    vclamp_period: float = 500.0
+    serca_scale: float = 1.0
+    time_step: float = 1.0
+    step_min: float = 1.e-6
+    step_max: float = 0.1
+    tolrk: float = 1.e-6
+
+    def to_c(self) -> GrwProtocol:
+        p = GrwProtocol()
+        for name, _ in GrwProtocol._fields_:
+            v = getattr(self, name)
+            setattr(p, name, int(v) if name in ("vclamp", "chf") else float(v))
+        return p
+
+
+def fcn(t: float, state, J, prot: Protocol | None = None, keepc: bool = True):
+    """fcn.c:29-1072 (host).  Returns (F, currents, state after the call)."""
+    L = _lib.load()
+    _declare(L)
+    p = (prot or Protocol()).to_c()
+    st = np.array(state, np.float64)
+    F = np.zeros(NGLOBAL)
+    cur = np.zeros(NCUR)
+    J = np.ascontiguousarray(J, np.float64)
+    check(L.grw_fcn(C.byref(p), float(t), _vp(st), _vp(F), _vp(cur), int(keepc), _vp(J)))
+    return F, cur, st
+
+
+@dataclass
+class RunResult:
+    t: np.ndarray                 # output times (ms), one per time_step
+    states: np.ndarray            # [n, 37]
+    currents: np.ndarray          # [n, 24]
+    other: np.ndarray             # [n, 12]
+    log: np.ndarray               # [m, 4] accepted steps: t, V, Cai, ICa
+    stats: dict = field(default_factory=dict)
+
+
+class Cell:
+    """One myocyte: FruCell + global state + protocol."""
+
+    def __init__(self, nfru: int, seed: int = 1234, precision: str = "fp64",
+                 protocol: Protocol | None = None, ic: dict | None = None, block: int = 0,
+                 sequential_sums: bool = False, k23_scale: float = 1.0, gamma_scale: float = 1.0,
+                 nfru_scale: float | None = None):
+        self.nfru = nfru
+        self.protocol = protocol or Protocol()
+        scale = 12500.0 / nfru if nfru_scale is None else nfru_scale
+        self.fru = FruCell(nfru, nfru_scale=scale, precision=precision, block=block,
+                           sequential_sums=sequential_sums, k23_scale=k23_scale,
+                           gamma_scale=gamma_scale)
+        ic = ic or load_stable(nfru, scale)
+        self.state = np.array(ic["state"], np.float64)
+        if self.protocol.vclamp:  # initialize_state.c:222-224
+            self.state[0] = self.protocol.vclamp_hold
+        self.errw = errweight()
+        self.fru.seed(seed)
+        st = self.fru.parallel_get_FRUs()
+        self.fru.initialize_mpi_state(ic["fru"], ic["ltype"], ic["ryr"], ic["ito2"], st["mti"], st["mt"])
+        self.t = 0.0
+        self.h = self.protocol.step_max
+        self._L = _lib.load()
+        _declare(self._L)
+
+    def close(self):
+        self.fru.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, nsteps: int, log_cap: int = 200000, outputs: bool = True) -> RunResult:
+        """nsteps calls of rk54pd of length time_step (caf.c:150-251)."""
+        p = self.protocol.to_c()
+        states = np.zeros((nsteps, NGLOBAL)) if outputs else None
+        cur = np.zeros((nsteps, NCUR)) if outputs else None
+        other = np.zeros((nsteps, NOTHER)) if outputs else None
+        log = np.zeros((max(log_cap, 1), 4))
+        nlog = C.c_int(0)
+        h = C.c_double(self.h)
+        st = GrwRkStats()
+        check(self._L.grw_run_cell(self.fru._h, C.byref(p), _vp(self.errw), float(self.t), int(nsteps),
+                                   _vp(self.state), C.byref(h), _vp(states), _vp(cur), _vp(other),
+                                   _vp(log) if log_cap else None, int(log_cap), C.byref(nlog),
+                                   C.byref(st)))
+        t_out = self.t + self.protocol.time_step * np.arange(1, nsteps + 1)
+        self.t = self.t + nsteps * self.protocol.time_step
+        self.h = h.value
+        return RunResult(t=t_out, states=states, currents=cur, other=other, log=log[: nlog.value],
+                         stats=dict(attempts=st.attempts, rejects=st.rejects, forced=st.forced))

@@ -83,6 +83,61 @@ struct MtLane {
   }
   // MersenneTwisterOne_local: y / 2^32 in [0, 1)
   __device__ __forceinline__ double uniform() { return ((double)next()) / 4294967296.0; }
+
+  // The 32 tempered words of one micro-step (MersenneTwister_fast(..., 32, ...),
+  // mt19937_local.c:97-154).  In the lazy regime a word depends only on words at
+  // least 227 positions older (or, for position 623, on word 0 of the same
+  // generation, 623 older), never on another word of the same 32-word window, so
+  // all 65 loads are issued before any store and the memory latency is paid
+  // once per step instead of once per draw.
+  __device__ __forceinline__ void block32(uint32_t out[32]) {
+    if (idx >= kMtN) {
+      const int p = idx - kMtN;
+      uint32_t cur[33], far[32];
+#pragma unroll
+      for (int k = 0; k < 33; ++k) {
+        int q = p + k;
+        q -= (q >= kMtN) ? kMtN : 0;
+        cur[k] = row[q];
+      }
+#pragma unroll
+      for (int k = 0; k < 32; ++k) {
+        int q = p + k;
+        q -= (q >= kMtN) ? kMtN : 0;
+        far[k] = row[q < kMtN - 397 ? q + 397 : q - (kMtN - 397)];
+      }
+#pragma unroll
+      for (int k = 0; k < 32; ++k) {
+        const uint32_t y = (cur[k] & 0x80000000u) | (cur[k + 1] & 0x7fffffffu);
+        far[k] = far[k] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        out[k] = temper(far[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < 32; ++k) {
+        int q = p + k;
+        q -= (q >= kMtN) ? kMtN : 0;
+        if (nlog < kMtN) {
+          log[q] = cur[k];
+          ++nlog;
+        }
+        row[q] = far[k];
+      }
+      idx += 32;
+      idx -= (idx >= 2 * kMtN) ? kMtN : 0;
+    } else if (idx + 32 <= kMtN) {
+#pragma unroll
+      for (int k = 0; k < 32; ++k) out[k] = temper(row[idx + k]);
+      idx += 32;
+    } else {  // crossing from a loaded full generation into the lazy regime
+#pragma unroll 1
+      for (int k = 0; k < 32; ++k) {
+        const uint32_t v = next();
+#pragma unroll
+        for (int j = 0; j < 31; ++j) out[j] = out[j + 1];
+        out[31] = v;
+      }
+    }
+  }
 };
 
 struct SimArgs {
@@ -244,24 +299,31 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
         R cold[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) cold[c] = (R)conc[c + 1];
+        // Clefts are processed in order 0..3 by always working on slot 0 and
+        // rotating the per-cleft registers, so the loop stays rolled (small code,
+        // static register indices).
         R maxr = R(0.0);
+#pragma unroll 1
         for (int c = 0; c < 4; ++c) {
+          const R cc = cold[0];
+          uint32_t w = ch[0];
           Exits<R> e;
-          if (!lcc_exits(ch_lcc(ch[c]), alpha, beta, gcf * cold[c], e)) bad = true;
+          if (!lcc_exits(ch_lcc(w), alpha, beta, gcf * cc, e)) bad = true;
           maxr = rmax(maxr, e.tot);
-          maxr = rmax(maxr, ch_y(ch[c]) == 2 ? vr_open : vr_closed);
-          maxr = rmax(maxr, ch_ito2(ch[c]) == 1 ? R(kfIto2) * cold[c] : R(kbIto2));
-          const RyrK<R> k = ryr_k(cold[c], k23c);
-          uint32_t w = ch[c];
+          maxr = rmax(maxr, ch_y(w) == 2 ? vr_open : vr_closed);
+          maxr = rmax(maxr, ch_ito2(w) == 1 ? R(kfIto2) * cc : R(kbIto2));
+          const RyrK<R> k = ryr_k(cc, k23c);
+#pragma unroll 1
           for (int r = 0; r < kNryr; ++r) {
-            int s = ryr_collapse(ch_ryr(w, r), k.regime);
-            if (ryr_needs_draw(s, k.regime)) s = ryr_split(s, m.uniform(), k);
-            w = ch_set_ryr(w, r, s);
+            int st = ryr_collapse(ch_ryr(w, r), k.regime);
+            if (ryr_needs_draw(st, k.regime)) st = ryr_split(st, m.uniform(), k);
+            w = ch_set_ryr(w, r, st);
             Exits<R> er;
-            ryr_exits(s, k, er);
+            ryr_exits(st, k, er);
             maxr = rmax(maxr, er.tot);
           }
-          ch[c] = w;
+          ch[0] = ch[1]; ch[1] = ch[2]; ch[2] = ch[3]; ch[3] = w;
+          cold[0] = cold[1]; cold[1] = cold[2]; cold[2] = cold[3]; cold[3] = cc;
         }
         if (bad) break;
         // ---- step length (simfru_local.c:135-146)
@@ -272,6 +334,10 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
         for (int c = 0; c < 4; ++c) cmax = rmax(cmax, conc[c + 1]);
         if (cmax > 1.5e-3) dt = rmin(dt, 5.001e-3);
         dt = rmin(dt, a.t1 - t);
+        // the step's 32 uniforms (simfru_local.c:211): loads issued here, their
+        // latency hidden behind the Heun arithmetic below
+        uint32_t draws[32];
+        m.block32(draws);
         // ---- Heun step of the local Ca (simfru_local.c:149-173)
         R cr[5], dd[5];
         double k1[5], y1[5];
@@ -292,20 +358,24 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
         fcn_fru<R>(cr, (R)dV, (R)dCai, (R)dNSR, lopen, nopen, dd);
 #pragma unroll
         for (int k = 0; k < 5; ++k) conc[k] = conc[k] + (k1[k] + dt * (double)dd[k]) / 2.0;
-        // ---- 32 draws, 8 per cleft, and the transitions (simfru_local.c:211-276)
+        // ---- transitions, 8 uniforms per cleft (simfru_local.c:215-276); same
+        // rotation as above, the cleft's uniforms are always draws[0..7]
+#pragma unroll 1
         for (int c = 0; c < 4; ++c) {
-          uint32_t w = ch[c];
+          const R cc = cold[0];
+          uint32_t w = ch[0];
           Exits<R> e;
-          double u = m.uniform();
-          lcc_exits(ch_lcc(w), alpha, beta, gcf * cold[c], e);
+          double u = ((double)draws[0]) / 4294967296.0;
+          lcc_exits(ch_lcc(w), alpha, beta, gcf * cc, e);
           int nd = pick<R>(u, dt, e);
           if (nd) w = ch_set_lcc(w, nd);
-          u = m.uniform();
+          u = ((double)draws[1]) / 4294967296.0;
           if (u >= 1.0 - dt * (double)(ch_y(w) == 2 ? vr_open : vr_closed)) w = ch_flip_y(w);
-          const RyrK<R> k = ryr_k(cold[c], k23c);
+          const RyrK<R> k = ryr_k(cc, k23c);
           bool moved = false;
+#pragma unroll
           for (int r = 0; r < kNryr; ++r) {
-            u = m.uniform();
+            u = ((double)draws[2 + r]) / 4294967296.0;
             ryr_exits(ch_ryr(w, r), k, e);
             nd = pick<R>(u, dt, e);
             if (nd) {
@@ -313,12 +383,17 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
               moved = true;
             }
           }
-          u = m.uniform();
-          if (u >= 1.0 - dt * (double)(ch_ito2(w) == 1 ? R(kfIto2) * cold[c] : R(kbIto2)))
+          u = ((double)draws[7]) / 4294967296.0;
+          if (u >= 1.0 - dt * (double)(ch_ito2(w) == 1 ? R(kfIto2) * cc : R(kbIto2)))
             w = ch_flip_ito2(w);
-          ch[c] = w;
-          lopen[c] = ch_lcc_open(w);
-          if (moved) nopen[c] = ch_ryr_open(w);
+          const bool lo = ch_lcc_open(w);
+          const int no = moved ? ch_ryr_open(w) : nopen[0];
+          ch[0] = ch[1]; ch[1] = ch[2]; ch[2] = ch[3]; ch[3] = w;
+          cold[0] = cold[1]; cold[1] = cold[2]; cold[2] = cold[3]; cold[3] = cc;
+          lopen[0] = lopen[1]; lopen[1] = lopen[2]; lopen[2] = lopen[3]; lopen[3] = lo;
+          nopen[0] = nopen[1]; nopen[1] = nopen[2]; nopen[2] = nopen[3]; nopen[3] = no;
+#pragma unroll
+          for (int j = 0; j < 24; ++j) draws[j] = draws[j + 8];
         }
         ++steps;
       }

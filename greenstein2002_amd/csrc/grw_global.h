@@ -1,0 +1,287 @@
+// grw_global.h -- the 37-state global model (membrane currents, SR and
+// cytosolic Ca, Ito1/HERG Markov chains) that consumes the release units'
+// fluxes, usable on the host and on the device.
+//
+// fcn.c:29-1072 restated with the reference's expression order; the Ito1
+// chains are evaluated by one lattice routine instead of 20 hand-written
+// derivatives.  Protocol switches that the reference fixes at compile time
+// (parameters.h:107-163) are fields of GrwProtocol.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/grw_cell.h"
+#include "grw_model.h"
+
+namespace grw {
+
+constexpr int kNglobal = 37;
+constexpr int kNcur = 24;
+
+// Ito1 lattice: two rows (activation C0..C3,O / inactivated CI0..CI3,OI).
+struct ItoRates {
+  double fw[2][5], bw[2][5], vx[2][5];
+};
+
+GRW_HD void ito_rates(double aa, double ba, double ai, double bi, const double f[4],
+                      const double b[4], ItoRates& q) {
+  q.fw[0][0] = 4.0 * aa; q.fw[0][1] = 3.0 * aa; q.fw[0][2] = 2.0 * aa; q.fw[0][3] = aa; q.fw[0][4] = 0;
+  q.fw[1][0] = 4.0 * b[0] * aa; q.fw[1][1] = 3.0 * b[1] * aa / b[0];
+  q.fw[1][2] = 2.0 * b[2] * aa / b[1]; q.fw[1][3] = b[3] * aa / b[2]; q.fw[1][4] = 0;
+  q.bw[0][0] = 0; q.bw[0][1] = ba; q.bw[0][2] = 2.0 * ba; q.bw[0][3] = 3.0 * ba; q.bw[0][4] = 4.0 * ba;
+  q.bw[1][0] = 0; q.bw[1][1] = ba / f[0]; q.bw[1][2] = 2.0 * f[0] * ba / f[1];
+  q.bw[1][3] = 3.0 * f[1] * ba / f[2]; q.bw[1][4] = 4.0 * f[2] * ba / f[3];
+  q.vx[0][0] = bi; q.vx[0][1] = f[0] * bi; q.vx[0][2] = f[1] * bi; q.vx[0][3] = f[2] * bi; q.vx[0][4] = f[3] * bi;
+  q.vx[1][0] = ai; q.vx[1][1] = ai / b[0]; q.vx[1][2] = ai / b[1]; q.vx[1][3] = ai / b[2]; q.vx[1][4] = ai / b[3];
+}
+
+// Derivatives of the lattice; summation order per state as in fcn.c:883-998.
+GRW_HD void ito_chain(const double x[2][5], const ItoRates& q, double dx[2][5]) {
+  for (int r = 0; r < 2; ++r) {
+    const int o = 1 - r;
+    for (int k = 0; k < 5; ++k) {
+      double out, in;
+      if (k == 0) {
+        out = (q.fw[r][0] + q.vx[r][0]) * x[r][0];
+        in = q.bw[r][1] * x[r][1] + q.vx[o][0] * x[o][0];
+      } else if (k == 4) {
+        out = (q.bw[r][4] + q.vx[r][4]) * x[r][4];
+        in = q.fw[r][3] * x[r][3] + q.vx[o][4] * x[o][4];
+      } else {
+        out = (r == 0) ? (q.fw[r][k] + q.bw[r][k] + q.vx[r][k]) * x[r][k]
+                       : (q.fw[r][k] + q.vx[r][k] + q.bw[r][k]) * x[r][k];
+        in = q.bw[r][k + 1] * x[r][k + 1] + q.vx[o][k] * x[o][k] + q.fw[r][k - 1] * x[r][k - 1];
+      }
+      dx[r][k] = in - out;
+    }
+  }
+}
+
+GRW_HD void ito_unpack(const double* s, int i0, double x[2][5]) {
+  for (int k = 0; k < 5; ++k) x[0][k] = s[i0 + k];
+  for (int k = 0; k < 4; ++k) x[1][k] = s[i0 + 5 + k];
+  x[1][4] = 1.0 - x[1][0] - x[1][1] - x[1][2] - x[1][3] - x[0][4] - x[0][0] - x[0][1] - x[0][2] - x[0][3];
+}
+
+GRW_HD void ito_pack(const double dx[2][5], int i0, double* F) {
+  for (int k = 0; k < 5; ++k) F[i0 + k] = dx[0][k];
+  for (int k = 0; k < 4; ++k) F[i0 + 5 + k] = dx[1][k];
+  F[i0 + 9] = -dx[1][0] - dx[1][1] - dx[1][2] - dx[1][3] - dx[0][4] - dx[0][0] - dx[0][1] - dx[0][2] - dx[0][3];
+}
+
+// fcn.c:29-1072.  J = {Jxfer, Jtr, ICa, Ito2} from the release units.  Mutates
+// state[1] below -90 mV (:663-669) and state[0] under voltage clamp (:811-814).
+GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, const double J[4],
+                const GrwProtocol& p) {
+  const double Jxfer = J[0], Jtr = J[1], ICa = J[2], Ito2 = J[3];
+  const double GKr = 0.029, GKs = 0.035, GK1 = 3.0, GKp = 1.2 * 0.002216, GNa = 12.8;
+  const double kNaCa = 0.9 * 0.30, KmNa = 87.5, KmCa = 1.38, KmK1 = 13.0, ksat = 0.2, eta = 0.35;
+  const double INaKmax = 1.3 * 0.693, KmNai = 10.0, KmKo = 1.5, IpCamax = 0.6 * 0.05, KmpCa = 0.0005;
+  const double GCab = 3.3 * 7.684e-5, GNab = 0.001;
+  const double Kfb = 0.26e-3, Krb = 1.8, KSR = 1.0, Nfb = 0.75, Nrb = 0.75;
+  const double vmax0 = 1.53 * 137.0e-6;
+  const double vmaxf = (p.serca_scale != 1.0) ? vmax0 * p.serca_scale : vmax0;
+  const double vmaxr = vmaxf;
+  const double KvScale = 1.13 * 1.03 * 1.55, Kv43Frac = 0.77;
+  const double GKv43 = Kv43Frac * KvScale * 0.1;
+  const double PKv14 = (1.0 - Kv43Frac) * KvScale * 4.792933e-7;
+  const double f43[4] = {1.8936, 14.224647456, 158.574378389, 142.936645351};
+  const double b43[4] = {6.77348, 15.6212705152, 28.7532603313, 524.576206679};
+  const double f14[4] = {0.20005, 0.320280, 13.50909223, 1151.7651385};
+  const double b14[4] = {2.2300, 12.000299, 5.3701338025, 5.2396395511};
+  const double TH = 5.320000001;
+  const double C2H_to_C3H = TH * 0.02608362043337, C3H_to_C2H = TH * 0.14832978132145;
+
+  double mNa = state[1];
+  const double hNa = state[2], jNa = state[3], Nai = state[4], Ki = state[5];
+  double Cai = state[6];
+  const double CaNSR = state[7], xKs = state[8], LTRPNCa = state[9], HTRPNCa = state[10];
+  double k43x[2][5], k14x[2][5];
+  ito_unpack(state, 11, k43x);
+  ito_unpack(state, 21, k14x);
+  const double C1H = state[32], C2H = state[33], C3H = state[34], OH = state[35];
+  const double IH = 1.0 - OH - C1H - C2H - C3H;
+  double V = state[0];
+
+  const double ton = floor((t - p.shift) / p.period) * p.period;  // :434-443
+  const double toff = ton + p.pulse_duration;
+  double Istim = 0.0;
+  if ((t - p.shift) >= ton && (t - p.shift) <= toff) Istim = Istim + p.pulse_amplitude;
+  if (p.vclamp) {  // :448-483
+    const double von = floor((t - p.vclamp_shift) / p.vclamp_period) * p.vclamp_period;
+    const double voff = von + p.vclamp_duration;
+    if (((t - p.vclamp_shift) >= von) && (von >= 0.0) && ((t - p.vclamp_shift) < voff))
+      V = p.vclamp_set;
+    else
+      V = p.vclamp_hold;
+  }
+
+  const double ENa = RT_over_F * log(Nao / Nai);
+  const double EK = RT_over_F * log(Ko / Ki);
+  const double EKs = RT_over_F * log((Ko + 0.01833 * Nao) / (Ki + 0.01833 * Nai));
+  if (Cai <= 0.0) Cai = 1.e-15;
+  const double ECa = 0.5 * RT_over_F * log(Cao / Cai);
+
+  if (mNa <= 0.0) mNa = 1.e-15;
+  const double INa = GNa * (mNa * mNa * mNa) * hNa * jNa * (V - ENa);
+  const double fKo = pow(Ko / 4.0, 0.5);
+  const double IKr = GKr * fKo * OH * (V - EK);
+  const double IKs = GKs * (xKs * xKs) * (V - EKs);
+  double IKv43 = GKv43 * k43x[0][4] * (V - EK);
+  double VF_over_RT = V / RT_over_F;
+  const double VFsq_over_RT = (1000.0 * Faraday) * VF_over_RT;
+  const double exp_VFRT = exp(VF_over_RT);
+  const double OK14 = k14x[0][4];
+  double a1, a2, a3, a4, a5;
+  a1 = Ki * exp_VFRT - Ko;
+  a2 = exp_VFRT - 1.0;
+  const double IKv14_K = (fabs(V) < 1.e-6) ? PKv14 * OK14 * 1000.0 * Faraday * (Ki - Ko)
+                                            : PKv14 * OK14 * VFsq_over_RT * (a1 / a2);
+  a1 = Nai * exp_VFRT - Nao;
+  const double IKv14_Na = (fabs(V) < 1.e-6) ? 0.02 * PKv14 * OK14 * 1000.0 * Faraday * (Nai - Nao)
+                                             : 0.02 * PKv14 * OK14 * VFsq_over_RT * (a1 / a2);
+  const double IKv14 = IKv14_K + IKv14_Na;
+  double Ito1 = IKv43 + IKv14;
+  const double K1_inf = 1.0 / (0.94 + exp(1.76 / RT_over_F * (V - EK)));
+  double IK1 = GK1 * Ko / (Ko + KmK1) * K1_inf * (V - EK);
+  const double INab = GNab * (V - ENa);
+  const double KpV = 1.0 / (1.0 + exp((7.488 - V) / 5.98));
+  const double IKp = GKp * KpV * (V - EK);
+
+  VF_over_RT = V / RT_over_F;
+  const double sigma = (exp(Nao / 67.3) - 1.0) / 7.0;
+  a1 = 1.0 + 0.1245 * exp(-0.1 * VF_over_RT);
+  a2 = 0.0365 * sigma / exp_VFRT;
+  const double fNaK = 1.0 / (a1 + a2);
+  a1 = Ko / (Ko + KmKo);
+  a2 = 1.0 + pow(KmNai / Nai, 1.5);
+  const double INaK = INaKmax * fNaK * (a1 / a2);
+  const double exp_eta = exp(eta * VF_over_RT);
+  a1 = exp_eta * (Nai * Nai * Nai) * Cao;
+  a2 = exp_eta / exp_VFRT * (Nao * Nao * Nao) * Cai;
+  a3 = 1.0 + ksat * exp_eta / exp_VFRT;
+  a4 = KmCa + Cao;
+  a5 = 5000.0 / (KmNa * KmNa * KmNa + Nao * Nao * Nao);
+  double INaCa = kNaCa * a5 * (a1 - a2) / (a4 * a3);
+  const double ICab = GCab * (V - ECa);
+  const double IpCa = IpCamax * Cai / (KmpCa + Cai);
+
+  double alpha_m;
+  if (fabs(V + 47.13) <= 1.e-4) alpha_m = 0.32 / (0.1 - 0.005 * (V + 47.13));
+  else alpha_m = (0.32 * (V + 47.13)) / (1.0 - exp(-0.1 * (V + 47.13)));
+  const double beta_m = 0.08 * exp(-V / 11.0);
+  double dmNa;
+  if (V >= -90.0) {
+    dmNa = alpha_m * (1.0 - mNa) - beta_m * mNa;
+  } else {
+    dmNa = 0.0;
+    mNa = alpha_m / (alpha_m + beta_m);
+    state[1] = mNa;
+  }
+  const double alpha_h = 0.135 * exp((80 + V) / -6.8);
+  const double beta_h = (V < -38.73809636838782) ? 3.56 * exp(0.079 * V) + 310000 * exp(0.35 * V)
+                                                 : 1 / (0.13 * (1 + exp((V + 10.66) / -11.1)));
+  double alpha_j = 0;
+  if (V < -37.78) {
+    a1 = -127140 * exp(0.2444 * V);
+    a2 = 3.474E-5 * exp(-0.04391 * V);
+    a3 = 1.0 + exp(0.311 * (V + 79.23));
+    alpha_j = (a1 - a2) * (V + 37.78) / a3;
+  }
+  const double beta_j = (V < -39.82600037702883)
+                            ? 0.1212 * exp(-0.01052 * V) / (1.0 + exp(-0.1378 * (V + 40.14)))
+                            : 0.3 * exp(-2.535E-7 * V) / (1 + exp(-0.1 * (V + 32)));
+  const double dhNa = alpha_h * (1.0 - hNa) - beta_h * hNa;
+  const double djNa = alpha_j * (1.0 - jNa) - beta_j * jNa;
+
+  const double C1H_to_C2H = TH * 0.017147641733086 * exp(0.03304608038835 * V);
+  const double C2H_to_C1H = TH * 0.03969328381141 * exp(-0.04306054163980 * V);
+  const double C3H_to_OH = TH * 0.02057448605977 * exp(0.02617412715118 * V);
+  const double OH_to_C3H = TH * 0.00134366604423 * exp(-0.02691385498399 * V);
+  const double OH_to_IH = TH * 0.10666316491288 * exp(0.00568908859717 * V);
+  const double IH_to_OH = TH * 0.00646393910049 * exp(-0.04536642959543 * V);
+  const double C3H_to_IH = TH * 0.00008039374403 * exp(0.00000069808924 * V);
+  const double IH_to_C3H = (OH_to_C3H * IH_to_OH * C3H_to_IH) / (C3H_to_OH * OH_to_IH);
+  const double dC1H = C2H_to_C1H * C2H - C1H_to_C2H * C1H;
+  const double dC2H = (C1H_to_C2H * C1H + C3H_to_C2H * C3H) - ((C2H_to_C1H + C2H_to_C3H) * C2H);
+  const double dC3H = (C2H_to_C3H * C2H + OH_to_C3H * OH + IH_to_C3H * IH) -
+                      ((C3H_to_IH + C3H_to_OH + C3H_to_C2H) * C3H);
+  const double dOH = (C3H_to_OH * C3H + IH_to_OH * IH) - ((OH_to_C3H + OH_to_IH) * OH);
+
+  const double xKs_inf = 1.0 / (1.0 + exp(-(V - 24.7) / 13.6));
+  if (fabs(V - 10) < 1.e-6) {
+    a1 = (7.19e-5) / 0.148;
+    a2 = (1.31e-4) / 0.0687;
+  } else {
+    a1 = 7.19e-5 * (V - 10.0) / (1.0 - exp(-0.148 * (V - 10.0)));
+    a2 = 1.31e-4 * (V - 10.0) / (exp(0.0687 * (V - 10.0)) - 1.0);
+  }
+  const double tau_xKs = 1.0 / (a1 + a2);
+  const double dxKs = (xKs_inf - xKs) / tau_xKs;
+
+  const double fb = pow(Cai / Kfb, Nfb);
+  const double rb = pow(CaNSR / Krb, Nrb);
+  double Jup = KSR * (vmaxf * fb - vmaxr * rb) / (1.0 + fb + rb);
+  const double dLTRPNCa = kltrpn_plus * Cai * (1.0 - LTRPNCa) - kltrpn_minus * LTRPNCa;
+  const double dHTRPNCa = khtrpn_plus * Cai * (1.0 - HTRPNCa) - khtrpn_minus * HTRPNCa;
+  const double Jtrpn = LTRPNtot * dLTRPNCa + HTRPNtot * dHTRPNCa;
+  a1 = CMDNtot * KmCMDN / (pow(Cai + KmCMDN, 2.0));
+  a2 = EGTAtot * KmEGTA / (pow(Cai + KmEGTA, 2.0));
+  const double beta_i = 1.0 / (1.0 + a1 + a2);
+
+  if (p.chf) {  // :780-786
+    IK1 = 0.68 * IK1;
+    Jup = 0.38 * Jup;
+    INaCa = 1.75 * INaCa;
+    IKv43 = 0.143 * IKv43;
+    Ito1 = IKv43 + IKv14;
+  }
+
+  a1 = Acap / (Vmyo * Faraday * 1000.0);
+  const double dNai = -(INa + INab + 3.0 * (INaCa + INaK) + IKv14_Na) * a1;
+  a3 = IKr + IKs + IK1 + IKp;
+  const double dKi = -(a3 - 2.0 * INaK + (Ito1 - IKv14_Na) + Istim) * a1;
+  a3 = ICab - 2.0 * INaCa + IpCa;
+  const double dCai = beta_i * (Jxfer * VSS / Vmyo - Jup - Jtrpn - a3 * 0.5 * a1);
+  const double dCaNSR = Jup * Vmyo / VNSR - Jtr * VJSR / VNSR;
+  const double dCaTOT = -(a3 + ICa) * 0.5 * a1 * Vmyo * 1.0e6;
+  a1 = INa + ICa + IKr + IKs;
+  a2 = IK1 + IKp + INaCa + INaK + Ito1 + Ito2;
+  a3 = IpCa + ICab + INab;
+  double dV, Itot;
+  if (p.vclamp) {
+    dV = 0.0;
+    state[0] = V;
+    Itot = a1 + a2 + a3;
+  } else {
+    Itot = a1 + a2 + a3 + Istim;
+    dV = -Itot;
+  }
+
+  ItoRates q;
+  double d43[2][5], d14[2][5];
+  ito_rates(0.543708 * exp(0.028983 * V), 0.080185 * exp(-0.0468437 * V),
+            0.0498424 * exp(-0.000373016 * V), 0.000819482 * exp(0.00000005374 * V), f43, b43, q);
+  ito_chain(k43x, q, d43);
+  ito_rates(6.0 * 0.31551 * exp(0.00695 * V), 6.0 * 0.001966 * exp(-0.08527 * V), 6.0 * 0.0004938,
+            6.0 * 0.0000176, f14, b14, q);
+  ito_chain(k14x, q, d14);
+
+  F[0] = dV; F[1] = dmNa; F[2] = dhNa; F[3] = djNa; F[4] = dNai; F[5] = dKi; F[6] = dCai;
+  F[7] = dCaNSR; F[8] = dxKs; F[9] = dLTRPNCa; F[10] = dHTRPNCa;
+  ito_pack(d43, 11, F);
+  ito_pack(d14, 21, F);
+  F[31] = dCaTOT; F[32] = dC1H; F[33] = dC2H; F[34] = dC3H; F[35] = dOH;
+  F[36] = -dC1H - dC2H - dC3H - dOH;
+
+  if (keepc && cur) {  // :1046-1071
+    cur[0] = INa; cur[1] = IKr; cur[2] = IKs; cur[3] = Ito1; cur[4] = IK1; cur[5] = IKp;
+    cur[6] = INaCa; cur[7] = INaK; cur[8] = IpCa; cur[9] = ICab; cur[10] = INab; cur[11] = ICa;
+    cur[12] = ICa * Acap / (-2.0 * Vmyo * Faraday * 1000.0);
+    cur[13] = Jup; cur[14] = Jtrpn; cur[15] = Jtr * VJSR / Vmyo; cur[16] = Jxfer * VSS / Vmyo;
+    cur[17] = IKv43; cur[18] = IKv14; cur[19] = IKv14_K; cur[20] = IKv14_Na; cur[21] = Ito2;
+    cur[22] = Istim; cur[23] = Itot;
+  }
+}
+
+}  // namespace grw

@@ -1,0 +1,70 @@
+/*
+ * grw_cell.h -- whole-cell driver over the release-unit library: the global
+ * 37-state model (fcn.c) integrated by the Prince-Dormand RK5(4)6M of
+ * rk_pd54m.c, with one Monte Carlo sub-interval of every FRU per RK stage.
+ *
+ * This is the caller side of the boundary in include/grw_fru.h: it performs,
+ * through that ABI, the same sequence of calls the reference's rk54pd makes
+ * into mpi_master.c.  Protocol switches that the reference fixes at compile
+ * time (parameters.h:92-163) are fields of GrwProtocol.
+ */
+#ifndef GRW_CELL_H
+#define GRW_CELL_H
+
+#include <stdint.h>
+
+#include "grw_fru.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  /* current-clamp stimulus, parameters.h:114-117 */
+  double pulse_duration, pulse_amplitude, period, shift;
+  /* voltage clamp, parameters.h:120-125 (vclamp != 0 enables) */
+  int32_t vclamp;
+  int32_t chf; /* heart-failure scalings, parameters.h:150-156 */
+  double vclamp_duration, vclamp_set, vclamp_shift, vclamp_hold, vclamp_period;
+  /* SERCA Vmax scale on vmaxf and vmaxr (fcn.c:218-219); 1 = reference */
+  double serca_scale;
+  /* integrator controls, parameters.h:95-98 */
+  double time_step, step_min, step_max, tolrk;
+} GrwProtocol;
+
+/* Per-run counters of the integrator. */
+typedef struct {
+  int64_t attempts, rejects, forced;
+} GrwRkStats;
+
+void grw_protocol_default(GrwProtocol *p);
+
+/* fcn.c:29-1072 on the host: derivatives F[37] of the global state for the
+ * release-unit fluxes J = {Jxfer, Jtr, ICa, Ito2}; cur[24] (may be NULL) is
+ * filled when keepc is nonzero.  Mutates state[1] below -90 mV and state[0]
+ * under voltage clamp, as the reference does. */
+int grw_fcn(const GrwProtocol *p, double t, double state[37], double F[37], double cur[24],
+            int keepc, const double J[4]);
+
+/* rk_pd54m.c:29-345 with the release units in ctx.  Integrates state[37] from
+ * t to tstep starting with step oldh; returns the next step size in *newh and
+ * the currents of the first accepted stage in cur[24] (fcn.c:1046-1071).
+ * errw[37] are the error weights (initialize_state.c:274-310).  If log is not
+ * NULL, each accepted step appends (t, V, Cai, ICa) until log_cap rows. */
+int grw_rk54pd(grw_ctx *ctx, const GrwProtocol *p, const double errw[37], double t, double tstep,
+               double oldh, double state[37], double cur[24], double *newh, double *log,
+               int log_cap, int *log_n, GrwRkStats *st);
+
+/* caf.c:150-251: from t0, nsteps calls of rk54pd of length time_step each; after
+ * every call row k of states[nsteps][37], currents[nsteps][24] (currents of the
+ * call's first stage, i.e. at its start) and other[nsteps][12]
+ * (send_calc_fru_avg) is written.  Any output pointer may be NULL.  *h carries
+ * the step size across calls (start with step_max). */
+int grw_run_cell(grw_ctx *ctx, const GrwProtocol *p, const double errw[37], double t0, int nsteps,
+                 double state[37], double *h, double *states, double *currents, double *other,
+                 double *log, int log_cap, int *log_n, GrwRkStats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

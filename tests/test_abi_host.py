@@ -1,0 +1,111 @@
+"""CPU-side checks of the native library: it loads, exports every entry point
+declared in include/*.h, rejects bad input without a GPU, and its host-side
+global model (grw_fcn) equals the reference's fcn.c on the golden fixture."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("grw_fru.h", "grw_cell.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(grw_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from greenstein2002_amd import _lib
+
+    L = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in sorted(names):
+        assert hasattr(L, n), f"{n} declared in include/ but not exported"
+    assert set(_lib.EXPORTS) <= names
+
+
+def test_create_without_gpu_fails_cleanly():
+    import greenstein2002_amd as g
+
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a GPU is visible")
+    with pytest.raises(g.GrwError):
+        g.FruCell(64)
+
+
+def test_bad_config_rejected():
+    from greenstein2002_amd import _lib
+
+    L = _lib.load()
+    cfg = _lib.GrwConfig()
+    L.grw_config_default(C.byref(cfg))
+    h = C.c_void_p()
+    cfg.nfru = 0
+    assert L.grw_create(C.byref(cfg), C.byref(h)) == -1
+    cfg.nfru = 64
+    cfg.block = 96
+    assert L.grw_create(C.byref(cfg), C.byref(h)) == -1
+    assert b"block" in L.grw_last_error()
+
+
+def test_host_fcn_matches_reference_fixture(golden):
+    from greenstein2002_amd.cell import fcn
+
+    g = golden("fcn_ref.npz")
+    for k in range(len(g["t"])):
+        F, cur, st = fcn(g["t"][k], g["state"][k], g["J"][k])
+        assert np.array_equal(F, g["F"][k]), k
+        assert np.array_equal(cur, g["cur"][k]), k
+        assert np.array_equal(st, g["state_after"][k]), k
+
+
+def test_host_fcn_matches_oracle_under_clamp_and_chf(oracle):
+    from greenstein2002_amd.cell import Protocol, fcn
+
+    rng = np.random.default_rng(5)
+    st0 = np.load(os.path.join(ROOT, "tests", "golden", "ic_stable.npz"))["state"]
+    for vclamp, chf, serca in ((1, 0, 1.0), (0, 1, 1.0), (1, 1, 0.8), (0, 0, 1.3)):
+        prot = Protocol(vclamp=bool(vclamp), chf=bool(chf), serca_scale=serca, vclamp_set=12.0,
+                        vclamp_shift=7.0)
+        op = oracle.protocol(vclamp=vclamp, chf=chf, serca_scale=serca, vclamp_set=12.0,
+                             vclamp_shift=7.0)
+        for _ in range(20):
+            s = st0 * rng.uniform(0.95, 1.05, 37)
+            t = rng.uniform(0, 600)
+            J = rng.normal(size=4)
+            F1, c1, s1 = fcn(t, s, J, prot)
+            F2, c2, s2 = oracle.fcn(t, s, J, prot=op)
+            assert np.array_equal(F1, F2) and np.array_equal(c1, c2) and np.array_equal(s1, s2)
+
+
+def test_ic_reader_and_packaged_ic():
+    """The text reader (initialize_state.c:43-133 rules) on the reference's files
+    gives the packaged ic/stable data; records repeat every 1250 FRUs."""
+    from greenstein2002_amd.ic import load_stable, read_ic_dir
+
+    d = load_stable(2600)
+    assert np.array_equal(d["fru"][1250:2500], d["fru"][:1250])
+    assert np.array_equal(d["ryr"][2500:], d["ryr"][:100])
+    ref = "/root/reference/ic/stable"
+    if not os.path.isdir(ref):
+        pytest.skip("reference ic/stable not present")
+    r = read_ic_dir(ref, 2600)
+    for k in ("fru", "ltype", "ryr", "ito2"):
+        assert np.array_equal(r[k], d[k]), k
+    assert np.array_equal(r["state"][:31], d["state"][:31])
+
+
+def test_total_ca_matches_reference_initialisation(golden):
+    """initialize_state.c:233-246 recomputes state[CaTOT] for NFRU and scale."""
+    from greenstein2002_amd.ic import load_stable
+
+    g = golden("ic_stable.npz")
+    d = load_stable(1250, 10.0)
+    assert d["state"][31] == g["state"][31]

@@ -1,0 +1,81 @@
+"""Whole-cell runs on the GPU: the host RK5(4)6M driver (include/grw_cell.h)
+around the device FRUs, against the CPU oracle's whole cell on the same seeds.
+
+With sequential (reference-order) reductions the FP64 GPU cell follows the
+oracle's trajectory to round-off: the only differences come from the device
+exp() in the FRU kernel (at most 1 ulp), so states agree to 1e-9 relative over
+the 20 ms window that includes the stimulus, upstroke and start of Ca release.
+"""
+import numpy as np
+import pytest
+
+from greenstein2002_amd.cell import Cell, Protocol
+from greenstein2002_amd.ic import load_stable
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_cell(oracle, n, seed, ms, prot=None):
+    from oracle.pyoracle import FRU_DTYPE, OracleCell
+
+    ic = load_stable(n)
+    f = np.zeros(n, FRU_DTYPE)
+    f["conc"] = ic["fru"]
+    f["lcc"] = ic["ltype"][:, :, 0]
+    f["yinact"] = ic["ltype"][:, :, 1]
+    f["ryr"] = ic["ryr"]
+    f["ito2"] = ic["ito2"]
+    for j in range(n):
+        m = oracle.mt_seed(oracle.seed_for_fru(seed, j))
+        f["mt"][j] = m["mt"]
+        f["mti"][j] = m["mti"]
+    cell = OracleCell(oracle, f, 12500.0 / n, prot)
+    state = np.array(ic["state"])
+    cur = np.zeros(24)
+    h = 0.1
+    out = []
+    for k in range(ms):
+        h, _ = cell.rk54pd(float(k), float(k + 1), h, state, cur)
+        out.append(state.copy())
+    att = int(cell.c.n_attempts)
+    cell.close()
+    return np.array(out), att
+
+
+def test_fp64_cell_follows_oracle(oracle):
+    n, seed, ms = 250, 1234, 20
+    ref, att = oracle_cell(oracle, n, seed, ms)
+    with Cell(n, seed=seed, sequential_sums=True) as cell:
+        res = cell.run(ms)
+    assert res.stats["attempts"] == att
+    np.testing.assert_allclose(res.states, ref, rtol=1e-9, atol=1e-14)
+    assert res.states[:, 0].max() > 0.0  # the stimulus at 10 ms fired an action potential
+
+
+def test_tree_reduction_cell_stays_close(oracle):
+    """Default (tree-order) reductions: same trajectory up to the round-off of a
+    different summation order, over the first 12 ms (upstroke included)."""
+    n, seed, ms = 250, 77, 12
+    ref, _ = oracle_cell(oracle, n, seed, ms)
+    with Cell(n, seed=seed) as cell:
+        res = cell.run(ms)
+    np.testing.assert_allclose(res.states[:, 0], ref[:, 0], rtol=1e-6, atol=1e-6)
+
+
+def test_voltage_clamp_cell(oracle):
+    """Voltage-clamp protocol (fcn.c:448-483): V is held, ICa flows at the step."""
+    prot = Protocol(vclamp=True, vclamp_shift=5.0, vclamp_duration=20.0, vclamp_set=10.0,
+                    vclamp_hold=-80.0, vclamp_period=500.0)
+    with Cell(500, seed=3, protocol=prot) as cell:
+        res = cell.run(15)
+    V = res.states[:, 0]
+    assert np.allclose(V[:4], -80.0) and np.allclose(V[6:15], 10.0)
+    assert res.currents[6:, 11].min() < -0.5  # inward ICa during the step (uA/uF)
+
+
+def test_mixed_cell_action_potential():
+    with Cell(250, seed=1234, precision="mixed") as cell:
+        res = cell.run(30)
+    assert np.all(np.isfinite(res.states))
+    assert res.states[:, 0].max() > 0.0
+    assert res.states[:, 6].max() > 2e-4

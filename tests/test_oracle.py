@@ -1,0 +1,313 @@
+"""Pins of the CPU oracle (oracle/grw_oracle.c) -- CPU only.
+
+Each oracle function is checked against the golden fixtures written from the
+reference's own compiled code (tests/golden/gen_golden.py) and, where the
+mathematics fixes the answer, against that: the MT19937 known answer, the
+closed form of the seed recurrence, conservation and equilibrium identities of
+the rate tables and of the local Ca right-hand side, and the categorical
+sampler's empirical frequencies.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import FRU_DTYPE, MTN
+
+
+def test_mt19937_known_answer(oracle):
+    """std::mt19937 / init_genrand(5489): the 10000th output is 4123659995."""
+    f = np.array([oracle.mt_seed(5489)])
+    w = oracle.mt_u32(f, 10000)
+    assert int(w[-1]) == 4123659995
+    assert int(w[0]) == 3499211612  # first output of the same generator
+
+
+def test_seed_formula_matches_reference_fixture(oracle, golden):
+    """initialize_ran.c:30-34 under gcc -O3 for all 12,500 FRUs (fixture from the
+    reference's object code), and the recurrence a_j = a_{j-1} + 23 j it sums."""
+    g = golden("rng_ref.npz")
+    a0 = int(g["a0"])
+    seeds = np.array([oracle.seed_for_fru(a0, j) for j in range(12500)], np.uint32)
+    assert np.array_equal(seeds, g["seeds"])
+    a = a0
+    for j in range(2000):  # below the int overflow the recurrence is exact
+        a = a + 23 * j
+        assert oracle.seed_for_fru(a0, j) == (abs(a + 4093 * j * j) + 1) % 2**32
+
+
+def test_mt_draws_match_reference_fixture(oracle, golden):
+    g = golden("rng_ref.npz")
+    for k, j in enumerate(g["draw_frus"]):
+        f = np.array([oracle.mt_seed(oracle.seed_for_fru(int(g["a0"]), int(j)))])
+        assert np.array_equal(oracle.mt_u32(f, 1300), g["draws"][k])
+
+
+def test_uniform_conversion_exact(oracle):
+    f = np.array([oracle.mt_seed(42)])
+    g = f.copy()
+    u = oracle.mt_block(f, 700)
+    w = oracle.mt_u32(g, 700)
+    assert np.array_equal(u, w.astype(np.float64) / 2.0**32)
+    assert u.max() < 1.0 and u.min() >= 0.0
+
+
+def _frus_from_rates_fixture(oracle, g):
+    n = len(g["V"])
+    f = np.zeros(n, FRU_DTYPE)
+    f["conc"] = g["in_fru"]
+    f["lcc"] = g["in_ltype"][:, :, 0]
+    f["yinact"] = g["in_ltype"][:, :, 1]
+    f["ryr"] = g["in_ryr"]
+    f["ito2"] = g["in_ito2"]
+    for k in range(n):
+        m = oracle.mt_seed(oracle.seed_for_fru(12350232, int(g["in_seed_rows"][k])))
+        f["mt"][k] = m["mt"]
+    f["mti"] = g["in_mti"]
+    return f
+
+
+def test_rates_match_reference_fixture(oracle, golden):
+    """fru_rates_local over 400 random channel states, V and CaSS in all three
+    RyR regimes, incl. the merged-state draws: bit-exact."""
+    g = golden("rates_ref.npz")
+    f = _frus_from_rates_fixture(oracle, g)
+    for k in range(len(g["V"])):
+        m, r = oracle.rates(f, k, [g["V"][k], 1.5e-4, 0.9])
+        assert m == g["max_rate"][k]
+        for c in range(4):
+            L = r["lcc_len"][c]
+            assert L == g["lcc_len"][k, c]
+            assert np.array_equal(r["lcc_rate"][c][:L], g["lcc_rate"][k, c][:L])
+            assert np.array_equal(r["lcc_dest"][c][1:L], g["lcc_dest"][k, c][1:L])
+            assert r["vinact_rate"][c] == g["vinact_rate"][k, c]
+            assert r["ito2_rate"][c] == g["ito2_rate"][k, c]
+            for i in range(5):
+                L = r["ryr_len"][c][i]
+                assert L == g["ryr_len"][k, c, i]
+                assert np.array_equal(r["ryr_rate"][c][i][:L], g["ryr_rate"][k, c, i][:L])
+                assert np.array_equal(r["ryr_dest"][c][i][1:L], g["ryr_dest"][k, c, i][1:L])
+    assert np.array_equal(f["ryr"], g["out_ryr"])
+    assert np.array_equal(f["mti"], g["out_mti"])
+
+
+def test_rate_tables_invariants(oracle):
+    """Totals equal the sum of the listed exits; max_rate bounds every total;
+    LCC detailed structure: every exit leads to a neighbour in the 12-state chain."""
+    rng = np.random.default_rng(3)
+    f = np.zeros(1, FRU_DTYPE)
+    f["mt"][0] = oracle.mt_seed(1)["mt"]
+    f["mti"] = 624
+    neighbours = {1: {2, 7}, 2: {1, 3, 8}, 3: {2, 4, 9}, 4: {3, 5, 10}, 5: {4, 6, 11}, 6: {5},
+                  7: {1, 8}, 8: {7, 2, 9}, 9: {8, 3, 10}, 10: {9, 4, 11}, 11: {10, 5, 12}, 12: {11}}
+    for _ in range(300):
+        f["lcc"] = rng.integers(1, 13, 4)
+        f["yinact"] = rng.integers(1, 3, 4)
+        f["ryr"] = rng.integers(1, 9, (4, 5))
+        f["ito2"] = rng.integers(1, 3, 4)
+        f["conc"][0, 1:] = 10 ** rng.uniform(-5, -0.7, 4)
+        m, r = oracle.rates(f, 0, [rng.uniform(-100, 60), 1e-4, 0.9])
+        for c in range(4):
+            L = r["lcc_len"][c]
+            assert np.isclose(r["lcc_rate"][c][0], r["lcc_rate"][c][1:L].sum(), rtol=1e-15)
+            assert set(r["lcc_dest"][c][1:L]) <= neighbours[int(f["lcc"][0, c])]
+            assert m >= r["lcc_rate"][c][0] and m >= r["vinact_rate"][c] and m >= r["ito2_rate"][c]
+            for i in range(5):
+                L = r["ryr_len"][c][i]
+                assert np.isclose(r["ryr_rate"][c][i][0], r["ryr_rate"][c][i][1:L].sum(), rtol=1e-15)
+                assert m >= r["ryr_rate"][c][i][0]
+    # Vinact: the two exit rates sum to 1/tau (fru_rates_local.c:345-362)
+    V = -20.0
+    f["yinact"][0] = [1, 2, 1, 2]
+    _, r = oracle.rates(f, 0, [V, 1e-4, 0.9])
+    tau = 60.0 + 340.0 / (1.0 + np.exp((V + 30.0) / 12.0))
+    assert np.isclose(r["vinact_rate"][0] + r["vinact_rate"][1], 1 / tau, rtol=1e-14)
+
+
+def test_fcn_fru_matches_reference_fixture(oracle, golden):
+    g = golden("fcn_fru_ref.npz")
+    for k in range(len(g["d"])):
+        d = oracle.fcn_fru(g["conc"][k], g["dep"][k], g["lopen"][k], g["nopen"][k])
+        assert np.array_equal(d, g["d"][k]), k
+
+
+def test_fcn_fru_equilibrium_and_conservation(oracle):
+    """All channels closed and CaSS = Cai = CaJSR = CaNSR: every flux vanishes.
+    Ring diffusion conserves Ca: with only Jss2ss acting, sum(dCaSS / beta) = 0."""
+    c = 2e-4
+    d = oracle.fcn_fru([c, c, c, c, c], [-80.0, c, c], [0, 0, 0, 0], [0, 0, 0, 0])
+    assert np.all(np.abs(d) < 1e-18)
+    conc = np.array([0.9, 1e-4, 3e-4, 2e-4, 5e-4])
+    d = oracle.fcn_fru(conc, [-80.0, 0.0, conc[0]], [0, 0, 0, 0], [0, 0, 0, 0])
+    from greenstein2002_amd import constants as K
+    beta = 1.0 / (1.0 + K.BSRtot * K.KBSR / (conc[1:] + K.KBSR) ** 2 + K.BSLtot * K.KBSL / (conc[1:] + K.KBSL) ** 2)
+    jxfer = conc[1:] / K.tauxfer
+    assert abs(np.sum(d[1:] / beta + jxfer)) < 1e-12 * np.sum(jxfer)
+
+
+def test_simfru_matches_reference_fixture(oracle, golden):
+    """Per-FRU event sequences under clamp and ramp (fixture from simfru_local)."""
+    g = golden("simfru_ref.npz")
+    ic = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "ic_stable.npz"))
+    nf = g["snap_fru"].shape[1]
+    nsub, every, dts = int(g["nsub"]), int(g["every"]), float(g["dt_sub"])
+    for p in range(len(g["v0"])):
+        f = np.zeros(nf, FRU_DTYPE)
+        f["conc"] = ic["fru"][:nf]
+        f["lcc"] = ic["ltype"][:nf, :, 0]
+        f["yinact"] = ic["ltype"][:nf, :, 1]
+        f["ryr"] = ic["ryr"][:nf]
+        f["ito2"] = ic["ito2"][:nf]
+        for j in range(nf):
+            m = oracle.mt_seed(oracle.seed_for_fru(12350232, j))
+            f["mt"][j] = m["mt"]
+            f["mti"][j] = m["mti"]
+        v0, v1 = float(g["v0"][p]), float(g["v1"][p])
+        for s in range(nsub):
+            dep0 = [v0 + (v1 - v0) * s / nsub, float(g["cai0"]), float(g["nsr"])]
+            depf = [v0 + (v1 - v0) * (s + 1) / nsub, float(g["cai1"]), float(g["nsr"])]
+            for j in range(nf):
+                oracle.simfru(dts * s, dts * (s + 1), dep0, depf, f, j)
+            if (s + 1) % every == 0:
+                q = (s + 1) // every - 1
+                assert np.array_equal(f["conc"], g["snap_fru"][p, :, q])
+                assert np.array_equal(f["lcc"], g["snap_ltype"][p, :, q, :, 0])
+                assert np.array_equal(f["ryr"], g["snap_ryr"][p, :, q])
+                assert np.array_equal(f["mti"], g["snap_mti"][p, :, q])
+        assert np.array_equal(f["mt"], g["mt_final"][p])
+
+
+def test_sampler_frequencies(oracle):
+    """The step's categorical draw realises P(exit j) = dt * k_j: brute force on
+    LCC state 2 (three exits) with all other channels frozen by zero rates."""
+    n = 4000
+    f = np.zeros(n, FRU_DTYPE)
+    f["lcc"] = 2
+    f["yinact"] = 1
+    f["ryr"] = 1
+    f["ito2"] = 2
+    f["conc"][:, 0] = 0.9
+    f["conc"][:, 1:] = 1e-12  # gamma*CaSS ~ 0: exits 2->1 and 2->3 only; RyR k12 ~ 0
+    for j in range(n):
+        f["mt"][j] = oracle.mt_seed(1000 + j)["mt"]
+    f["mti"] = 624
+    V = 0.0
+    dep = [V, 1e-12, 0.9]
+    _, r = oracle.rates(f, 0, dep)
+    kt = r["lcc_rate"][0][0]
+    dt = 0.1 / r["max_rate"]
+    # one micro-step: choose t1 = dt exactly
+    for j in range(n):
+        oracle.simfru(0.0, dt, dep, dep, f, j)
+    s = f["lcc"][:, 0]
+    p1 = r["lcc_rate"][0][1] * dt
+    p3 = r["lcc_rate"][0][2] * dt
+    for val, p in ((1, p1), (3, p3)):
+        cnt = np.sum(s == val)
+        assert abs(cnt - n * p) < 5 * np.sqrt(n * p * (1 - p)) + 1, (val, cnt, n * p)
+    assert np.sum(s == 2) > n * (1 - kt * dt) - 5 * np.sqrt(n * kt * dt) - 1
+
+
+def test_sums_match_reference_fixture(oracle, golden):
+    from greenstein2002_amd.ic import load_stable
+
+    g = golden("sums_ref.npz")
+    ic = load_stable(1250)
+    f = np.zeros(1250, FRU_DTYPE)
+    f["conc"] = ic["fru"]
+    f["lcc"] = ic["ltype"][:, :, 0]
+    f["yinact"] = ic["ltype"][:, :, 1]
+    f["ryr"] = ic["ryr"]
+    f["ito2"] = ic["ito2"]
+    assert np.array_equal(oracle.flux_sums(f), g["flux"])
+    assert np.array_equal(oracle.avg_sums(f), g["avg"])
+    b = np.zeros(1250, FRU_DTYPE)
+    b["conc"] = g["b_fru"]
+    b["lcc"] = g["b_ltype"][:, :, 0]
+    b["yinact"] = g["b_ltype"][:, :, 1]
+    b["ryr"] = g["b_ryr"]
+    b["ito2"] = g["b_ito2"]
+    assert np.array_equal(oracle.flux_sums(b), g["flux_b"])
+    assert np.array_equal(oracle.avg_sums(b), g["avg_b"])
+    for k, dep in enumerate(g["deps"]):
+        out = oracle.flux_currents(oracle.flux_sums(b), dep, 1250, float(g["scale"]))
+        assert np.array_equal(out, g["conv"][k])
+    # brute force of the counts
+    op = ((b["lcc"] == 6) | (b["lcc"] == 12)) & (b["yinact"] == 2)
+    assert oracle.flux_sums(b)[3] == op.sum()
+    assert oracle.flux_sums(b)[4] == (b["ito2"] == 2).sum()
+
+
+def test_global_fcn_matches_reference_fixture(oracle, golden):
+    g = golden("fcn_ref.npz")
+    for k in range(len(g["t"])):
+        F, cur, st = oracle.fcn(g["t"][k], g["state"][k], g["J"][k])
+        assert np.array_equal(F, g["F"][k]) and np.array_equal(cur, g["cur"][k])
+        assert np.array_equal(st, g["state_after"][k])
+
+
+def test_rk54pd_tableau_orders():
+    """rk_pd54m.c:110-229 is Dormand-Prince RK5(4)6M: the embedded weights ym
+    (19/216, ...) satisfy the quadrature order conditions of order 5 and the
+    propagated weights y (31/540, ...) those of order 4 -- the reference keeps
+    the 4th-order solution and uses the 5th-order one only for the error."""
+    b4 = np.array([31 / 540, 0, 190 / 297, -145 / 108, 351 / 220, 1 / 20])
+    b5 = np.array([19 / 216, 0, 1000 / 2079, -125 / 216, 81 / 88, 5 / 56])
+    c = np.array([0, 1 / 5, 3 / 10, 3 / 5, 2 / 3, 1])
+    for q in range(0, 5):
+        assert abs(np.dot(b5, c ** q) - 1 / (q + 1)) < 1e-14, q
+    for q in range(0, 4):
+        assert abs(np.dot(b4, c ** q) - 1 / (q + 1)) < 1e-14, q
+    assert abs(np.dot(b4, c ** 4) - 1 / 5) > 1e-3
+
+
+def test_oracle_cell_rollback_restores_bitwise(oracle):
+    """orc_save_state / orc_resume_state restore every FRU incl. its MT stream."""
+    from greenstein2002_amd.ic import load_stable
+    from oracle.pyoracle import OracleCell
+
+    n = 32
+    ic = load_stable(n)
+    f = np.zeros(n, FRU_DTYPE)
+    f["conc"] = ic["fru"]
+    f["lcc"] = ic["ltype"][:, :, 0]
+    f["yinact"] = ic["ltype"][:, :, 1]
+    f["ryr"] = ic["ryr"]
+    f["ito2"] = ic["ito2"]
+    for j in range(n):
+        f["mt"][j] = oracle.mt_seed(j + 1)["mt"]
+    f["mti"] = 624
+    cell = OracleCell(oracle, f, 12500 / n)
+    cell.save()
+    before = cell.fru.copy()
+    cell.distrib(0.0, 2.0, [10.0, 5e-4, 0.9], [20.0, 5e-4, 0.9])
+    assert not np.array_equal(cell.fru["mt"], before["mt"])
+    cell.resume()
+    assert np.array_equal(cell.fru, before)
+    cell.close()
+
+
+@pytest.mark.skipif(not __import__("os").path.exists("/root/reference/simfru_local.c"),
+                    reason="reference tree not present")
+def test_oracle_equals_live_reference_objects(oracle):
+    """Where oracle/_ref is built: random states through both, bit-exact."""
+    from oracle.pyoracle import Ref
+
+    r = Ref()
+    rng = np.random.default_rng(11)
+    state, a, _ = r.initialize_state("/root/reference", 256, 12500 / 256)
+    mt, mti = r.initialize_ran(256)
+    a["mt"][:] = mt
+    a["mti"][:] = mti
+    f = Ref.to_orc(a)
+    a2 = Ref.from_orc(f)
+    for s in range(30):
+        V = rng.uniform(-90, 50)
+        dep0 = [V, 1.5e-4, 0.92]
+        depf = [V + rng.uniform(-5, 5), 2e-4, 0.91]
+        for k in range(256):
+            oracle.simfru(0.05 * s, 0.05 * (s + 1), dep0, depf, f, k)
+            r.simfru(0.05 * s, 0.05 * (s + 1), dep0, depf, a2, k)
+    g = Ref.to_orc(a2)
+    for key in ("conc", "lcc", "yinact", "ryr", "ito2", "mt", "mti"):
+        assert np.array_equal(f[key], g[key]), key
