@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=4, choices=[1, 4], help="lanes per FRU")
+    ap.add_argument("--driver", choices=["device", "host"], default="device",
+                    help="integrator on the GPU (default) or on the host calling the FRU ABI")
     ap.add_argument("--cpu-ms", type=float, default=3.0,
                     help="simulated ms of the CPU-oracle sample (0 disables)")
     ap.add_argument("--no-timing", action="store_true",
@@ -121,9 +124,10 @@ def main():
     from greenstein2002_amd.cell import Cell, Protocol
 
     prot = Protocol()
-    cell = Cell(args.nfru, seed=args.seed, precision=args.precision, protocol=prot, block=args.block)
+    cell = Cell(args.nfru, seed=args.seed, precision=args.precision, protocol=prot, block=args.block,
+                lanes_per_fru=args.lanes)
     if args.warmup:
-        cell.run(args.warmup, outputs=False, log_cap=0)
+        cell.run(args.warmup, outputs=False, log_cap=0, driver=args.driver)
     if not args.no_timing:
         cell.fru.set_timing(True)
     cell.fru.reset_stats()
@@ -134,7 +138,7 @@ def main():
     done = 0
     while done < args.steps:
         n = min(100, args.steps - done)
-        chunks.append(cell.run(n, log_cap=0))
+        chunks.append(cell.run(n, log_cap=0, driver=args.driver))
         done += n
         print(f"[bench] t = {args.warmup + done} ms simulated, {time.perf_counter() - t0:.1f} s wall",
               file=sys.stderr, flush=True)
@@ -153,9 +157,14 @@ def main():
         achieved = alg_bytes / kernel_s / 1e9
         roof = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 6), traffic=None,
-                    kernel="k_simfru<double> (FRU sub-interval + fused flux reduction)",
+                    kernel=("k_simfru4" if args.lanes == 4 else "k_simfru") +
+                           ("<double>" if args.precision == "fp64" else "<float>") +
+                           " (FRU sub-interval + fused flux reduction)",
                     avg_launch_us=round(st["kernel_ms"] / st["launches"] * 1e3, 3),
                     launches=st["launches"], fru_steps=st["fru_steps"],
+                    critical_steps_per_launch=round(st["critical_steps"] / st["launches"], 2),
+                    us_per_critical_step=round(st["kernel_ms"] * 1e3 / max(st["critical_steps"], 1), 3),
+                    serial_tail_us=round(st["tail_ms"] * 1e3 / st["launches"], 3),
                     alg_bytes_per_launch=round(alg_bytes / st["launches"], 1))
     V = states[:, 0]
     line = {
@@ -175,7 +184,8 @@ def main():
                                "%s, t = %d..%d ms" % (args.nfru, "FP64 drop-in" if args.precision == "fp64"
                                                       else "mixed FP32/FP64", args.warmup,
                                                       args.warmup + args.steps),
-                   "nfru": args.nfru, "precision": args.precision,
+                   "nfru": args.nfru, "precision": args.precision, "lanes_per_fru": args.lanes,
+                   "driver": args.driver,
                    "rk_attempts": attempts, "rk_rejects": rejects,
                    "fru_steps": st["fru_steps"], "peak_V_mV": round(float(V.max()), 3),
                    "peak_Cai_uM": round(float(states[:, 6].max()) * 1e3, 4)},

@@ -48,7 +48,7 @@ class GrwConfig(C.Structure):
         ("k23_scale", C.c_double),
         ("gamma_scale", C.c_double),
         ("device", C.c_int32),
-        ("reserved", C.c_int32),
+        ("lanes_per_fru", C.c_int32),
     ]
 
 
@@ -58,6 +58,8 @@ class GrwStats(C.Structure):
         ("simfru_calls", C.c_uint64),
         ("launches", C.c_uint64),
         ("kernel_ms", C.c_double),
+        ("critical_steps", C.c_uint64),
+        ("tail_ms", C.c_double),
     ]
 
 

@@ -64,6 +64,8 @@ def _declare(L):
                                C.POINTER(C.c_double), vp, vp, vp, vp, C.c_int, C.POINTER(C.c_int),
                                C.POINTER(GrwRkStats)]
     L.grw_run_cell.restype = C.c_int
+    L.grw_run_cell_device.argtypes = L.grw_run_cell.argtypes
+    L.grw_run_cell_device.restype = C.c_int
     L._grw_cell_declared = True
 
 
@@ -129,13 +131,13 @@ class Cell:
     def __init__(self, nfru: int, seed: int = 1234, precision: str = "fp64",
                  protocol: Protocol | None = None, ic: dict | None = None, block: int = 0,
                  sequential_sums: bool = False, k23_scale: float = 1.0, gamma_scale: float = 1.0,
-                 nfru_scale: float | None = None):
+                 nfru_scale: float | None = None, lanes_per_fru: int = 4):
         self.nfru = nfru
         self.protocol = protocol or Protocol()
         scale = 12500.0 / nfru if nfru_scale is None else nfru_scale
         self.fru = FruCell(nfru, nfru_scale=scale, precision=precision, block=block,
                            sequential_sums=sequential_sums, k23_scale=k23_scale,
-                           gamma_scale=gamma_scale)
+                           gamma_scale=gamma_scale, lanes_per_fru=lanes_per_fru)
         ic = ic or load_stable(nfru, scale)
         self.state = np.array(ic["state"], np.float64)
         if self.protocol.vclamp:  # initialize_state.c:222-224
@@ -158,8 +160,16 @@ class Cell:
     def __exit__(self, *exc):
         self.close()
 
-    def run(self, nsteps: int, log_cap: int = 200000, outputs: bool = True) -> RunResult:
-        """nsteps calls of rk54pd of length time_step (caf.c:150-251)."""
+    def run(self, nsteps: int, log_cap: int = 200000, outputs: bool = True,
+            driver: str = "device") -> RunResult:
+        """nsteps calls of rk54pd of length time_step (caf.c:150-251).
+
+        driver="device": the integrator runs on the GPU (grw_run_cell_device);
+        driver="host": the host integrator calls the FRU ABI per sub-interval
+        (grw_run_cell), exactly the reference's call sequence."""
+        if driver not in ("device", "host"):
+            raise ValueError("driver must be 'device' or 'host'")
+        fn = self._L.grw_run_cell_device if driver == "device" else self._L.grw_run_cell
         p = self.protocol.to_c()
         states = np.zeros((nsteps, NGLOBAL)) if outputs else None
         cur = np.zeros((nsteps, NCUR)) if outputs else None
@@ -168,7 +178,7 @@ class Cell:
         nlog = C.c_int(0)
         h = C.c_double(self.h)
         st = GrwRkStats()
-        check(self._L.grw_run_cell(self.fru._h, C.byref(p), _vp(self.errw), float(self.t), int(nsteps),
+        check(fn(self.fru._h, C.byref(p), _vp(self.errw), float(self.t), int(nsteps),
                                    _vp(self.state), C.byref(h), _vp(states), _vp(cur), _vp(other),
                                    _vp(log) if log_cap else None, int(log_cap), C.byref(nlog),
                                    C.byref(st)))

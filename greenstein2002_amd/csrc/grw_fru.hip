@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/grw_fru.h"
+#include "grw_global.h"
 #include "grw_model.h"
 
 #pragma clang fp contract(off)
@@ -140,6 +141,10 @@ struct MtLane {
   }
 };
 
+// Micro-steps one FRU may take in one sub-interval before the launch gives up
+// with GRW_ESTATE (the fastest exit rate, ~6e5/ms, needs ~6e4 steps per 0.1 ms).
+constexpr int kStepCap = 2000000;
+
 struct SimArgs {
   double t0, t1;
   double dep0[3], depf[3];
@@ -164,21 +169,23 @@ struct Dev {
   double* out;               // [32]: 0-3 fluxes, 4-8 flux sums, 16-26 avg sums
   unsigned long long* nsteps;
   int* err;
+  unsigned int* maxsteps;           // largest FRU-step count of the current launch
+  unsigned long long* critsteps;    // sum over launches of maxsteps
+  unsigned long long* tailticks;    // wall-clock ticks spent in the serial launch tails
 };
 
 // Categorical transition draw, simfru_local.c:217-227 / :234-245: stay with
 // probability 1 - dt*k_tot, else walk the cumulative exit probabilities with '>'.
 template <class R>
 __device__ __forceinline__ int pick(double u, double dt, const Exits<R>& e) {
+  // unrolled walk (at most three exits) so the list stays in registers
   double acc = 1.0 - dt * (double)e.tot;
   if (!(u > acc)) return 0;
-  int j = 0;
   acc += dt * (double)e.r[0];
-  while (u > acc && j + 1 < e.n) {
-    ++j;
-    acc += dt * (double)e.r[j];
-  }
-  return e.d[j];
+  if (!(u > acc) || e.n == 1) return e.d[0];
+  acc += dt * (double)e.r[1];
+  if (!(u > acc) || e.n == 2) return e.d[1];
+  return e.d[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -187,7 +194,7 @@ __device__ __forceinline__ int pick(double u, double dt, const Exits<R>& e) {
 // last block to finish (ticket) sums the slots in a fixed order.  Returns true
 // in thread 0 of the last block, with the totals in tot[].
 template <int NV>
-__device__ bool reduce_grid(double v[NV], const Dev& d, int slot_off, double tot[NV]) {
+__device__ bool reduce_grid_block(double v[NV], const Dev& d, int slot_off, double* tot_sh) {
   __shared__ double wsum[4][NV];
   __shared__ bool last;
   for (int off = 32; off >= 1; off >>= 1)
@@ -202,8 +209,7 @@ __device__ bool reduce_grid(double v[NV], const Dev& d, int slot_off, double tot
     for (int k = 0; k < NV; ++k) {
       double s = wsum[0][k];
       for (int w = 1; w < nw; ++w) s += wsum[w][k];
-      __hip_atomic_store(&d.part[(size_t)blockIdx.x * 16 + slot_off + k], s, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      d.part[(size_t)blockIdx.x * 16 + slot_off + k] = s;
     }
     __threadfence();
     const unsigned t = atomicAdd(d.ticket, 1u);
@@ -212,7 +218,7 @@ __device__ bool reduce_grid(double v[NV], const Dev& d, int slot_off, double tot
   __syncthreads();
   if (!last) return false;
   __threadfence();
-  // every thread of the last block sums a strided subset of the slots
+  // every thread of the last block sums a strided subset of the slots (fixed order)
   double s[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) s[k] = 0.0;
@@ -229,13 +235,25 @@ __device__ bool reduce_grid(double v[NV], const Dev& d, int slot_off, double tot
 #pragma unroll
     for (int k = 0; k < NV; ++k) wsum[wave][k] = s[k];
   __syncthreads();
-  if (threadIdx.x != 0) return false;
-  for (int k = 0; k < NV; ++k) {
-    double a = wsum[0][k];
-    for (int w = 1; w < nw; ++w) a += wsum[w][k];
-    tot[k] = a;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NV; ++k) {
+      double a = wsum[0][k];
+      for (int w = 1; w < nw; ++w) a += wsum[w][k];
+      tot_sh[k] = a;
+    }
+    *d.ticket = 0u;  // ready for the next launch
   }
-  *d.ticket = 0u;  // ready for the next launch
+  __syncthreads();
+  return true;
+}
+
+// Thread 0 of the last block gets the totals in tot[] (returns true there only).
+template <int NV>
+__device__ bool reduce_grid(double v[NV], const Dev& d, int slot_off, double tot[NV]) {
+  __shared__ double tsh[NV];
+  if (!reduce_grid_block<NV>(v, d, slot_off, tsh)) return false;
+  if (threadIdx.x != 0) return false;
+  for (int k = 0; k < NV; ++k) tot[k] = tsh[k];
   return true;
 }
 
@@ -287,6 +305,7 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
       const R k23c = (R)a.k23c, gcf = (R)a.gcf;
       double t = a.t0;
       bool bad = false;
+      int nguard = 0;
       while (t < a.t1) {
         // ---- transition rates at the step start (fru_rates_local.c:126-777)
         const R V = (R)dV;
@@ -334,6 +353,10 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
         for (int c = 0; c < 4; ++c) cmax = rmax(cmax, conc[c + 1]);
         if (cmax > 1.5e-3) dt = rmin(dt, 5.001e-3);
         dt = rmin(dt, a.t1 - t);
+        if (!(t + dt > t) || ++nguard > kStepCap) {  // non-finite rates or runaway
+          bad = true;
+          break;
+        }
         // the step's 32 uniforms (simfru_local.c:211): loads issued here, their
         // latency hidden behind the Heun arithmetic below
         uint32_t draws[32];
@@ -420,6 +443,690 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
       flux_currents(tot, a.fdep[0], a.fdep[1], a.fdep[2], a.nfru, a.scale, o4);
       for (int k = 0; k < 4; ++k) d.out[k] = o4[k];
       for (int k = 0; k < 5; ++k) d.out[4 + k] = tot[k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Four lanes per FRU: lane c of a quad owns cleft c (its LCC, gate, five RyRs,
+// Ito2 and CaSS); CaJSR, time, step length and the MT position are kept
+// identically in all four lanes.  Cross-cleft terms (the JRyR sum, the ring
+// diffusion, the largest exit rate, the count of merged-state uniforms) move
+// through DPP quad permutes, and every sum keeps the reference's order, so the
+// arithmetic is operation-for-operation that of simfru_local.c / fcn_fru.c.
+
+// DPP quad_perm controls: broadcast lane k, next / previous lane, xor 1 / 2.
+constexpr int QB0 = 0x00, QB1 = 0x55, QB2 = 0xAA, QB3 = 0xFF;
+constexpr int QNEXT = 0x39, QPREV = 0x93, QX1 = 0xB1, QX2 = 0x4E;
+
+template <int CTRL>
+__device__ __forceinline__ int qperm(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float qperm(float v) {
+  return __int_as_float(qperm<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double qperm(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = qperm<CTRL>((int)(unsigned)(b & 0xffffffffll));
+  const int hi = qperm<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// LCC exits without per-state branches (fru_rates_local.c:176-334).  For mode
+// normal (states 1-5, k = s-1) the exits are back k*beta, forward (4-k)*alpha
+// (fL from C4) and mode switch 2^k*gamma; for mode Ca (states 7-11, k = s-7)
+// back k*beta', forward (4-k)*alpha' = 2(4-k)*alpha (fL' from CCa4) and mode
+// switch omega/bL^k; states 6 and 12 have the single exit gL / gL'.  Multiplying
+// by the exact small integers gives the reference's products bit for bit, and
+// the total keeps its summation order ((back + fwd) + mode, but
+// (back + mode) + fwd for CCa4).
+template <class R>
+__device__ __forceinline__ void lcc_arith(int s, R alpha, R beta, R bp, R gam, R& back, R& fwd,
+                                          R& mode, R& tot) {
+  const bool ca = (s >= 7) & (s <= 11);
+  const int k = ca ? s - 7 : s - 1;
+  back = (k == 0) ? R(0.0) : (R)k * (ca ? bp : beta);
+  const R fwd_n = (k < 4) ? (R)(4 - k) * alpha : R(fL);
+  const R fwd_c = (k < 4) ? (R)(2 * (4 - k)) * alpha : R(fLprime);
+  fwd = ca ? fwd_c : fwd_n;
+  const R om = (k == 0) ? R(omega) : (k == 1) ? R(omega / bL) : (k == 2) ? R(omega / bL2)
+             : (k == 3) ? R(omega / bL3) : R(omega / bL4);
+  mode = ca ? om : (R)(1 << (k & 7)) * gam;
+  tot = (ca && k == 4) ? (back + mode) + fwd : (back + fwd) + mode;
+  if (s == 6) tot = R(gL);
+  if (s == 12) tot = R(gLprime);
+}
+
+// The exit list of an LCC that leaves its state, in the reference's order.
+template <class R>
+__device__ __forceinline__ int lcc_move(int s, double u, double dt, R back, R fwd, R mode, R tot) {
+  Exits<R> e;
+  e.tot = tot;
+  if (s == 6 || s == 12) {
+    e.r[0] = tot; e.d[0] = (s == 6) ? 5 : 11; e.n = 1;
+  } else if (s == 1) {
+    e.r[0] = fwd; e.d[0] = 2; e.r[1] = mode; e.d[1] = 7; e.n = 2;
+  } else if (s <= 5) {
+    e.r[0] = back; e.d[0] = s - 1; e.r[1] = fwd; e.d[1] = s + 1; e.r[2] = mode; e.d[2] = s + 6; e.n = 3;
+  } else if (s == 7) {
+    e.r[0] = mode; e.d[0] = 1; e.r[1] = fwd; e.d[1] = 8; e.n = 2;
+  } else {
+    e.r[0] = back; e.d[0] = s - 1; e.r[1] = mode; e.d[1] = s - 6; e.r[2] = fwd; e.d[2] = s + 1; e.n = 3;
+  }
+  return pick<R>(u, dt, e);
+}
+
+// Total exit rate of a RyR in (resolved) state s: fru_rates_local.c:390-727.
+template <class R>
+__device__ __forceinline__ R ryr_total(int s, R T1, R T2, R T3, R T5, R T7, R T8) {
+  return s == 1 ? T1 : s == 2 ? T2 : s == 3 ? T3 : s == 4 ? R(k43 + k45) : s == 5 ? T5
+       : s == 6 ? R(k65) : s == 7 ? T7 : T8;
+}
+
+// MT words of one lane for absolute stream positions g0 .. g0+N-1 (only the
+// first n are produced).  Positions below 624 are read as is (a loaded full
+// generation); the others are regenerated lazily, and the first overwrite of
+// a word since the checkpoint is logged.  lazy0 is the first lazy position of
+// this micro-step and nlog0 the log fill at its start.  All loads precede all
+// stores; no word depends on another word of the same micro-step.
+template <int N>
+__device__ __forceinline__ void mt_run(uint32_t* row, uint32_t* lg, int lazy0, int nlog0, int g0,
+                                       int n, uint32_t out[N]) {
+  uint32_t cur[N], nxt[N], far[N];
+  int q[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int g = g0 + k;
+    int qq = (g >= kMtN) ? g - kMtN : g;
+    qq -= (qq >= kMtN) ? kMtN : 0;
+    q[k] = qq;
+    cur[k] = 0u; nxt[k] = 0u; far[k] = 0u;
+    if (k < n) {
+      cur[k] = row[qq];
+      if (g >= kMtN) {
+        nxt[k] = row[qq < kMtN - 1 ? qq + 1 : 0];
+        far[k] = row[qq < kMtN - 397 ? qq + 397 : qq - (kMtN - 397)];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int g = g0 + k;
+    if (k < n) {
+      if (g >= kMtN) {
+        const uint32_t y = (cur[k] & 0x80000000u) | (nxt[k] & 0x7fffffffu);
+        const uint32_t wv = far[k] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        if (nlog0 + (g - lazy0) < kMtN) lg[q[k]] = cur[k];
+        row[q[k]] = wv;
+        out[k] = MtLane::temper(wv);
+      } else {
+        out[k] = MtLane::temper(cur[k]);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ double unif(uint32_t y) { return ((double)y) / 4294967296.0; }
+
+// fcn_fru.c:23-137 for one cleft of the quad: returns d(CaJSR) (same in all
+// four lanes) and d(CaSS) of this lane's cleft.
+template <class R>
+__device__ __forceinline__ void fcn_fru4(R CaJSR, R s, R V, R Cai, R CaNSR, bool lo, int no, R& dj,
+                                         R& ds) {
+  const R JDHconstant = R(1.0) / (R(2.0) * R(VSS) * R(Faraday) * R(1000.0));
+  const R Jtr = (CaNSR - CaJSR) / R(tautr);
+  const R JRyR = R(JRyRmax) * ((R)no) * (CaJSR - s);
+  const R JRyRtot = qperm<QB0>(JRyR) + qperm<QB1>(JRyR) + qperm<QB2>(JRyR) + qperm<QB3>(JRyR);
+  const R Jxfer = (s - Cai) / R(tauxfer);
+  const R Jss = (s - qperm<QNEXT>(s)) / R(tauss2ss);
+  const R Jss_prev = qperm<QPREV>(Jss);
+  const R VF_over_RT = V / R(RT_over_F);
+  const R VFsq_over_RT = (R(1000.0) * R(Faraday)) * VF_over_RT;
+  const R e = xexp(R(2.0) * VF_over_RT);
+  const R open = lo ? R(1.0) : R(0.0);
+  R JDHPR;
+  if (fabs(V) < R(1.e-6)) {
+    JDHPR = -R(PCa) * R(2.0) * R(1000.0) * R(Faraday) * (s - R(Cao) * R(0.341)) * open * JDHconstant;
+  } else {
+    const R a2 = R(PCa) * R(4.0) * VFsq_over_RT / (e - R(1.0)) * JDHconstant;
+    JDHPR = -(s * e - R(Cao) * R(0.341)) * a2 * open;
+  }
+  const R a1s = R(BSRtot) * R(KBSR) / ((s + R(KBSR)) * (s + R(KBSR)));
+  const R a2s = R(BSLtot) * R(KBSL) / ((s + R(KBSL)) * (s + R(KBSL)));
+  const R bSS = R(1.0) / (R(1.0) + a1s + a2s);
+  const R a1j = R(CSQNtot) * R(KmCSQN) / ((CaJSR + R(KmCSQN)) * (CaJSR + R(KmCSQN)));
+  const R bJSR = R(1.0) / (R(1.0) + a1j);
+  dj = bJSR * (Jtr - R(VSS) / R(VJSR) * JRyRtot);
+  ds = bSS * (JDHPR + JRyR - Jxfer - Jss + Jss_prev);
+}
+
+// One sub-interval of every FRU with four lanes per FRU; returns this lane's
+// FRU-step count (nonzero only in lane 0 of a quad) and its flux terms in acc.
+template <class R>
+__device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[5]) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = gid >> 2;
+  const int c = gid & 3;
+  unsigned steps = 0;
+  if (f < a.nfru) {  // quads are active or inactive as a whole
+    double cj = d.conc[f];
+    double css = d.conc[(c + 1) * a.ld + f];
+    uint32_t w = d.chan[c * a.ld + f];
+    uint32_t* const row = d.mt + (size_t)f * kMtStride;
+    uint32_t* const lg = d.mtlog + (size_t)f * kMtStride;
+    int idx = d.mtidx[f];
+    int nlog = d.nlog[f];
+    const double span0 = a.depf[0] - a.dep0[0];
+    const double span1 = a.depf[1] - a.dep0[1];
+    const double span2 = a.depf[2] - a.dep0[2];
+    const double tint = a.t1 - a.t0;
+    double dV = a.dep0[0], dCai = a.dep0[1], dNSR = a.dep0[2];
+    bool lopen = ch_lcc_open(w);
+    int nopen = ch_ryr_open(w);
+    const R k23c = (R)a.k23c, gcf = (R)a.gcf;
+    double t = a.t0;
+    int bad = 0;
+    int nguard = 0;
+    while (t < a.t1) {
+      // ---- rates at the step start (fru_rates_local.c:126-777)
+      const R V = (R)dV;
+      const R vm35 = V - R(35.0);
+      const R e0 = R(0.012) * vm35, e1 = R(-0.05) * vm35;
+      const R e2 = (V + R(12.5)) / R(5.0), e3 = (V + R(30.0)) / R(12.0);
+      const R ex = xexp(c == 0 ? e0 : c == 1 ? e1 : c == 2 ? e2 : e3);  // one exp per lane
+      const R alpha = R(alphacf) * qperm<QB0>(ex);
+      const R beta = R(betacf) * qperm<QB1>(ex);
+      const R yinf = R(yCa_frac) / (R(1.0) + qperm<QB2>(ex)) + (R(1.0) - R(yCa_frac));
+      const R tau = R(60.0) + R(340.0) / (R(1.0) + qperm<QB3>(ex));
+      const R bp = beta / R(bL * 1.0);
+      const R cc = (R)css;
+      const int ls = ch_lcc(w);
+      bad |= (ls < 1 || ls > 12);
+      R lb, lf, lm, lt;
+      lcc_arith<R>(ls, alpha, beta, bp, gcf * cc, lb, lf, lm, lt);
+      const R vr = (ch_y(w) == 2 ? (R(1.0) - yinf) : yinf) / tau;
+      const R ir = (ch_ito2(w) == 1) ? R(kfIto2) * cc : R(kbIto2);
+      R maxr = rmax(rmax(rmax(R(0.0), lt), vr), ir);
+      const RyrK<R> k = ryr_k(cc, k23c);
+      int st[kNryr];
+      int nd = 0;
+#pragma unroll
+      for (int r = 0; r < kNryr; ++r) {
+        st[r] = ryr_collapse(ch_ryr(w, r), k.regime);
+        nd += ryr_needs_draw(st[r], k.regime) ? 1 : 0;
+      }
+      // merged-state uniforms come first, cleft by cleft (fru_rates_local.c:452,476,581)
+      const int n0 = qperm<QB0>(nd), n1 = qperm<QB1>(nd), n2 = qperm<QB2>(nd), n3 = qperm<QB3>(nd);
+      const int E = n0 + n1 + n2 + n3;
+      const int lazy0 = idx > kMtN ? idx : kMtN;
+      if (nd) {
+        const int Ec = (c > 0 ? n0 : 0) + (c > 1 ? n1 : 0) + (c > 2 ? n2 : 0);
+        uint32_t vw[kNryr];
+        mt_run<kNryr>(row, lg, lazy0, nlog, idx + Ec, nd, vw);
+#pragma unroll
+        for (int r = 0; r < kNryr; ++r) {
+          if (ryr_needs_draw(st[r], k.regime)) {
+            st[r] = ryr_split(st[r], unif(vw[0]), k);
+#pragma unroll
+            for (int j = 0; j < kNryr - 1; ++j) vw[j] = vw[j + 1];
+          }
+        }
+      }
+      const R T2 = (R(k21) + k.k23 + k.k25);
+      const R T3 = (R(k32) + k.k34);
+      const R T5 = (R(k52) + k.k54 + k.k56);
+      const R a7 = k.k34 / (R(k43) + k.k34);
+      const R T7 = ((R(1.0) - a7) * R(k32)) + (a7 * R(k45));
+      const R a8 = R(k65) / (k.k56 + R(k65));
+      const R T8 = (a8 * R(k52)) + (a8 * k.k54);
+      R tr[kNryr];
+#pragma unroll
+      for (int r = 0; r < kNryr; ++r) {
+        w = ch_set_ryr(w, r, st[r]);
+        tr[r] = ryr_total<R>(st[r], k.k12, T2, T3, T5, T7, T8);
+        maxr = rmax(maxr, tr[r]);
+      }
+      maxr = rmax(maxr, qperm<QX1>(maxr));
+      maxr = rmax(maxr, qperm<QX2>(maxr));
+      bad |= qperm<QX1>(bad);
+      bad |= qperm<QX2>(bad);
+      if (bad) break;
+      // ---- step length (simfru_local.c:135-146)
+      double dt = 0.1 / (double)maxr;
+      dt = rmin(dt, 10.001e-3);
+      double cm = rmax(css, qperm<QX1>(css));
+      cm = rmax(cm, qperm<QX2>(cm));
+      const double cmax = rmax(0.e-3, cm);
+      if (cmax > 1.5e-3) dt = rmin(dt, 5.001e-3);
+      dt = rmin(dt, a.t1 - t);
+      if (!(t + dt > t) || ++nguard > kStepCap) {  // non-finite rates or runaway (quad-uniform)
+        bad = 1;
+        break;
+      }
+      // ---- this cleft's 8 of the step's 32 uniforms (simfru_local.c:211)
+      uint32_t dw[8];
+      mt_run<8>(row, lg, lazy0, nlog, idx + E + 8 * c, 8, dw);
+      {
+        const int idn = idx + E + 32;
+        const int adv = idn - lazy0;
+        nlog = min(kMtN, nlog + (adv > 0 ? adv : 0));
+        idx = (idn >= 2 * kMtN) ? idn - kMtN : idn;
+      }
+      // ---- Heun step (simfru_local.c:149-173)
+      R djs, dss;
+      fcn_fru4<R>((R)cj, cc, (R)dV, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
+      const double k1j = dt * (double)djs, k1s = dt * (double)dss;
+      const double y1j = cj + k1j, y1s = css + k1s;
+      t = t + dt;
+      dV = a.dep0[0] + (t - a.t0) / tint * span0;
+      dCai = a.dep0[1] + (t - a.t0) / tint * span1;
+      dNSR = a.dep0[2] + (t - a.t0) / tint * span2;
+      fcn_fru4<R>((R)y1j, (R)y1s, (R)dV, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
+      cj = cj + (k1j + dt * (double)djs) / 2.0;
+      css = css + (k1s + dt * (double)dss) / 2.0;
+      // ---- transitions of this cleft (simfru_local.c:215-276)
+      if (unif(dw[0]) > 1.0 - dt * (double)lt) w = ch_set_lcc(w, lcc_move<R>(ls, unif(dw[0]), dt, lb, lf, lm, lt));
+      if (unif(dw[1]) >= 1.0 - dt * (double)vr) w = ch_flip_y(w);
+      bool moved = false;
+#pragma unroll
+      for (int r = 0; r < kNryr; ++r) {
+        const double u = unif(dw[2 + r]);
+        if (u > 1.0 - dt * (double)tr[r]) {
+          Exits<R> e;
+          ryr_exits(st[r], k, e);
+          w = ch_set_ryr(w, r, pick<R>(u, dt, e));
+          moved = true;
+        }
+      }
+      if (unif(dw[7]) >= 1.0 - dt * (double)ir) w = ch_flip_ito2(w);
+      lopen = ch_lcc_open(w);
+      if (moved) nopen = ch_ryr_open(w);
+      steps += (c == 0);
+    }
+    if (bad && c == 0) atomicOr(d.err, 1);
+    d.conc[(c + 1) * a.ld + f] = css;
+    d.chan[c * a.ld + f] = w;
+    if (c == 0) {
+      d.conc[f] = cj;
+      d.mtidx[f] = idx;
+      d.nlog[f] = nlog;
+    }
+    acc[0] = css;
+    acc[1] = (c == 0) ? cj : 0.0;
+    acc[2] = lopen ? css : 0.0;
+    acc[3] = lopen ? 1.0 : 0.0;
+    acc[4] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
+  }
+  return steps;
+}
+
+__device__ __forceinline__ void count_steps(const Dev& d, unsigned steps) {
+  unsigned long long s = steps;
+  unsigned m = steps;
+  for (int off = 32; off >= 1; off >>= 1) {
+    s += __shfl_xor(s, off);
+    m = max(m, (unsigned)__shfl_xor((int)m, off));
+  }
+  if ((threadIdx.x & 63) == 0 && s) {
+    atomicAdd(d.nsteps, s);
+    atomicMax(d.maxsteps, m);
+  }
+}
+
+// After the grid's last block has seen every block's atomics: fold the
+// launch's largest FRU-step count into the running sum.
+__device__ __forceinline__ void close_launch_stats(const Dev& d) {
+  const unsigned m = atomicExch(d.maxsteps, 0u);
+  atomicAdd(d.critsteps, (unsigned long long)m);
+}
+
+template <class R, bool REDUCE>
+__global__ void __launch_bounds__(256) k_simfru4(const SimArgs a, const Dev d) {
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  count_steps(d, simfru4_body<R>(a, d, acc));
+  if (REDUCE) {
+    double tot[5];
+    if (reduce_grid<5>(acc, d, 0, tot)) {
+      close_launch_stats(d);
+      double o4[4];
+      flux_currents(tot, a.fdep[0], a.fdep[1], a.fdep[2], a.nfru, a.scale, o4);
+      for (int q = 0; q < 4; ++q) d.out[q] = o4[q];
+      for (int q = 0; q < 5; ++q) d.out[4 + q] = tot[q];
+    }
+  }
+}
+
+// ===========================================================================
+// The global integrator on the device.
+//
+// rk_pd54m.c:29-345 and the caf.c:150-251 output loop run in the tails of the
+// Monte Carlo launches: the last block of every sub-interval kernel (after the
+// flux reduction) evaluates fcn for that RK stage and prepares the next
+// sub-interval, and the last stage decides accept / reject.  The host only
+// enqueues attempts (one begin kernel and five stage kernels each) and
+// synchronises once per batch; launches after the run is complete return at
+// once.  One RkCtl per cell holds the integrator's state.
+struct RkCtl {
+  GrwProtocol prot;
+  double errw[kNglobal];
+  double state[kNglobal];
+  double y[kNglobal];
+  double F1[kNglobal];
+  double k[6][kNglobal];
+  double cur[kNcur];
+  double J[4];
+  double dep0[3], depf[3];
+  double start, h, oh, tcall, tstep, ta, tb;
+  int stage, success, keepc, resume;
+  int active, ms_done, ms_target, avg_pending, avg_row;
+  int log_n, log_cap, err;
+  long long attempts, rejects, forced;
+  double* out_states;  // [ms_target][37]
+  double* out_cur;     // [ms_target][24]
+  double* out_other;   // [ms_target][12]
+  double* out_log;     // [log_cap][4]
+};
+
+// Stage s (1..5) sub-interval bounds, rk_pd54m.c:124,145-146,166-167,187-188,208-209.
+__device__ __forceinline__ void stage_bounds(int s, double start, double h, double& ta, double& tb) {
+  switch (s) {
+    case 1: ta = start; tb = start + h / 5.0; break;
+    case 2: ta = start + h / 5.0; tb = start + h * 3.0 / 10.0; break;
+    case 3: ta = start + h * 3.0 / 10.0; tb = start + h * 3.0 / 5.0; break;
+    case 4: ta = start + h * 3.0 / 5.0; tb = start + h * 2.0 / 3.0; break;
+    default: ta = start + h * 2.0 / 3.0; tb = start + h; break;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_dev_begin(const SimArgs a, const Dev d, RkCtl* ctl) {
+  const int active = ctl->active, avgp = ctl->avg_pending;
+  if (!active && !avgp) return;
+  const int resume = ctl->resume, success = ctl->success;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = 0.0;
+  if (i < a.nfru) {
+    if (active && resume) {  // mpi_slave.c:150-184
+      for (int q = 0; q < 5; ++q) d.conc[q * a.ld + i] = d.conc_h[q * a.ld + i];
+      for (int q = 0; q < 4; ++q) d.chan[q * a.ld + i] = d.chan_h[q * a.ld + i];
+      const int idx0 = d.mtidx_h[i];
+      const int p0 = idx0 >= kMtN ? idx0 - kMtN : 0;
+      const int n = min(d.nlog[i], kMtN);
+      uint32_t* row = d.mt + (size_t)i * kMtStride;
+      const uint32_t* lg = d.mtlog + (size_t)i * kMtStride;
+      for (int q = 0; q < n; ++q) {
+        int p = p0 + q;
+        if (p >= kMtN) p -= kMtN;
+        row[p] = lg[p];
+      }
+      d.mtidx[i] = idx0;
+      d.nlog[i] = 0;
+    } else if (active && success) {  // mpi_slave.c:192-226
+      for (int q = 0; q < 5; ++q) d.conc_h[q * a.ld + i] = d.conc[q * a.ld + i];
+      for (int q = 0; q < 4; ++q) d.chan_h[q * a.ld + i] = d.chan[q * a.ld + i];
+      d.mtidx_h[i] = d.mtidx[i];
+      d.nlog[i] = 0;
+    }
+    double conc[5];
+    uint32_t ch[4];
+    for (int q = 0; q < 5; ++q) conc[q] = d.conc[q * a.ld + i];
+    for (int q = 0; q < 4; ++q) ch[q] = d.chan[q * a.ld + i];
+    flux_terms(conc, ch, v);
+    if (avgp) {  // calc_fru_avg_local.c:135-176
+      const double cj = conc[0];
+      v[6] = cj;
+      v[11] = cj * (1.0 + CSQNtot / (KmCSQN + cj));
+      for (int q = 0; q < 4; ++q) {
+        const double s = conc[q + 1];
+        const uint32_t w = ch[q];
+        v[5] += s;
+        v[10] += s * (1.0 + BSRtot / (KBSR + s) + BSLtot / (KBSL + s));
+        int no = 0, nr = 0;
+        for (int r = 0; r < kNryr; ++r) {
+          const int st = ch_ryr(w, r);
+          no += (st == 3 || st == 4 || st == 7);
+          nr += (st == 1 || st == 2);
+        }
+        v[7] += JRyRmax * ((double)no) * (cj - s);
+        v[8] += (double)no;
+        v[9] += (double)nr;
+        v[12] += (ch_lcc(w) <= 6) ? 1.0 : 0.0;
+        v[13] += (ch_y(w) == 2) ? 1.0 : 0.0;
+        v[14] += ch_lcc_open(w) ? 1.0 : 0.0;
+        v[15] += (ch_ito2(w) == 2) ? 1.0 : 0.0;
+      }
+    }
+  }
+  __shared__ double tot[16];
+  if (!reduce_grid_block<16>(v, d, 0, tot)) return;
+  // ---- last block: outputs of the finished call, fcn at the attempt start
+  __shared__ double F1[kNglobal];
+  __shared__ double st[kNglobal];
+  const int j = threadIdx.x;
+  if (j < kNglobal) st[j] = ctl->state[j];
+  __syncthreads();
+  if (j == 0) {
+    if (avgp) {
+      double other[12];
+      avg_outputs(tot + 5, st, a.nfru, a.scale, other);
+      for (int q = 0; q < 12; ++q) ctl->out_other[(size_t)ctl->avg_row * 12 + q] = other[q];
+      ctl->avg_pending = 0;
+    }
+    if (active) {
+      if (resume) {
+        ctl->resume = 0;
+      } else if (success) {  // rk_pd54m.c:93-108
+        double J[4];
+        flux_currents(tot, st[0], st[6], st[7], a.nfru, a.scale, J);
+        double cur[kNcur];
+        fcn(ctl->start, st, F1, cur, ctl->keepc != 0, J, ctl->prot);
+        if (ctl->keepc)
+          for (int q = 0; q < kNcur; ++q) ctl->cur[q] = cur[q];
+        ctl->keepc = 0;
+        for (int q = 0; q < 4; ++q) ctl->J[q] = J[q];
+      }
+      ctl->attempts++;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const double h = ctl->h, start = ctl->start;
+  if (j < kNglobal) {
+    if (success && !resume) {
+      ctl->F1[j] = F1[j];
+      ctl->state[j] = st[j];  // fcn may have set mNa / V (fcn.c:663-669, :811-814)
+    } else {
+      F1[j] = ctl->F1[j];
+    }
+    const double k0 = h * F1[j];
+    ctl->k[0][j] = k0;
+    ctl->y[j] = st[j] + k0 / 5.0;
+  }
+  __syncthreads();
+  if (j == 0) {
+    ctl->dep0[0] = st[0]; ctl->dep0[1] = st[6]; ctl->dep0[2] = st[7];
+    ctl->depf[0] = ctl->y[0]; ctl->depf[1] = ctl->y[6]; ctl->depf[2] = ctl->y[7];
+    double ta, tb;
+    stage_bounds(1, start, h, ta, tb);
+    ctl->ta = ta;
+    ctl->tb = tb;
+    ctl->stage = 1;
+  }
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctl) {
+  if (!ctl->active) return;
+  SimArgs a = base;
+  a.t0 = ctl->ta;
+  a.t1 = ctl->tb;
+  for (int q = 0; q < 3; ++q) {
+    a.dep0[q] = ctl->dep0[q];
+    a.depf[q] = ctl->depf[q];
+  }
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  count_steps(d, simfru4_body<R>(a, d, acc));
+  __shared__ double tot[5];
+  if (!reduce_grid_block<5>(acc, d, 0, tot)) return;
+  const unsigned long long tail0 = wall_clock64();
+  if (threadIdx.x == 0) close_launch_stats(d);
+  // ---- last block: fcn of this stage and the next stage / the step decision
+  __shared__ double st[kNglobal], yv[kNglobal], F[kNglobal], kk[6][kNglobal], ev[kNglobal];
+  __shared__ int decision;
+  const int j = threadIdx.x;
+  const int s = ctl->stage;
+  if (s < 1 || s > 5) {  // launches out of sequence: stop the run
+    if (j == 0) {
+      ctl->err = 1;
+      ctl->active = 0;
+    }
+    return;
+  }
+  const double h = ctl->h, start = ctl->start;
+  if (j < kNglobal) {
+    st[j] = ctl->state[j];
+    yv[j] = ctl->y[j];
+    for (int q = 0; q < s; ++q) kk[q][j] = ctl->k[q][j];
+  }
+  __syncthreads();
+  if (j == 0) {
+    double J[4];
+    flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], a.nfru, a.scale, J);  // mpi_master.c:401-468
+    fcn(a.t1, yv, F, nullptr, false, J, ctl->prot);
+    for (int q = 0; q < 4; ++q) ctl->J[q] = J[q];
+  }
+  __syncthreads();
+  if (j < kNglobal) {
+    kk[s][j] = h * F[j];
+    ctl->k[s][j] = kk[s][j];
+  }
+  if (s < 5) {  // next stage's y (rk_pd54m.c:138,159,180,201)
+    if (j < kNglobal) {
+      double y;
+      if (s == 1) y = st[j] + (3.0 / 40.0 * kk[0][j] + 9.0 / 40.0 * kk[1][j]);
+      else if (s == 2) y = st[j] + (3.0 / 10.0 * kk[0][j] - 9.0 / 10.0 * kk[1][j] + 6.0 / 5.0 * kk[2][j]);
+      else if (s == 3)
+        y = st[j] + 226.0 / 729.0 * kk[0][j] - 25.0 / 27.0 * kk[1][j] + 880.0 / 729.0 * kk[2][j] +
+            55.0 / 729.0 * kk[3][j];
+      else
+        y = st[j] - 181.0 / 270.0 * kk[0][j] + 5.0 / 2.0 * kk[1][j] - 266.0 / 297.0 * kk[2][j] -
+            91.0 / 27.0 * kk[3][j] + 189.0 / 55.0 * kk[4][j];
+      yv[j] = y;
+      ctl->y[j] = y;
+    }
+    __syncthreads();
+    if (j == 0) {
+      ctl->dep0[0] = st[0]; ctl->dep0[1] = st[6]; ctl->dep0[2] = st[7];
+      ctl->depf[0] = yv[0]; ctl->depf[1] = yv[6]; ctl->depf[2] = yv[7];
+      double ta, tb;
+      stage_bounds(s + 1, start, h, ta, tb);
+      ctl->ta = ta;
+      ctl->tb = tb;
+      ctl->stage = s + 1;
+      atomicAdd(d.tailticks, wall_clock64() - tail0);
+    }
+    return;
+  }
+  // ---- truncation error and step control (rk_pd54m.c:225-333)
+  const double huge_number = 1e99;
+  if (j < kNglobal) {
+    const double y4 = st[j] + 31.0 / 540.0 * kk[0][j] + 190.0 / 297.0 * kk[2][j] - 145.0 / 108.0 * kk[3][j] +
+                      351.0 / 220.0 * kk[4][j] + 1.0 / 20.0 * kk[5][j];
+    const double ym = st[j] + 19.0 / 216.0 * kk[0][j] + 1000.0 / 2079.0 * kk[2][j] -
+                      125.0 / 216.0 * kk[3][j] + 81.0 / 88.0 * kk[4][j] + 5.0 / 56.0 * kk[5][j];
+    ev[j] = fabs((ym - y4) * 0.2 * ctl->errw[j]);
+    yv[j] = y4;
+  }
+  __syncthreads();
+  if (j == 0) {
+    const GrwProtocol& p = ctl->prot;
+    double err = 0.0;
+    for (int q = 0; q < kNglobal; ++q) {
+      if (ev[q] < huge_number) err = rmax(ev[q], err);
+      else err = huge_number;
+    }
+    const bool accept = err < p.tolrk;
+    double hn = h;
+    if (accept || h <= p.step_min) {
+      decision = 1;
+      const double ns = start + h;
+      ctl->start = ns;
+      ctl->success = 1;
+      if (!accept) ctl->forced++;
+      if (ns >= ctl->tstep) {
+        decision = 2;  // this rk54pd call is complete
+      } else if (accept) {
+        hn = rmin(0.85 * h * pow(p.tolrk / err, 0.2), 4.0 * h);
+        ctl->oh = hn;
+        double tts = p.shift - fmod(ns, p.period);
+        if (tts < 0.0) tts = tts + p.period;
+        if (tts < hn) hn = rmax(p.step_min, tts - 0.5 * p.step_min);
+      } else {
+        hn = p.step_min;
+      }
+      if (!accept) {
+        ctl->oh = hn;
+        hn = rmin(p.step_max, rmax(p.step_min, hn));
+        hn = rmin(hn, ctl->tstep - ns);
+      }
+    } else {
+      decision = 0;
+      ctl->rejects++;
+      hn = 0.85 * h * pow(p.tolrk / err, 0.2);
+      double tts = p.shift - fmod(start, p.period);
+      if (tts < 0.0) tts = tts + p.period;
+      if (tts < hn) hn = rmax(p.step_min, tts - 0.5 * p.step_min);
+      ctl->success = 0;
+      ctl->resume = 1;
+      hn = rmin(p.step_max, rmax(p.step_min, hn));
+      hn = rmin(hn, ctl->tstep - start);
+    }
+    ctl->h = hn;
+    ctl->stage = 0;
+    atomicAdd(d.tailticks, wall_clock64() - tail0);
+  }
+  __syncthreads();
+  if (decision == 0) return;
+  if (j < kNglobal) {
+    double y = yv[j];
+    if (fabs(y) < 1.e-10) y = 0.0;
+    yv[j] = y;
+    ctl->state[j] = y;
+  }
+  __syncthreads();
+  if (j == 0 && ctl->log_n < ctl->log_cap) {
+    double* L = ctl->out_log + 4 * (size_t)ctl->log_n;
+    L[0] = ctl->start; L[1] = yv[0]; L[2] = yv[6]; L[3] = ctl->J[2];
+    ctl->log_n++;
+  }
+  if (decision != 2) return;
+  // ---- the caf.c loop: record this call's outputs, set up the next call
+  const int row = ctl->ms_done;
+  if (j < kNglobal) ctl->out_states[(size_t)row * kNglobal + j] = yv[j];
+  if (j < kNcur) ctl->out_cur[(size_t)row * kNcur + j] = ctl->cur[j];
+  __syncthreads();
+  if (j == 0) {
+    const GrwProtocol& p = ctl->prot;
+    ctl->ms_done = row + 1;
+    ctl->avg_pending = 1;
+    ctl->avg_row = row;
+    if (row + 1 >= ctl->ms_target) {
+      ctl->active = 0;
+    } else {
+      const double tc = ctl->tstep;  // the next call starts on the output grid (caf.c:150)
+      ctl->tcall = tc;
+      ctl->tstep = tc + p.time_step;
+      double hn = rmin(ctl->oh, ctl->tstep - tc);
+      hn = rmin(p.step_max, hn);
+      ctl->h = hn;
+      ctl->oh = hn;
+      ctl->start = tc;
+      ctl->success = 1;
+      ctl->keepc = 1;
     }
   }
 }
@@ -583,12 +1290,14 @@ int fail(int code, const std::string& msg) {
 struct grw_ctx {
   grw_config cfg;
   int n = 0, ld = 0, block = 64, grid = 0;
+  int grid_sim = 0;  // grid of the Monte Carlo kernel (4 lanes per FRU: 4x the FRU grid)
   Dev d{};
   hipStream_t stream = nullptr;
   double* host_out = nullptr;  // pinned [32]
   grw_stats stats{};
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> evpool;  // per-launch timing of device-driven runs
   std::vector<void*> allocs;
 };
 
@@ -618,14 +1327,17 @@ int check_err(grw_ctx* c) {
   int e = 0;
   HIPCHK(hipMemcpyAsync(&e, c->d.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (e) return fail(GRW_ESTATE, "kernel met an unknown channel state");
+  if (e) return fail(GRW_ESTATE, "kernel met an unknown channel state, a non-finite rate or a runaway step count");
   return 0;
 }
 
 template <class R, bool SIM, bool REDUCE>
 int launch_sim(grw_ctx* c, const SimArgs& a) {
   if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL((k_simfru<R, SIM, REDUCE>), dim3(c->grid), dim3(c->block), 0, c->stream, a, c->d);
+  if (SIM && c->cfg.lanes_per_fru == 4)
+    hipLaunchKernelGGL((k_simfru4<R, REDUCE>), dim3(c->grid_sim), dim3(c->block), 0, c->stream, a, c->d);
+  else
+    hipLaunchKernelGGL((k_simfru<R, SIM, REDUCE>), dim3(c->grid), dim3(c->block), 0, c->stream, a, c->d);
   HIPCHK(hipGetLastError());
   if (c->timing) {
     HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -700,6 +1412,7 @@ void grw_config_default(grw_config* cfg) {
   cfg->k23_scale = 1.0;
   cfg->gamma_scale = 1.0;
   cfg->device = 0;
+  cfg->lanes_per_fru = 4;
 }
 
 int grw_create(const grw_config* cfg, grw_ctx** out) {
@@ -708,8 +1421,10 @@ int grw_create(const grw_config* cfg, grw_ctx** out) {
   if (cfg->nfru <= 0 || cfg->nfru > (1 << 26)) return fail(GRW_EINVAL, "nfru out of range");
   if (cfg->precision != GRW_FP64 && cfg->precision != GRW_MIXED)
     return fail(GRW_EINVAL, "precision must be GRW_FP64 or GRW_MIXED");
-  const int blk = cfg->block ? cfg->block : 64;
+  const int blk = cfg->block ? cfg->block : (cfg->lanes_per_fru == 4 ? 256 : 64);
   if (blk % 64 || blk < 64 || blk > 256) return fail(GRW_EINVAL, "block must be 64, 128, 192 or 256");
+  if (cfg->lanes_per_fru != 1 && cfg->lanes_per_fru != 4)
+    return fail(GRW_EINVAL, "lanes_per_fru must be 1 or 4");
   HIPCHK(hipSetDevice(cfg->device));
   grw_ctx* c = new grw_ctx();
   c->cfg = *cfg;
@@ -717,15 +1432,18 @@ int grw_create(const grw_config* cfg, grw_ctx** out) {
   c->ld = (cfg->nfru + 63) / 64 * 64;
   c->block = blk;
   c->grid = (c->n + blk - 1) / blk;
+  c->grid_sim = cfg->lanes_per_fru == 4 ? (4 * c->n + blk - 1) / blk : c->grid;
   int rc = 0;
   const size_t ld = c->ld, n = c->n;
   if ((rc = dalloc(c, &c->d.conc, 5 * ld)) || (rc = dalloc(c, &c->d.chan, 4 * ld)) ||
       (rc = dalloc(c, &c->d.mtidx, ld)) || (rc = dalloc(c, &c->d.nlog, ld)) ||
       (rc = dalloc(c, &c->d.mt, n * kMtStride)) || (rc = dalloc(c, &c->d.mtlog, n * kMtStride)) ||
       (rc = dalloc(c, &c->d.conc_h, 5 * ld)) || (rc = dalloc(c, &c->d.chan_h, 4 * ld)) ||
-      (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, (size_t)c->grid * 16)) ||
+      (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, (size_t)c->grid_sim * 16)) ||
       (rc = dalloc(c, &c->d.ticket, 1)) || (rc = dalloc(c, &c->d.out, 32)) ||
-      (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1))) {
+      (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
+      (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
+      (rc = dalloc(c, &c->d.tailticks, 1))) {
     grw_destroy(c);
     return rc;
   }
@@ -748,6 +1466,7 @@ int grw_destroy(grw_ctx* c) {
   if (c->host_out) (void)hipHostFree(c->host_out);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t ev : c->evpool) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -933,17 +1652,26 @@ int grw_resume_state(grw_ctx* c) {
 
 int grw_get_stats(grw_ctx* c, grw_stats* st) {
   if (!c || !st) return fail(GRW_EINVAL, "null argument");
-  unsigned long long s = 0;
+  unsigned long long s = 0, cs = 0, tt = 0;
   HIPCHK(hipMemcpyAsync(&s, c->d.nsteps, sizeof s, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&cs, c->d.critsteps, sizeof cs, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&tt, c->d.tailticks, sizeof tt, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->cfg.device) != hipSuccess || khz <= 0)
+    khz = 100000;
   *st = c->stats;
   st->fru_steps = s;
+  st->critical_steps = cs;
+  st->tail_ms = (double)tt / (double)khz;
   return 0;
 }
 
 int grw_reset_stats(grw_ctx* c) {
   if (!c) return fail(GRW_EINVAL, "null context");
   HIPCHK(hipMemsetAsync(c->d.nsteps, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMemsetAsync(c->d.critsteps, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMemsetAsync(c->d.tailticks, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->stats = grw_stats{};
   return 0;
@@ -955,4 +1683,168 @@ int grw_set_timing(grw_ctx* c, int on) {
   return 0;
 }
 
+// ---- device-resident cell run (include/grw_cell.h)
+int grw_run_cell_device(grw_ctx* c, const GrwProtocol* p, const double errw[37], double t0,
+                        int nsteps, double state[37], double* h, double* states, double* currents,
+                        double* other, double* log, int log_cap, int* log_n, GrwRkStats* stt) {
+  if (!c || !p || !errw || !state || !h || nsteps < 0 || log_cap < 0)
+    return fail(GRW_EINVAL, "bad argument to grw_run_cell_device");
+  if (nsteps == 0) return 0;
+  if (!std::isfinite(t0) || !(p->time_step > 0.0) || !(p->step_min > 0.0) || !(p->step_max >= p->step_min))
+    return fail(GRW_EINVAL, "bad protocol time controls");
+  const size_t ns = (size_t)nsteps, lc = (size_t)std::max(log_cap, 1);
+  RkCtl* dctl = nullptr;
+  double* dbuf = nullptr;
+  HIPCHK(hipMalloc((void**)&dctl, sizeof(RkCtl)));
+  hipError_t e0 = hipMalloc((void**)&dbuf, sizeof(double) * (ns * (kNglobal + kNcur + 12) + lc * 4));
+  if (e0 != hipSuccess) {
+    (void)hipFree(dctl);
+    return fail(GRW_EDEVICE, std::string("hipMalloc failed: ") + hipGetErrorString(e0));
+  }
+  RkCtl hc;
+  std::memset(&hc, 0, sizeof hc);
+  hc.prot = *p;
+  for (int q = 0; q < kNglobal; ++q) {
+    hc.errw[q] = errw[q];
+    hc.state[q] = state[q];
+  }
+  hc.tcall = t0;
+  hc.start = t0;
+  hc.tstep = t0 + p->time_step;
+  double h0 = (*h < hc.tstep - t0) ? *h : hc.tstep - t0;  // rk_pd54m.c:71-74
+  h0 = (p->step_max < h0) ? p->step_max : h0;
+  hc.h = h0;
+  hc.oh = h0;
+  hc.success = 1;
+  hc.keepc = 1;
+  hc.active = 1;
+  hc.ms_target = nsteps;
+  hc.log_cap = log ? log_cap : 0;
+  hc.out_states = dbuf;
+  hc.out_cur = dbuf + ns * kNglobal;
+  hc.out_other = hc.out_cur + ns * kNcur;
+  hc.out_log = hc.out_other + ns * 12;
+  int rc = 0;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(dctl);
+    (void)hipFree(dbuf);
+  };
+  hipError_t e = hipMemcpyAsync(dctl, &hc, sizeof hc, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(GRW_EDEVICE, std::string("ctl upload failed: ") + hipGetErrorString(e));
+  }
+  const SimArgs base = base_args(c);
+  const int blk = 256;
+  const int grid_b = (c->n + blk - 1) / blk;
+  const int grid_s = (4 * c->n + blk - 1) / blk;
+  const bool mixed = c->cfg.precision == GRW_MIXED;
+  std::vector<hipEvent_t>& evs = c->evpool;
+  long long att_prev = 0;
+  double ms_per_attempt = 1.0 / 40.0;
+  int ms_prev = 0;
+  while (true) {
+    const int remaining = nsteps - ms_prev;
+    const int chunk = std::min(remaining, 25);
+    int K = (int)std::ceil(chunk / ms_per_attempt * 1.1) + 2;
+    K = std::max(4, std::min(K, 4000));
+    if (c->timing) {
+      while ((int)evs.size() < 10 * K) {
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) break;
+        evs.push_back(ev);
+      }
+    }
+    const bool timed = c->timing && (int)evs.size() >= 10 * K;
+    for (int k = 0; k < K; ++k) {
+      hipLaunchKernelGGL(k_dev_begin, dim3(grid_b), dim3(blk), 0, c->stream, base, c->d, dctl);
+      for (int s5 = 0; s5 < 5; ++s5) {
+        if (timed) (void)hipEventRecord(evs[10 * k + 2 * s5], c->stream);
+        if (mixed)
+          hipLaunchKernelGGL(k_dev_stage<float>, dim3(grid_s), dim3(blk), 0, c->stream, base, c->d, dctl);
+        else
+          hipLaunchKernelGGL(k_dev_stage<double>, dim3(grid_s), dim3(blk), 0, c->stream, base, c->d, dctl);
+        if (timed) (void)hipEventRecord(evs[10 * k + 2 * s5 + 1], c->stream);
+      }
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&hc, dctl, sizeof hc, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      cleanup();
+      return fail(GRW_EDEVICE, std::string("device run failed: ") + hipGetErrorString(e));
+    }
+    int kerr = 0;
+    (void)hipMemcpy(&kerr, c->d.err, sizeof(int), hipMemcpyDeviceToHost);
+    bool finite = true;
+    for (int q = 0; q < kNglobal; ++q) finite = finite && std::isfinite(hc.state[q]);
+    if (!finite) {
+      cleanup();
+      return fail(GRW_ESTATE, "global state became non-finite");
+    }
+    if (hc.attempts > 20000LL * (long long)nsteps + 100000LL) {
+      cleanup();
+      return fail(GRW_ESTATE, "integrator made no progress (step size collapsed)");
+    }
+    if (kerr || hc.err) {
+      cleanup();
+      return fail(GRW_ESTATE, "kernel met an unknown channel state, a non-finite rate or a runaway step count");
+    }
+    const long long done_att = hc.attempts - att_prev;
+    if (timed) {
+      const long long eff = std::min<long long>(done_att, K);
+      for (long long k = 0; k < eff; ++k)
+        for (int s5 = 0; s5 < 5; ++s5) {
+          float ms = 0.f;
+          if (hipEventElapsedTime(&ms, evs[10 * k + 2 * s5], evs[10 * k + 2 * s5 + 1]) == hipSuccess)
+            c->stats.kernel_ms += ms;
+        }
+    }
+    c->stats.launches += (uint64_t)(5 * done_att);
+    c->stats.simfru_calls += (uint64_t)(5 * done_att) * (uint64_t)c->n;
+    att_prev = hc.attempts;
+    if (hc.ms_done > ms_prev && done_att > 0)
+      ms_per_attempt = (double)(hc.ms_done - ms_prev) / (double)done_att;
+    ms_prev = hc.ms_done;
+    if (!hc.active) break;
+  }
+  if (hc.avg_pending) {
+    hipLaunchKernelGGL(k_dev_begin, dim3(grid_b), dim3(blk), 0, c->stream, base, c->d, dctl);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&hc, dctl, sizeof hc, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      cleanup();
+      return fail(GRW_EDEVICE, std::string("device run failed: ") + hipGetErrorString(e));
+    }
+  }
+  std::vector<double> hb(ns * (kNglobal + kNcur + 12) + (size_t)hc.log_n * 4);
+  e = hipMemcpy(hb.data(), dbuf, sizeof(double) * hb.size(), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(GRW_EDEVICE, std::string("output download failed: ") + hipGetErrorString(e));
+  }
+  if (states) std::memcpy(states, hb.data(), sizeof(double) * ns * kNglobal);
+  if (currents) std::memcpy(currents, hb.data() + ns * kNglobal, sizeof(double) * ns * kNcur);
+  if (other) std::memcpy(other, hb.data() + ns * (kNglobal + kNcur), sizeof(double) * ns * 12);
+  if (log && log_n) {
+    const int add = std::min(hc.log_n, log_cap - *log_n);
+    if (add > 0)
+      std::memcpy(log + 4 * (size_t)(*log_n), hb.data() + ns * (kNglobal + kNcur + 12),
+                  sizeof(double) * 4 * (size_t)add);
+    *log_n += std::max(add, 0);
+  }
+  for (int q = 0; q < kNglobal; ++q) state[q] = hc.state[q];
+  *h = hc.oh;
+  if (stt) {
+    stt->attempts += hc.attempts;
+    stt->rejects += hc.rejects;
+    stt->forced += hc.forced;
+  }
+  cleanup();
+  return rc;
+}
+
 }  // extern "C"
+

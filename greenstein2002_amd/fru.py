@@ -40,7 +40,7 @@ class FruCell:
 
     def __init__(self, nfru: int, nfru_scale: float | None = None, precision: str = "fp64",
                  block: int = 0, sequential_sums: bool = False, k23_scale: float = 1.0,
-                 gamma_scale: float = 1.0, device: int = 0):
+                 gamma_scale: float = 1.0, device: int = 0, lanes_per_fru: int = 4):
         L = _lib.load()
         cfg = GrwConfig()
         L.grw_config_default(C.byref(cfg))
@@ -54,6 +54,7 @@ class FruCell:
         cfg.k23_scale = float(k23_scale)
         cfg.gamma_scale = float(gamma_scale)
         cfg.device = int(device)
+        cfg.lanes_per_fru = int(lanes_per_fru)
         h = C.c_void_p()
         check(L.grw_create(C.byref(cfg), C.byref(h)))
         self._L = L
@@ -161,7 +162,7 @@ class FruCell:
         st = GrwStats()
         check(self._L.grw_get_stats(self._h, C.byref(st)))
         return dict(fru_steps=st.fru_steps, simfru_calls=st.simfru_calls, launches=st.launches,
-                    kernel_ms=st.kernel_ms)
+                    kernel_ms=st.kernel_ms, critical_steps=st.critical_steps, tail_ms=st.tail_ms)
 
     def reset_stats(self) -> None:
         check(self._L.grw_reset_stats(self._h))

@@ -64,6 +64,15 @@ int grw_run_cell(grw_ctx *ctx, const GrwProtocol *p, const double errw[37], doub
                  double state[37], double *h, double *states, double *currents, double *other,
                  double *log, int log_cap, int *log_n, GrwRkStats *st);
 
+/* grw_run_cell with the integrator on the device: the same results and
+ * arguments, but rk54pd and fcn run in the tails of the Monte Carlo launches
+ * and the host only enqueues batches of RK attempts (no host round trip per
+ * sub-interval).  Uses the four-lanes-per-FRU kernel. */
+int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37], double t0,
+                        int nsteps, double state[37], double *h, double *states,
+                        double *currents, double *other, double *log, int log_cap, int *log_n,
+                        GrwRkStats *st);
+
 #ifdef __cplusplus
 }
 #endif

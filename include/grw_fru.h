@@ -44,7 +44,7 @@ typedef struct {
   double k23_scale;      /* RyR opening-rate scale on k23cf (fru_rates_local.c:103); 1 = reference */
   double gamma_scale;    /* LCC mode-switch scale on gammacf (fru_rates_local.c:70); 1 = reference */
   int32_t device;        /* HIP device ordinal */
-  int32_t reserved;
+  int32_t lanes_per_fru; /* 4: one lane per cleft (default); 1: one lane per FRU */
 } grw_config;
 
 typedef struct {
@@ -52,6 +52,8 @@ typedef struct {
   uint64_t simfru_calls; /* FRU sub-interval calls (FRUs x distrib calls) */
   uint64_t launches;     /* Monte Carlo kernel launches */
   double kernel_ms;      /* summed device time of the Monte Carlo kernels */
+  uint64_t critical_steps; /* sum over launches of the largest FRU-step count in the launch */
+  double tail_ms;        /* summed device time of the serial launch tails (integrator) */
 } grw_stats;
 
 const char *grw_last_error(void);

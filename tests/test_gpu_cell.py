@@ -43,10 +43,11 @@ def oracle_cell(oracle, n, seed, ms, prot=None):
 
 
 def test_fp64_cell_follows_oracle(oracle):
+    """Host driver (the reference's call sequence through the ABI)."""
     n, seed, ms = 250, 1234, 20
     ref, att = oracle_cell(oracle, n, seed, ms)
     with Cell(n, seed=seed, sequential_sums=True) as cell:
-        res = cell.run(ms)
+        res = cell.run(ms, driver="host")
     assert res.stats["attempts"] == att
     np.testing.assert_allclose(res.states, ref, rtol=1e-9, atol=1e-14)
     assert res.states[:, 0].max() > 0.0  # the stimulus at 10 ms fired an action potential
@@ -58,8 +59,37 @@ def test_tree_reduction_cell_stays_close(oracle):
     n, seed, ms = 250, 77, 12
     ref, _ = oracle_cell(oracle, n, seed, ms)
     with Cell(n, seed=seed) as cell:
-        res = cell.run(ms)
+        res = cell.run(ms, driver="host")
     np.testing.assert_allclose(res.states[:, 0], ref[:, 0], rtol=1e-6, atol=1e-6)
+
+
+def test_device_driver_matches_host_driver(oracle):
+    """The integrator on the device takes the same decisions as the host one:
+    same attempts, rejections and trajectory (the device fcn's exp/log/pow may
+    differ from glibc's in the last ulp), over 25 ms with stimulus and release,
+    including the per-ms outputs and the accepted-step log."""
+    n, seed, ms = 512, 4321, 25
+    with Cell(n, seed=seed) as a:
+        ra = a.run(ms, driver="host", log_cap=5000)
+    with Cell(n, seed=seed) as b:
+        rb = b.run(ms, driver="device", log_cap=5000)
+    assert rb.stats["attempts"] == ra.stats["attempts"]
+    assert rb.stats["rejects"] == ra.stats["rejects"]
+    np.testing.assert_allclose(rb.states, ra.states, rtol=1e-8, atol=1e-13)
+    np.testing.assert_allclose(rb.currents, ra.currents, rtol=1e-7, atol=1e-10)
+    np.testing.assert_allclose(rb.other, ra.other, rtol=1e-8, atol=1e-13)
+    assert len(rb.log) == len(ra.log)
+    np.testing.assert_allclose(rb.log, ra.log, rtol=1e-7, atol=1e-10)
+
+
+def test_device_driver_follows_oracle(oracle):
+    """Whole cell with the device integrator vs the CPU oracle (20 ms)."""
+    n, seed, ms = 250, 99, 20
+    ref, att = oracle_cell(oracle, n, seed, ms)
+    with Cell(n, seed=seed) as cell:
+        res = cell.run(ms, driver="device")
+    np.testing.assert_allclose(res.states[:, 0], ref[:, 0], rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(res.states[:, 6], ref[:, 6], rtol=1e-6, atol=1e-12)
 
 
 def test_voltage_clamp_cell(oracle):

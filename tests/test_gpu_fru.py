@@ -62,16 +62,18 @@ def test_device_seeding_matches_reference(golden, oracle):
         assert np.array_equal(st["mt"][j], m["mt"])
 
 
-def test_simfru_golden_clamp_and_ramp(golden, oracle):
+@pytest.mark.parametrize("lanes", [4, 1])
+def test_simfru_golden_clamp_and_ramp(golden, oracle, lanes):
     """Single-FRU event sequences under voltage clamp with frozen global state and
-    under a ramp: the reference's own simfru_local outputs (fixture)."""
+    under a ramp: the reference's own simfru_local outputs (fixture); both kernel
+    layouts (4 lanes per FRU and 1 lane per FRU)."""
     g = golden("simfru_ref.npz")
     ic = load_stable(1250)
     nf = g["snap_fru"].shape[1]
     nsub, every, dts = int(g["nsub"]), int(g["every"]), float(g["dt_sub"])
     for p in range(len(g["v0"])):
         f = orc_frus(oracle, ic, 12350232, nf)
-        with FruCell(nf, nfru_scale=1.0) as cell:
+        with FruCell(nf, nfru_scale=1.0, lanes_per_fru=lanes) as cell:
             load_cell(cell, f)
             v0, v1 = float(g["v0"][p]), float(g["v1"][p])
             for s in range(nsub):
@@ -89,10 +91,9 @@ def test_simfru_golden_clamp_and_ramp(golden, oracle):
             assert np.array_equal(cell.parallel_get_FRUs()["mt"], g["mt_final"][p])
 
 
-@pytest.mark.parametrize("precision_seq", [(False,), (True,)])
-def test_distrib_matches_oracle(oracle, precision_seq):
+@pytest.mark.parametrize("seq,lanes", [(False, 4), (True, 4), (False, 1), (True, 1)])
+def test_distrib_matches_oracle(oracle, seq, lanes):
     """4,096 FRUs through an action-potential-like sequence of sub-intervals."""
-    (seq,) = precision_seq
     n = 4096
     ic = load_stable(n)
     rng = np.random.default_rng(7)
@@ -103,7 +104,7 @@ def test_distrib_matches_oracle(oracle, precision_seq):
     f["conc"][: n // 4, 1:] *= 10 ** rng.uniform(0, 2.5, (n // 4, 4))
     f["mti"][:64] = rng.integers(0, 625, 64)
     volt = [-85.0, -60.0, 0.0, 30.0, 45.0, 20.0, -10.0, -40.0]
-    with FruCell(n, sequential_sums=seq) as cell:
+    with FruCell(n, sequential_sums=seq, lanes_per_fru=lanes) as cell:
         load_cell(cell, f)
         t = 0.0
         for k in range(len(volt) - 1):
@@ -126,7 +127,8 @@ def test_distrib_matches_oracle(oracle, precision_seq):
         np.testing.assert_allclose(avg, oracle.avg_sums(f), rtol=1e-12 if not seq else 0, atol=1e-15)
 
 
-def test_save_resume_replays_identically(oracle):
+@pytest.mark.parametrize("lanes", [4, 1])
+def test_save_resume_replays_identically(oracle, lanes):
     """send_save_state / send_resume_state: rollback restores FRU and RNG state
     bit for bit, and the replayed Monte Carlo repeats the same events."""
     n = 1024
@@ -134,7 +136,7 @@ def test_save_resume_replays_identically(oracle):
     f = orc_frus(oracle, ic, 99, n)
     dep0 = np.array([-20.0, 3e-4, 0.9])
     depf = np.array([30.0, 6e-4, 0.9])
-    with FruCell(n) as cell:
+    with FruCell(n, lanes_per_fru=lanes) as cell:
         load_cell(cell, f)
         cell.distrib_simFRU(0.0, 0.1, dep0, dep0)
         cell.send_save_state()
@@ -162,14 +164,15 @@ def test_save_resume_replays_identically(oracle):
             assert np.array_equal(b2[key], s2[key]), key
 
 
-def test_state_roundtrip_all_rng_positions(oracle):
+@pytest.mark.parametrize("lanes", [4, 1])
+def test_state_roundtrip_all_rng_positions(oracle, lanes):
     """load/get convert the reference's (mt, mti) to the device's lazy stream and
     back for every position 0..624, including after regeneration."""
     n = 625
     ic = load_stable(n)
     f = orc_frus(oracle, ic, 5, n)
     f["mti"] = np.arange(625)
-    with FruCell(n) as cell:
+    with FruCell(n, lanes_per_fru=lanes) as cell:
         load_cell(cell, f)
         st = cell.parallel_get_FRUs()
         assert np.array_equal(st["mti"], f["mti"])
