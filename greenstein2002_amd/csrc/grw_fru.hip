@@ -5,10 +5,10 @@
 //   conc  double [5][ld]   CaJSR, CaSS1..4                      (SoA, coalesced)
 //   chan  uint32 [4][ld]   packed LCC/gate/5 RyR/Ito2 per cleft  (grw_model.h)
 //   mtidx int32  [ld]      stream position, see MtLane
-//   nlog  int32  [ld]      words saved in the undo log since the last checkpoint
-//   mt    uint32 [n][640]  one MT19937 row per FRU (624 words used)
-//   mtlog uint32 [n][640]  undo log: first pre-checkpoint value of each rewritten word
-//   *_h                    checkpoint of conc / chan / mtidx
+//   nreg  int32  [ld]      words regenerated since the last checkpoint
+//   mt    uint32 [n][640]  one MT19937 row per FRU (624 words used, 16-byte aligned)
+//   *_h                    checkpoint of conc / chan / mtidx, and mtw_h: the
+//                          first word the next regeneration overwrites
 //
 // One lane advances one FRU through the whole sub-interval (simfru_local.c:127-277)
 // with every continuous and discrete state in registers; each lane draws from
@@ -50,14 +50,14 @@ namespace {
 // idx in [624, 1248): lazy regime, p = idx - 624; words [0, p) belong to the
 // new generation, [p, 624) to the previous one.
 //
-// Rollback: the first overwrite of each word after a checkpoint stores the old
-// value in the undo log (nlog counts them, at most 624), so a checkpoint costs
-// nothing for the MT rows and a rollback restores only the rewritten words.
+// Rollback: a checkpoint keeps the stream position and the one word the next
+// regeneration overwrites first; nreg counts the words regenerated since.  The
+// regeneration is invertible (mt_rollback), so a rollback recomputes the
+// overwritten words from the row itself and nothing is logged per draw.
 struct MtLane {
   uint32_t* row;
-  uint32_t* log;
   int idx;
-  int nlog;
+  int nreg;
 
   __device__ __forceinline__ static uint32_t temper(uint32_t y) {
     y ^= y >> 11;
@@ -74,10 +74,7 @@ struct MtLane {
     const uint32_t far = row[p < kMtN - 397 ? p + 397 : p - (kMtN - 397)];
     const uint32_t y = (cur & 0x80000000u) | (nxt & 0x7fffffffu);
     const uint32_t w = far ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
-    if (nlog < kMtN) {
-      log[p] = cur;
-      ++nlog;
-    }
+    ++nreg;
     row[p] = w;
     idx = (p == kMtN - 1) ? kMtN : idx + 1;
     return temper(w);
@@ -117,12 +114,9 @@ struct MtLane {
       for (int k = 0; k < 32; ++k) {
         int q = p + k;
         q -= (q >= kMtN) ? kMtN : 0;
-        if (nlog < kMtN) {
-          log[q] = cur[k];
-          ++nlog;
-        }
         row[q] = far[k];
       }
+      nreg += 32;
       idx += 32;
       idx -= (idx >= 2 * kMtN) ? kMtN : 0;
     } else if (idx + 32 <= kMtN) {
@@ -158,12 +152,12 @@ struct Dev {
   double* conc;
   uint32_t* chan;
   int32_t* mtidx;
-  int32_t* nlog;
+  int32_t* nreg;
   uint32_t* mt;
-  uint32_t* mtlog;
   double* conc_h;
   uint32_t* chan_h;
   int32_t* mtidx_h;
+  uint32_t* mtw_h;
   double* part;              // [grid][16] block partial sums
   unsigned int* ticket;      // last-block election counter (self-resetting)
   double* out;               // [32]: 0-3 fluxes, 4-8 flux sums, 16-26 avg sums
@@ -289,7 +283,7 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) ch[c] = d.chan[c * a.ld + i];
     if (SIM) {
-      MtLane m{d.mt + (size_t)i * kMtStride, d.mtlog + (size_t)i * kMtStride, d.mtidx[i], d.nlog[i]};
+      MtLane m{d.mt + (size_t)i * kMtStride, d.mtidx[i], d.nreg[i]};
       const double span0 = a.depf[0] - a.dep0[0];
       const double span1 = a.depf[1] - a.dep0[1];
       const double span2 = a.depf[2] - a.dep0[2];
@@ -426,7 +420,7 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) d.chan[c * a.ld + i] = ch[c];
       d.mtidx[i] = m.idx;
-      d.nlog[i] = m.nlog;
+      d.nreg[i] = m.nreg;
     }
     if (REDUCE) flux_terms(conc, ch, acc);
   }
@@ -526,23 +520,30 @@ __device__ __forceinline__ R ryr_total(int s, R T1, R T2, R T3, R T5, R T7, R T8
        : s == 6 ? R(k65) : s == 7 ? T7 : T8;
 }
 
+// Row position of absolute stream position g in [0, 1872): positions below 624
+// are words of a loaded full generation, read as is; the others are lazy.
+__device__ __forceinline__ int mt_row_pos(int g) {
+  int q = (g >= kMtN) ? g - kMtN : g;
+  return q - ((q >= kMtN) ? kMtN : 0);
+}
+
+// One regeneration step of the twist (mt19937_local.c:121-141 for one word).
+__device__ __forceinline__ uint32_t mt_twist(uint32_t cur, uint32_t nxt, uint32_t far) {
+  const uint32_t y = (cur & 0x80000000u) | (nxt & 0x7fffffffu);
+  return far ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+}
+
 // MT words of one lane for absolute stream positions g0 .. g0+N-1 (only the
-// first n are produced).  Positions below 624 are read as is (a loaded full
-// generation); the others are regenerated lazily, and the first overwrite of
-// a word since the checkpoint is logged.  lazy0 is the first lazy position of
-// this micro-step and nlog0 the log fill at its start.  All loads precede all
-// stores; no word depends on another word of the same micro-step.
+// first n are produced), with dword accesses; used for the merged-state draws
+// and after them.  All loads precede all stores: no word of a micro-step
+// depends on another word of the same micro-step.
 template <int N>
-__device__ __forceinline__ void mt_run(uint32_t* row, uint32_t* lg, int lazy0, int nlog0, int g0,
-                                       int n, uint32_t out[N]) {
+__device__ __forceinline__ void mt_run(uint32_t* row, int g0, int n, uint32_t out[N]) {
   uint32_t cur[N], nxt[N], far[N];
-  int q[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const int g = g0 + k;
-    int qq = (g >= kMtN) ? g - kMtN : g;
-    qq -= (qq >= kMtN) ? kMtN : 0;
-    q[k] = qq;
+    const int qq = mt_row_pos(g);
     cur[k] = 0u; nxt[k] = 0u; far[k] = 0u;
     if (k < n) {
       cur[k] = row[qq];
@@ -557,10 +558,8 @@ __device__ __forceinline__ void mt_run(uint32_t* row, uint32_t* lg, int lazy0, i
     const int g = g0 + k;
     if (k < n) {
       if (g >= kMtN) {
-        const uint32_t y = (cur[k] & 0x80000000u) | (nxt[k] & 0x7fffffffu);
-        const uint32_t wv = far[k] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
-        if (nlog0 + (g - lazy0) < kMtN) lg[q[k]] = cur[k];
-        row[q[k]] = wv;
+        const uint32_t wv = mt_twist(cur[k], nxt[k], far[k]);
+        row[mt_row_pos(g)] = wv;
         out[k] = MtLane::temper(wv);
       } else {
         out[k] = MtLane::temper(cur[k]);
@@ -568,6 +567,111 @@ __device__ __forceinline__ void mt_run(uint32_t* row, uint32_t* lg, int lazy0, i
     }
   }
 }
+
+// The eight words of one cleft's draws, fetched as 16-byte chunks.  Row
+// positions q..q+8 (the draws and the word after them) lie in the three chunks
+// from q & ~3, and the words they are xored with, q+397..q+404 (mod 624, both
+// for q < 227 and q >= 227), in the three chunks from (q+397 mod 624) & ~3;
+// 624 = 4 * 156, so chunks never straddle the end of a generation and chunk
+// indices simply wrap mod 156.
+struct MtWin8 {
+  uint4 a0, a1, a2, b0, b1, b2;
+};
+
+__device__ __forceinline__ int mt_chunk_next(int j) { return j == kMtN / 4 - 1 ? 0 : j + 1; }
+
+__device__ __forceinline__ void mt_fetch8(const uint32_t* row, int g0, MtWin8& w) {
+  const int q = mt_row_pos(g0);
+  int qf = q + 397;
+  qf -= (qf >= kMtN) ? kMtN : 0;
+  const uint4* r4 = reinterpret_cast<const uint4*>(row);
+  const int ja = q >> 2, jb = qf >> 2;
+  const int ja1 = mt_chunk_next(ja), jb1 = mt_chunk_next(jb);
+  w.a0 = r4[ja];
+  w.a1 = r4[ja1];
+  w.a2 = r4[mt_chunk_next(ja1)];
+  w.b0 = r4[jb];
+  w.b1 = r4[jb1];
+  w.b2 = r4[mt_chunk_next(jb1)];
+}
+
+// Words r .. r+M-1 of the twelve words of three chunks (r in 0..3), by masks
+// rather than selects, which the compiler would turn into an indexed load from
+// a scratch copy of the array.
+template <int M>
+__device__ __forceinline__ void mt_select(const uint4& x0, const uint4& x1, const uint4& x2, int r,
+                                          uint32_t out[M]) {
+  const uint32_t S[12] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w, x2.x, x2.y, x2.z, x2.w};
+  const uint32_t m1 = 0u - (uint32_t)(r & 1), m2 = 0u - (uint32_t)((r >> 1) & 1);
+  uint32_t t[M + 2];
+#pragma unroll
+  for (int j = 0; j < M + 2; ++j) t[j] = (S[j] & ~m1) | (S[j + 1] & m1);
+#pragma unroll
+  for (int k = 0; k < M; ++k) out[k] = (t[k] & ~m2) | (t[k + 2] & m2);
+}
+
+// Regeneration and stores for stream positions g0 .. g0+7 from a fetched window.
+__device__ __forceinline__ void mt_finish8(uint32_t* row, int g0, const MtWin8& w, uint32_t out[8]) {
+  const int q = mt_row_pos(g0);
+  uint32_t cur[9], far[8];
+  mt_select<9>(w.a0, w.a1, w.a2, q & 3, cur);
+  mt_select<8>(w.b0, w.b1, w.b2, (q + 1) & 3, far);  // 397 = 1 mod 4
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int g = g0 + k;
+    if (g >= kMtN) {
+      const uint32_t wv = mt_twist(cur[k], cur[k + 1], far[k]);
+      row[mt_row_pos(g)] = wv;
+      out[k] = MtLane::temper(wv);
+    } else {
+      out[k] = MtLane::temper(cur[k]);
+    }
+  }
+}
+
+// Undo the nreg regenerations made since a checkpoint at stream position idx_h,
+// whose first overwritten word was w_first.  A regenerated word w at row
+// position q came from w = far ^ (y >> 1) ^ (y & 1 ? 0x9908b0df : 0), with
+// y = (old[q] & 0x80000000) | (next & 0x7fffffff) and far the word at
+// (q + 397) mod 624 as it was then.  The magic constant's top bit makes the
+// inversion unique: t = w ^ far gives y & 1 = t >> 31, then y >> 1.  So the
+// regeneration at q yields old[q]'s top bit and the low 31 bits of the word
+// after q, and old[q] = top bit from q | low bits from the regeneration before
+// it.  Walking from the newest word back, every far word is what the row holds
+// at that moment (words 397 later are already restored, words 227 earlier not
+// yet touched), for any count, including more than one generation.  Chunks
+// of 32 positions keep the loads of a chunk independent of its stores.
+__device__ __forceinline__ uint32_t mt_untwist(const uint32_t* row, int q) {
+  const uint32_t t = row[q] ^ row[q < kMtN - 397 ? q + 397 : q - (kMtN - 397)];
+  const uint32_t b = t >> 31;
+  const uint32_t yh = t ^ (b ? 0x9908b0dfu : 0u);
+  return (yh << 1) | b;
+}
+
+__device__ void mt_rollback(uint32_t* row, int idx_h, int nreg, uint32_t w_first) {
+  if (nreg <= 0) return;
+  const int q0 = idx_h >= kMtN ? idx_h - kMtN : 0;  // row position of the first regenerated word
+  int hi = nreg - 1;                                // offsets from q0, restored newest first
+  while (hi >= 1) {
+    const int lo = hi >= 32 ? hi - 31 : 1;
+    uint32_t y[33];
+#pragma unroll
+    for (int k = 0; k < 33; ++k) {
+      const int o = lo - 1 + k;
+      y[k] = (o <= hi) ? mt_untwist(row, (q0 + o) % kMtN) : 0u;
+    }
+#pragma unroll
+    for (int k = 1; k < 33; ++k) {
+      const int o = lo - 1 + k;
+      if (o <= hi) row[(q0 + o) % kMtN] = (y[k] & 0x80000000u) | (y[k - 1] & 0x7fffffffu);
+    }
+    hi = lo - 1;
+  }
+  row[q0] = w_first;
+}
+
+// Row word that the first regeneration after a checkpoint at idx overwrites.
+__device__ __forceinline__ int mt_first_pos(int idx) { return idx >= kMtN ? idx - kMtN : 0; }
 
 __device__ __forceinline__ double unif(uint32_t y) { return ((double)y) / 4294967296.0; }
 
@@ -616,9 +720,8 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     double css = d.conc[(c + 1) * a.ld + f];
     uint32_t w = d.chan[c * a.ld + f];
     uint32_t* const row = d.mt + (size_t)f * kMtStride;
-    uint32_t* const lg = d.mtlog + (size_t)f * kMtStride;
     int idx = d.mtidx[f];
-    int nlog = d.nlog[f];
+    int nreg = d.nreg[f];
     const double span0 = a.depf[0] - a.dep0[0];
     const double span1 = a.depf[1] - a.dep0[1];
     const double span2 = a.depf[2] - a.dep0[2];
@@ -630,6 +733,12 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     double t = a.t0;
     int bad = 0;
     int nguard = 0;
+    MtWin8 win;  // words of the next step's draws, fetched one step ahead
+    // Complete the FRU-state loads here: otherwise the loop header merges them
+    // with the prefetch below and the first use of the state in every step
+    // waits for all outstanding loads, prefetch included.
+    __builtin_amdgcn_s_waitcnt(0);
+    if (t < a.t1) mt_fetch8(row, idx + 8 * c, win);
     while (t < a.t1) {
       // ---- rates at the step start (fru_rates_local.c:126-777)
       const R V = (R)dV;
@@ -665,7 +774,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       if (nd) {
         const int Ec = (c > 0 ? n0 : 0) + (c > 1 ? n1 : 0) + (c > 2 ? n2 : 0);
         uint32_t vw[kNryr];
-        mt_run<kNryr>(row, lg, lazy0, nlog, idx + Ec, nd, vw);
+        mt_run<kNryr>(row, idx + Ec, nd, vw);
 #pragma unroll
         for (int r = 0; r < kNryr; ++r) {
           if (ryr_needs_draw(st[r], k.regime)) {
@@ -706,15 +815,28 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
         bad = 1;
         break;
       }
-      // ---- this cleft's 8 of the step's 32 uniforms (simfru_local.c:211)
+      // ---- this cleft's 8 of the step's 32 uniforms (simfru_local.c:211), from
+      // the words fetched during the previous step when no merged-state draw
+      // moved the window
       uint32_t dw[8];
-      mt_run<8>(row, lg, lazy0, nlog, idx + E + 8 * c, 8, dw);
+      if (E == 0)
+        mt_finish8(row, idx + 8 * c, win, dw);
+      else
+        mt_run<8>(row, idx + E + 8 * c, 8, dw);
       {
         const int idn = idx + E + 32;
         const int adv = idn - lazy0;
-        nlog = min(kMtN, nlog + (adv > 0 ? adv : 0));
+        nreg += (adv > 0 ? adv : 0);
         idx = (idn >= 2 * kMtN) ? idn - kMtN : idn;
       }
+      // Next step's words, fetched now so their latency hides behind this
+      // step's Heun step and transitions and the next step's rates.  The words
+      // used are positions idx .. idx+31 and those they regenerate from:
+      // positions idx+1 .. idx+32 (not yet regenerated) and words at least 195
+      // positions older than idx, none of which this step's stores (positions
+      // idx-32 .. idx-1) or any store before their use touches.  The 16-byte
+      // chunks also bring up to three unused words on either side.
+      mt_fetch8(row, idx + 8 * c, win);
       // ---- Heun step (simfru_local.c:149-173)
       R djs, dss;
       fcn_fru4<R>((R)cj, cc, (R)dV, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
@@ -752,7 +874,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     if (c == 0) {
       d.conc[f] = cj;
       d.mtidx[f] = idx;
-      d.nlog[f] = nlog;
+      d.nreg[f] = nreg;
     }
     acc[0] = css;
     acc[1] = (c == 0) ? cj : 0.0;
@@ -797,6 +919,27 @@ __global__ void __launch_bounds__(256) k_simfru4(const SimArgs a, const Dev d) {
       for (int q = 0; q < 5; ++q) d.out[4 + q] = tot[q];
     }
   }
+}
+
+// Checkpoint of FRU i (mpi_slave.c:192-226): concentrations, channels, the
+// stream position and the one MT word the next regeneration overwrites.
+__device__ __forceinline__ void fru_checkpoint(const SimArgs& a, const Dev& d, int i) {
+  for (int k = 0; k < 5; ++k) d.conc_h[k * a.ld + i] = d.conc[k * a.ld + i];
+  for (int c = 0; c < 4; ++c) d.chan_h[c * a.ld + i] = d.chan[c * a.ld + i];
+  const int idx = d.mtidx[i];
+  d.mtidx_h[i] = idx;
+  d.mtw_h[i] = d.mt[(size_t)i * kMtStride + mt_first_pos(idx)];
+  d.nreg[i] = 0;
+}
+
+// Rollback of FRU i to its checkpoint (mpi_slave.c:150-184).
+__device__ __forceinline__ void fru_rollback(const SimArgs& a, const Dev& d, int i) {
+  for (int k = 0; k < 5; ++k) d.conc[k * a.ld + i] = d.conc_h[k * a.ld + i];
+  for (int c = 0; c < 4; ++c) d.chan[c * a.ld + i] = d.chan_h[c * a.ld + i];
+  const int idx = d.mtidx_h[i];
+  mt_rollback(d.mt + (size_t)i * kMtStride, idx, d.nreg[i], d.mtw_h[i]);
+  d.mtidx[i] = idx;
+  d.nreg[i] = 0;
 }
 
 // ===========================================================================
@@ -850,27 +993,8 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs a, const Dev d,
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = 0.0;
   if (i < a.nfru) {
-    if (active && resume) {  // mpi_slave.c:150-184
-      for (int q = 0; q < 5; ++q) d.conc[q * a.ld + i] = d.conc_h[q * a.ld + i];
-      for (int q = 0; q < 4; ++q) d.chan[q * a.ld + i] = d.chan_h[q * a.ld + i];
-      const int idx0 = d.mtidx_h[i];
-      const int p0 = idx0 >= kMtN ? idx0 - kMtN : 0;
-      const int n = min(d.nlog[i], kMtN);
-      uint32_t* row = d.mt + (size_t)i * kMtStride;
-      const uint32_t* lg = d.mtlog + (size_t)i * kMtStride;
-      for (int q = 0; q < n; ++q) {
-        int p = p0 + q;
-        if (p >= kMtN) p -= kMtN;
-        row[p] = lg[p];
-      }
-      d.mtidx[i] = idx0;
-      d.nlog[i] = 0;
-    } else if (active && success) {  // mpi_slave.c:192-226
-      for (int q = 0; q < 5; ++q) d.conc_h[q * a.ld + i] = d.conc[q * a.ld + i];
-      for (int q = 0; q < 4; ++q) d.chan_h[q * a.ld + i] = d.chan[q * a.ld + i];
-      d.mtidx_h[i] = d.mtidx[i];
-      d.nlog[i] = 0;
-    }
+    if (active && resume) fru_rollback(a, d, i);
+    else if (active && success) fru_checkpoint(a, d, i);
     double conc[5];
     uint32_t ch[4];
     for (int q = 0; q < 5; ++q) conc[q] = d.conc[q * a.ld + i];
@@ -1222,35 +1346,14 @@ __global__ void k_seq_sums(const SimArgs a, const Dev d) {
   for (int k = 0; k < 11; ++k) d.out[16 + k] = avg[k];
 }
 
-// Checkpoint (mpi_slave.c:192-226): the MT rows are covered by the undo log.
 __global__ void k_save(const SimArgs a, const Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.nfru) return;
-  for (int k = 0; k < 5; ++k) d.conc_h[k * a.ld + i] = d.conc[k * a.ld + i];
-  for (int c = 0; c < 4; ++c) d.chan_h[c * a.ld + i] = d.chan[c * a.ld + i];
-  d.mtidx_h[i] = d.mtidx[i];
-  d.nlog[i] = 0;
+  if (i < a.nfru) fru_checkpoint(a, d, i);
 }
 
-// Rollback (mpi_slave.c:150-184): restore the words rewritten since the
-// checkpoint, which are consecutive from the checkpoint's lazy position.
 __global__ void k_resume(const SimArgs a, const Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.nfru) return;
-  for (int k = 0; k < 5; ++k) d.conc[k * a.ld + i] = d.conc_h[k * a.ld + i];
-  for (int c = 0; c < 4; ++c) d.chan[c * a.ld + i] = d.chan_h[c * a.ld + i];
-  const int idx0 = d.mtidx_h[i];
-  const int p0 = idx0 >= kMtN ? idx0 - kMtN : 0;
-  const int n = min(d.nlog[i], kMtN);
-  uint32_t* row = d.mt + (size_t)i * kMtStride;
-  const uint32_t* log = d.mtlog + (size_t)i * kMtStride;
-  for (int k = 0; k < n; ++k) {
-    int q = p0 + k;
-    if (q >= kMtN) q -= kMtN;
-    row[q] = log[q];
-  }
-  d.mtidx[i] = idx0;
-  d.nlog[i] = 0;
+  if (i < a.nfru) fru_rollback(a, d, i);
 }
 
 // initialize_ran.c:23-36 + sgrnd_local (mt19937_local.c:75-92) per FRU.
@@ -1267,7 +1370,7 @@ __global__ void k_seed(const SimArgs a, const Dev d, uint32_t a0) {
     row[k] = x;
   }
   d.mtidx[i] = kMtN;  // the first draw regenerates word 0
-  d.nlog[i] = 0;
+  d.nreg[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1436,8 +1539,8 @@ int grw_create(const grw_config* cfg, grw_ctx** out) {
   int rc = 0;
   const size_t ld = c->ld, n = c->n;
   if ((rc = dalloc(c, &c->d.conc, 5 * ld)) || (rc = dalloc(c, &c->d.chan, 4 * ld)) ||
-      (rc = dalloc(c, &c->d.mtidx, ld)) || (rc = dalloc(c, &c->d.nlog, ld)) ||
-      (rc = dalloc(c, &c->d.mt, n * kMtStride)) || (rc = dalloc(c, &c->d.mtlog, n * kMtStride)) ||
+      (rc = dalloc(c, &c->d.mtidx, ld)) || (rc = dalloc(c, &c->d.nreg, ld)) ||
+      (rc = dalloc(c, &c->d.mt, n * kMtStride)) || (rc = dalloc(c, &c->d.mtw_h, ld)) ||
       (rc = dalloc(c, &c->d.conc_h, 5 * ld)) || (rc = dalloc(c, &c->d.chan_h, 4 * ld)) ||
       (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, (size_t)c->grid_sim * 16)) ||
       (rc = dalloc(c, &c->d.ticket, 1)) || (rc = dalloc(c, &c->d.out, 32)) ||
@@ -1499,7 +1602,7 @@ int grw_load_state(grw_ctx* c, const double* fru, const int32_t* ltype, const in
   HIPCHK(hipMemcpyAsync(c->d.conc, conc.data(), conc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d.chan, chan.data(), chan.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d.mtidx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemsetAsync(c->d.nlog, 0, (size_t)ld * sizeof(int32_t), c->stream));
+  HIPCHK(hipMemsetAsync(c->d.nreg, 0, (size_t)ld * sizeof(int32_t), c->stream));
   HIPCHK(hipMemcpyAsync(c->d.mt, rows.data(), rows.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return grw_save_state(c);

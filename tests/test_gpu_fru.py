@@ -178,6 +178,14 @@ def test_state_roundtrip_all_rng_positions(oracle, lanes):
         assert np.array_equal(st["mti"], f["mti"])
         assert np.array_equal(st["mt"], f["mt"])
         dep = np.array([10.0, 2e-4, 0.9])
+        # rollback from every checkpoint position, including positions inside a
+        # loaded generation (no word regenerated yet at the checkpoint)
+        cell.send_save_state()
+        cell.simfru_local(0.0, 0.3, dep, dep)
+        cell.send_resume_state()
+        back = cell.parallel_get_FRUs()
+        assert np.array_equal(back["mti"], f["mti"])
+        assert np.array_equal(back["mt"], f["mt"])
         cell.simfru_local(0.0, 0.3, dep, dep)
         for j in range(n):
             oracle.simfru(0.0, 0.3, dep, dep, f, j)
