@@ -157,9 +157,11 @@ def main():
         achieved = alg_bytes / kernel_s / 1e9
         roof = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 6), traffic=None,
-                    kernel=("k_simfru4" if args.lanes == 4 else "k_simfru") +
-                           ("<double>" if args.precision == "fp64" else "<float>") +
-                           " (FRU sub-interval + fused flux reduction)",
+                    kernel=(("k_dev_stage" if args.driver == "device" else
+                             "k_simfru4" if args.lanes == 4 else "k_simfru") +
+                            ("<double>" if args.precision == "fp64" else "<float>") +
+                            " (FRU sub-interval + fused flux reduction" +
+                            (" + integrator stage)" if args.driver == "device" else ")")),
                     avg_launch_us=round(st["kernel_ms"] / st["launches"] * 1e3, 3),
                     launches=st["launches"], fru_steps=st["fru_steps"],
                     critical_steps_per_launch=round(st["critical_steps"] / st["launches"], 2),
