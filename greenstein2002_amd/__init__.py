@@ -10,3 +10,15 @@ from ._lib import GrwError
 from .fru import FruCell
 
 __all__ = ["FruCell", "GrwError"]
+
+
+def __getattr__(name):  # the whole-cell and batch layers load on first use
+    if name in ("Cell", "Protocol"):
+        from . import cell
+
+        return getattr(cell, name)
+    if name in ("Batch", "CellSpec"):
+        from . import batch
+
+        return getattr(batch, name)
+    raise AttributeError(name)

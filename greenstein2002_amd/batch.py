@@ -1,0 +1,153 @@
+"""Many independent cells on one GPU (include/grw_batch.h).
+
+``Batch`` runs cells of different FRU counts, seeds, parameter scales and
+protocols together: every RK stage of every cell is one set of launches, and
+each cell's global ODE is integrated on the device with its own step sizes.
+All cells start from ic/stable (FRU j takes record j mod 1250, the reference's
+rewind, initialize_state.c:38-41) with each cell's total-Ca entry computed for
+its own scale (initialize_state.c:233-246).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import GRW_FP64, GRW_MIXED, GrwStats, check
+from .beats import NSTAT
+from .cell import NGLOBAL, GrwProtocol, GrwRkStats, Protocol, _declare as _declare_cell
+from .ic import REWIND, errweight, load_stable
+
+
+class GrwBatchConfig(C.Structure):
+    _fields_ = [("ncells", C.c_int32), ("precision", C.c_int32), ("device", C.c_int32),
+                ("beat_cap", C.c_int32)]
+
+
+class GrwCellSpec(C.Structure):
+    _fields_ = [("nfru", C.c_int32), ("seed", C.c_uint32), ("nfru_scale", C.c_double),
+                ("k23_scale", C.c_double), ("gamma_scale", C.c_double), ("prot", GrwProtocol)]
+
+
+@dataclass
+class CellSpec:
+    nfru: int
+    seed: int = 1234
+    nfru_scale: float | None = None   # None: 12500 / nfru
+    k23_scale: float = 1.0            # RyR opening rate (fru_rates_local.c:103)
+    gamma_scale: float = 1.0          # LCC mode switch (fru_rates_local.c:70)
+    protocol: Protocol = field(default_factory=Protocol)
+
+    @property
+    def scale(self) -> float:
+        return 12500.0 / self.nfru if self.nfru_scale is None else float(self.nfru_scale)
+
+
+def _declare(L):
+    if getattr(L, "_grw_batch_declared", False):
+        return
+    _declare_cell(L)
+    vp = C.c_void_p
+    L.grw_batch_create.argtypes = [C.POINTER(GrwBatchConfig), vp, C.POINTER(vp)]
+    L.grw_batch_load_ic.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp]
+    L.grw_batch_run.argtypes = [vp, vp, vp, vp]
+    L.grw_batch_get_beats.argtypes = [vp, C.c_int, vp, C.c_int, C.POINTER(C.c_int), C.c_int]
+    for n in ("grw_batch_create", "grw_batch_load_ic", "grw_batch_run", "grw_batch_get_beats"):
+        getattr(L, n).restype = C.c_int
+    L._grw_batch_declared = True
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Batch:
+    """Cells ``specs`` on one GPU, started from ic/stable at t = 0."""
+
+    def __init__(self, specs, precision: str = "fp64", beat_cap: int = 64, device: int = 0):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.specs = list(specs)
+        L = _lib.load()
+        _declare(L)
+        self._L = L
+        n = len(self.specs)
+        cfg = GrwBatchConfig(ncells=n, precision=GRW_FP64 if precision == "fp64" else GRW_MIXED,
+                             device=device, beat_cap=beat_cap)
+        arr = (GrwCellSpec * n)()
+        for k, sp in enumerate(self.specs):
+            arr[k].nfru = int(sp.nfru)
+            arr[k].seed = int(sp.seed) & 0xFFFFFFFF
+            arr[k].nfru_scale = sp.scale
+            arr[k].k23_scale = float(sp.k23_scale)
+            arr[k].gamma_scale = float(sp.gamma_scale)
+            arr[k].prot = sp.protocol.to_c()
+        h = C.c_void_p()
+        check(L.grw_batch_create(C.byref(cfg), C.cast(arr, C.c_void_p), C.byref(h)))
+        self._h = h
+        self.ncells = n
+        self.beat_cap = beat_cap
+        self.t = np.zeros(n)
+        self.attempts = np.zeros(n, np.int64)
+        self.rejects = np.zeros(n, np.int64)
+        base = load_stable(REWIND)
+        states = np.array([load_stable(sp.nfru, sp.scale)["state"] for sp in self.specs], np.float64)
+        self.errw = errweight()
+        fru = np.ascontiguousarray(base["fru"], np.float64)
+        lt = np.ascontiguousarray(base["ltype"], np.int32)
+        ryr = np.ascontiguousarray(base["ryr"], np.int32)
+        ito2 = np.ascontiguousarray(base["ito2"], np.int32)
+        check(L.grw_batch_load_ic(h, _vp(states), _vp(self.errw), REWIND, _vp(fru), _vp(lt), _vp(ryr),
+                                  _vp(ito2)))
+        self.state = states
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.grw_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps) -> np.ndarray:
+        """Advance cell k by nsteps[k] output steps (an int applies to all);
+        returns the global states [ncells, 37]."""
+        ns = np.broadcast_to(np.asarray(nsteps, np.int32), (self.ncells,)).copy()
+        st = (GrwRkStats * self.ncells)()
+        out = np.zeros((self.ncells, NGLOBAL))
+        check(self._L.grw_batch_run(self._h, _vp(ns), _vp(out), C.cast(st, C.c_void_p)))
+        self.attempts += np.array([s.attempts for s in st], np.int64)
+        self.rejects += np.array([s.rejects for s in st], np.int64)
+        self.t = self.t + ns * np.array([sp.protocol.time_step for sp in self.specs])
+        self.state = out
+        return out
+
+    def beats(self, cell: int, include_current: bool = True) -> np.ndarray:
+        cap = self.beat_cap + 1
+        out = np.zeros((cap, NSTAT))
+        n = C.c_int(0)
+        check(self._L.grw_batch_get_beats(self._h, int(cell), _vp(out), cap, C.byref(n), int(include_current)))
+        return out[: n.value].copy()
+
+    def stats(self) -> dict:
+        st = GrwStats()
+        check(self._L.grw_get_stats(self._h, C.byref(st)))
+        return dict(fru_steps=st.fru_steps, simfru_calls=st.simfru_calls, launches=st.launches,
+                    kernel_ms=st.kernel_ms, tail_ms=st.tail_ms)
+
+    def reset_stats(self) -> None:
+        check(self._L.grw_reset_stats(self._h))
+
+    def set_timing(self, on: bool) -> None:
+        check(self._L.grw_set_timing(self._h, int(bool(on))))

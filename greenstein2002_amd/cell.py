@@ -66,6 +66,8 @@ def _declare(L):
     L.grw_run_cell.restype = C.c_int
     L.grw_run_cell_device.argtypes = L.grw_run_cell.argtypes
     L.grw_run_cell_device.restype = C.c_int
+    L.grw_get_beats.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.c_int]
+    L.grw_get_beats.restype = C.c_int
     L._grw_cell_declared = True
 
 
@@ -159,6 +161,17 @@ class Cell:
 
     def __exit__(self, *exc):
         self.close()
+
+    def beats(self, include_current: bool = True) -> np.ndarray:
+        """Per-beat statistics of the device-driven runs so far (grw_get_beats;
+        fields in ``greenstein2002_amd.beats.FIELDS``)."""
+        from .beats import NSTAT
+
+        cap = 4097
+        out = np.zeros((cap, NSTAT))
+        n = C.c_int(0)
+        check(self._L.grw_get_beats(self.fru._h, _vp(out), cap, C.byref(n), int(include_current)))
+        return out[: n.value].copy()
 
     def run(self, nsteps: int, log_cap: int = 200000, outputs: bool = True,
             driver: str = "device") -> RunResult:

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/grw_fru.h"
+#include "../../include/grw_batch.h"
 #include "grw_global.h"
 #include "grw_model.h"
 
@@ -146,6 +147,19 @@ struct SimArgs {
   double scale;
   double k23c, gcf; // scaled k23cf, gammacf
   int nfru, ld;
+  int f0, b0;       // the cell's first FRU in the arrays and first block of the grid
+  int ncells;
+};
+
+// One cell of a batch: its FRUs are [f0, f0 + nfru) of the shared arrays (f0 a
+// multiple of 64); the Monte Carlo grid (64 FRUs per block) gives it blocks
+// [sb0, sb0 + snb), the one-thread-per-FRU grid blocks [bb0, bb0 + bnb).
+struct CellInfo {
+  int nfru, f0;
+  int sb0, snb;
+  int bb0, bnb;
+  double scale, k23c, gcf;
+  unsigned seed;  // master seed of the cell's MT19937 streams
 };
 
 struct Dev {
@@ -159,13 +173,16 @@ struct Dev {
   int32_t* mtidx_h;
   uint32_t* mtw_h;
   double* part;              // [grid][16] block partial sums
-  unsigned int* ticket;      // last-block election counter (self-resetting)
+  unsigned int* ticket;      // last-block election counters, one per cell (self-resetting)
   double* out;               // [32]: 0-3 fluxes, 4-8 flux sums, 16-26 avg sums
   unsigned long long* nsteps;
   int* err;
   unsigned int* maxsteps;           // largest FRU-step count of the current launch
   unsigned long long* critsteps;    // sum over launches of maxsteps
   unsigned long long* tailticks;    // wall-clock ticks spent in the serial launch tails
+  const CellInfo* cells;            // [ncells]
+  const int* smap;                  // Monte Carlo grid block -> cell
+  const int* bmap;                  // one-thread-per-FRU grid block -> cell
 };
 
 // Categorical transition draw, simfru_local.c:217-227 / :234-245: stay with
@@ -188,7 +205,8 @@ __device__ __forceinline__ int pick(double u, double dt, const Exits<R>& e) {
 // last block to finish (ticket) sums the slots in a fixed order.  Returns true
 // in thread 0 of the last block, with the totals in tot[].
 template <int NV>
-__device__ bool reduce_grid_block(double v[NV], const Dev& d, int slot_off, double* tot_sh) {
+__device__ bool reduce_range(double v[NV], const Dev& d, unsigned int* ticket, int b0, int nb,
+                             int slot_off, double* tot_sh) {
   __shared__ double wsum[4][NV];
   __shared__ bool last;
   for (int off = 32; off >= 1; off >>= 1)
@@ -206,8 +224,8 @@ __device__ bool reduce_grid_block(double v[NV], const Dev& d, int slot_off, doub
       d.part[(size_t)blockIdx.x * 16 + slot_off + k] = s;
     }
     __threadfence();
-    const unsigned t = atomicAdd(d.ticket, 1u);
-    last = (t == gridDim.x - 1);
+    const unsigned t = atomicAdd(ticket, 1u);
+    last = (t == (unsigned)nb - 1u);
   }
   __syncthreads();
   if (!last) return false;
@@ -216,7 +234,7 @@ __device__ bool reduce_grid_block(double v[NV], const Dev& d, int slot_off, doub
   double s[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) s[k] = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+  for (int b = b0 + (int)threadIdx.x; b < b0 + nb; b += blockDim.x)
 #pragma unroll
     for (int k = 0; k < NV; ++k)
       s[k] += __hip_atomic_load(&d.part[(size_t)b * 16 + slot_off + k], __ATOMIC_RELAXED,
@@ -235,10 +253,15 @@ __device__ bool reduce_grid_block(double v[NV], const Dev& d, int slot_off, doub
       for (int w = 1; w < nw; ++w) a += wsum[w][k];
       tot_sh[k] = a;
     }
-    *d.ticket = 0u;  // ready for the next launch
+    *ticket = 0u;  // ready for the next launch
   }
   __syncthreads();
   return true;
+}
+
+template <int NV>
+__device__ bool reduce_grid_block(double v[NV], const Dev& d, int slot_off, double* tot_sh) {
+  return reduce_range<NV>(v, d, d.ticket, 0, gridDim.x, slot_off, tot_sh);
 }
 
 // Thread 0 of the last block gets the totals in tot[] (returns true there only).
@@ -711,11 +734,11 @@ __device__ __forceinline__ void fcn_fru4(R CaJSR, R s, R V, R Cai, R CaNSR, bool
 // FRU-step count (nonzero only in lane 0 of a quad) and its flux terms in acc.
 template <class R>
 __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[5]) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int f = gid >> 2;
+  const int gid = (blockIdx.x - a.b0) * blockDim.x + threadIdx.x;
+  const int f = a.f0 + (gid >> 2);
   const int c = gid & 3;
   unsigned steps = 0;
-  if (f < a.nfru) {  // quads are active or inactive as a whole
+  if ((gid >> 2) < a.nfru) {  // quads are active or inactive as a whole
     double cj = d.conc[f];
     double css = d.conc[(c + 1) * a.ld + f];
     uint32_t w = d.chan[c * a.ld + f];
@@ -952,6 +975,20 @@ __device__ __forceinline__ void fru_rollback(const SimArgs& a, const Dev& d, int
 // enqueues attempts (one begin kernel and five stage kernels each) and
 // synchronises once per batch; launches after the run is complete return at
 // once.  One RkCtl per cell holds the integrator's state.
+// Per-beat statistics kept by the integrator from its accepted steps (the
+// reference writes only 1 ms samples, too coarse for APD90 or peak ICaL).
+// Beat b of a paced cell starts at the stimulus t_s = shift + b * period; its
+// accepted steps are those ending in [t_s, t_s + period).  Record layout:
+//   0 t_s, 1 V at the t_s output sample (rest), 2 peak V, 3 its time - t_s,
+//   4 APD90: time - t_s at which V, after the peak, first falls through
+//     V_peak - 0.9 (V_peak - V_rest) (linear between accepted steps; NaN if it
+//     does not within the beat, or if the peak is under 20 mV above rest),
+//   5 peak Cai, 6 its time - t_s, 7 Cai at the t_s sample, 8 peak (most
+//   negative) ICa of the FRU flux conversion, 9 CaNSR at the t_s sample (SR
+//   load), 10 mean CaJSR at the t_s sample (JSR load, calc_fru_avg output 1),
+//   11 accepted steps in the beat.
+constexpr int kBeatStats = 12;
+
 struct RkCtl {
   GrwProtocol prot;
   double errw[kNglobal];
@@ -964,14 +1001,79 @@ struct RkCtl {
   double dep0[3], depf[3];
   double start, h, oh, tcall, tstep, ta, tb;
   int stage, success, keepc, resume;
-  int active, ms_done, ms_target, avg_pending, avg_row;
+  int active, ms_done, ms_target, ms_base, avg_pending, avg_row;  // rows count from ms_base
   int log_n, log_cap, err;
   long long attempts, rejects, forced;
-  double* out_states;  // [ms_target][37]
-  double* out_cur;     // [ms_target][24]
-  double* out_other;   // [ms_target][12]
-  double* out_log;     // [log_cap][4]
+  double* out_states;  // [ms_target - ms_base][37] or null
+  double* out_cur;     // [ms_target - ms_base][24] or null
+  double* out_other;   // [ms_target - ms_base][12] or null
+  double* out_log;     // [log_cap][4]: accepted steps (t, V, Cai, ICa)
+  double* out_beats;   // [beat_cap][kBeatStats] or null
+  int beat_cap, beat_n, beat_cur, pend_row;
+  double bt[kBeatStats];
+  double prev_t, prev_V;
+  double pend_t, pend_V, pend_Cai, pend_NSR, pend_JSR;  // the last output sample
 };
+
+__device__ __forceinline__ void beat_flush(RkCtl* c) {
+  if (c->beat_cur >= 0 && c->beat_n < c->beat_cap) {
+    for (int q = 0; q < kBeatStats; ++q) c->out_beats[(size_t)c->beat_n * kBeatStats + q] = c->bt[q];
+    c->beat_n++;
+  }
+  c->beat_cur = -1;
+}
+
+// The output sample at the beat's stimulus time supplies its rest values; it
+// may be taken before or after the beat's first accepted step.
+__device__ __forceinline__ void beat_take_sample(RkCtl* c) {
+  if (c->beat_cur >= 0 && c->pend_t == c->bt[0]) {
+    c->bt[1] = c->pend_V;
+    c->bt[7] = c->pend_Cai;
+    c->bt[9] = c->pend_NSR;
+    c->bt[10] = c->pend_JSR;
+  }
+}
+
+__device__ void beat_update(RkCtl* c, double t, double V, double Cai, double ICa) {
+  if (!c->out_beats) return;
+  const GrwProtocol& p = c->prot;
+  const int b = (t >= p.shift) ? (int)floor((t - p.shift) / p.period) : -1;
+  if (b != c->beat_cur) {
+    beat_flush(c);
+    c->beat_cur = b;
+    if (b >= 0) {
+      double* bt = c->bt;
+      bt[0] = p.shift + b * p.period;
+      bt[1] = bt[4] = bt[7] = bt[9] = bt[10] = __builtin_nan("");
+      bt[2] = V;
+      bt[3] = t - bt[0];
+      bt[5] = Cai;
+      bt[6] = t - bt[0];
+      bt[8] = ICa;
+      bt[11] = 1.0;
+      beat_take_sample(c);
+    }
+  } else if (b >= 0) {
+    double* bt = c->bt;
+    if (V > bt[2]) {
+      bt[2] = V;
+      bt[3] = t - bt[0];
+    }
+    if (Cai > bt[5]) {
+      bt[5] = Cai;
+      bt[6] = t - bt[0];
+    }
+    if (ICa < bt[8]) bt[8] = ICa;
+    bt[11] += 1.0;
+    if (bt[4] != bt[4] && bt[1] == bt[1] && bt[2] - bt[1] > 20.0 && t - bt[0] > bt[3]) {
+      const double thr = bt[2] - 0.9 * (bt[2] - bt[1]);
+      if (c->prev_V >= thr && V < thr)
+        bt[4] = c->prev_t + (thr - c->prev_V) * (t - c->prev_t) / (V - c->prev_V) - bt[0];
+    }
+  }
+  c->prev_t = t;
+  c->prev_V = V;
+}
 
 // Stage s (1..5) sub-interval bounds, rk_pd54m.c:124,145-146,166-167,187-188,208-209.
 __device__ __forceinline__ void stage_bounds(int s, double start, double h, double& ta, double& tb) {
@@ -984,15 +1086,25 @@ __device__ __forceinline__ void stage_bounds(int s, double start, double h, doub
   }
 }
 
-__global__ void __launch_bounds__(256) k_dev_begin(const SimArgs a, const Dev d, RkCtl* ctl) {
+// Attempt start of every cell: rollback or checkpoint, the flux sums (and the
+// average sums of a finished output sample), then in the cell's last block
+// fcn at the attempt start and the first stage (rk_pd54m.c:93-138).  One
+// thread per FRU; blocks map to cells through d.bmap.
+__global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev d, RkCtl* ctls) {
+  const int cell = d.bmap[blockIdx.x];
+  const CellInfo ci = d.cells[cell];
+  RkCtl* ctl = ctls + cell;
   const int active = ctl->active, avgp = ctl->avg_pending;
   if (!active && !avgp) return;
+  SimArgs a = base;
+  a.nfru = ci.nfru;
   const int resume = ctl->resume, success = ctl->success;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int il = (blockIdx.x - ci.bb0) * blockDim.x + threadIdx.x;
+  const int i = ci.f0 + il;
   double v[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = 0.0;
-  if (i < a.nfru) {
+  if (il < ci.nfru) {
     if (active && resume) fru_rollback(a, d, i);
     else if (active && success) fru_checkpoint(a, d, i);
     double conc[5];
@@ -1026,8 +1138,8 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs a, const Dev d,
     }
   }
   __shared__ double tot[16];
-  if (!reduce_grid_block<16>(v, d, 0, tot)) return;
-  // ---- last block: outputs of the finished call, fcn at the attempt start
+  if (!reduce_range<16>(v, d, d.ticket + cell, ci.bb0, ci.bnb, 0, tot)) return;
+  // ---- last block of the cell: outputs of the finished call, fcn at the attempt start
   __shared__ double F1[kNglobal];
   __shared__ double st[kNglobal];
   const int j = threadIdx.x;
@@ -1036,8 +1148,13 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs a, const Dev d,
   if (j == 0) {
     if (avgp) {
       double other[12];
-      avg_outputs(tot + 5, st, a.nfru, a.scale, other);
-      for (int q = 0; q < 12; ++q) ctl->out_other[(size_t)ctl->avg_row * 12 + q] = other[q];
+      avg_outputs(tot + 5, st, ci.nfru, ci.scale, other);
+      if (ctl->out_other)
+        for (int q = 0; q < 12; ++q) ctl->out_other[(size_t)(ctl->avg_row - ctl->ms_base) * 12 + q] = other[q];
+      if (ctl->avg_row == ctl->pend_row) {
+        ctl->pend_JSR = other[1];
+        beat_take_sample(ctl);
+      }
       ctl->avg_pending = 0;
     }
     if (active) {
@@ -1045,7 +1162,7 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs a, const Dev d,
         ctl->resume = 0;
       } else if (success) {  // rk_pd54m.c:93-108
         double J[4];
-        flux_currents(tot, st[0], st[6], st[7], a.nfru, a.scale, J);
+        flux_currents(tot, st[0], st[6], st[7], ci.nfru, ci.scale, J);
         double cur[kNcur];
         fcn(ctl->start, st, F1, cur, ctl->keepc != 0, J, ctl->prot);
         if (ctl->keepc)
@@ -1082,8 +1199,16 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs a, const Dev d,
   }
 }
 
+// One RK stage of every cell: the Monte Carlo sub-interval of its FRUs (four
+// lanes per FRU, 64 FRUs per block, blocks map to cells through d.smap), the
+// flux reduction, and in the cell's last block fcn for this stage and then
+// the next stage or the step decision (rk_pd54m.c:124-333) and the caf.c
+// output loop (caf.c:150-251).
 template <class R>
-__global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctl) {
+__global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctls) {
+  const int cell = d.smap[blockIdx.x];
+  const CellInfo ci = d.cells[cell];
+  RkCtl* ctl = ctls + cell;
   if (!ctl->active) return;
   SimArgs a = base;
   a.t0 = ctl->ta;
@@ -1092,13 +1217,18 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     a.dep0[q] = ctl->dep0[q];
     a.depf[q] = ctl->depf[q];
   }
+  a.nfru = ci.nfru;
+  a.f0 = ci.f0;
+  a.b0 = ci.sb0;
+  a.k23c = ci.k23c;
+  a.gcf = ci.gcf;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   count_steps(d, simfru4_body<R>(a, d, acc));
   __shared__ double tot[5];
-  if (!reduce_grid_block<5>(acc, d, 0, tot)) return;
+  if (!reduce_range<5>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)) return;
   const unsigned long long tail0 = wall_clock64();
-  if (threadIdx.x == 0) close_launch_stats(d);
-  // ---- last block: fcn of this stage and the next stage / the step decision
+  if (threadIdx.x == 0 && base.ncells == 1) close_launch_stats(d);
+  // ---- last block of the cell: fcn of this stage and the next stage / the step decision
   __shared__ double st[kNglobal], yv[kNglobal], F[kNglobal], kk[6][kNglobal], ev[kNglobal];
   __shared__ int decision;
   const int j = threadIdx.x;
@@ -1119,7 +1249,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   __syncthreads();
   if (j == 0) {
     double J[4];
-    flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], a.nfru, a.scale, J);  // mpi_master.c:401-468
+    flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], ci.nfru, ci.scale, J);  // mpi_master.c:401-468
     fcn(a.t1, yv, F, nullptr, false, J, ctl->prot);
     for (int q = 0; q < 4; ++q) ctl->J[q] = J[q];
   }
@@ -1222,36 +1352,46 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     ctl->state[j] = y;
   }
   __syncthreads();
-  if (j == 0 && ctl->log_n < ctl->log_cap) {
-    double* L = ctl->out_log + 4 * (size_t)ctl->log_n;
-    L[0] = ctl->start; L[1] = yv[0]; L[2] = yv[6]; L[3] = ctl->J[2];
-    ctl->log_n++;
+  if (j == 0) {
+    if (ctl->log_n < ctl->log_cap) {
+      double* L = ctl->out_log + 4 * (size_t)ctl->log_n;
+      L[0] = ctl->start; L[1] = yv[0]; L[2] = yv[6]; L[3] = ctl->J[2];
+      ctl->log_n++;
+    }
+    beat_update(ctl, ctl->start, yv[0], yv[6], ctl->J[2]);
   }
   if (decision != 2) return;
   // ---- the caf.c loop: record this call's outputs, set up the next call
   const int row = ctl->ms_done;
-  if (j < kNglobal) ctl->out_states[(size_t)row * kNglobal + j] = yv[j];
-  if (j < kNcur) ctl->out_cur[(size_t)row * kNcur + j] = ctl->cur[j];
+  const size_t orow = (size_t)(row - ctl->ms_base);
+  if (ctl->out_states && j < kNglobal) ctl->out_states[orow * kNglobal + j] = yv[j];
+  if (ctl->out_cur && j < kNcur) ctl->out_cur[orow * kNcur + j] = ctl->cur[j];
   __syncthreads();
   if (j == 0) {
     const GrwProtocol& p = ctl->prot;
+    ctl->pend_t = ctl->tstep;
+    ctl->pend_row = row;
+    ctl->pend_V = yv[0];
+    ctl->pend_Cai = yv[6];
+    ctl->pend_NSR = yv[7];
+    ctl->pend_JSR = __builtin_nan("");
+    beat_take_sample(ctl);
     ctl->ms_done = row + 1;
     ctl->avg_pending = 1;
     ctl->avg_row = row;
-    if (row + 1 >= ctl->ms_target) {
-      ctl->active = 0;
-    } else {
-      const double tc = ctl->tstep;  // the next call starts on the output grid (caf.c:150)
-      ctl->tcall = tc;
-      ctl->tstep = tc + p.time_step;
-      double hn = rmin(ctl->oh, ctl->tstep - tc);
-      hn = rmin(p.step_max, hn);
-      ctl->h = hn;
-      ctl->oh = hn;
-      ctl->start = tc;
-      ctl->success = 1;
-      ctl->keepc = 1;
-    }
+    // the next call starts on the output grid (caf.c:150) with the step of
+    // rk_pd54m.c:71-74; a later run continues from here
+    const double tc = ctl->tstep;
+    ctl->tcall = tc;
+    ctl->tstep = tc + p.time_step;
+    double hn = rmin(ctl->oh, ctl->tstep - tc);
+    hn = rmin(p.step_max, hn);
+    ctl->h = hn;
+    ctl->oh = hn;
+    ctl->start = tc;
+    ctl->success = 1;
+    ctl->keepc = 1;
+    if (row + 1 >= ctl->ms_target) ctl->active = 0;
   }
 }
 
@@ -1356,21 +1496,51 @@ __global__ void k_resume(const SimArgs a, const Dev d) {
   if (i < a.nfru) fru_rollback(a, d, i);
 }
 
-// initialize_ran.c:23-36 + sgrnd_local (mt19937_local.c:75-92) per FRU.
-__global__ void k_seed(const SimArgs a, const Dev d, uint32_t a0) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.nfru) return;
-  const long long j = i;
-  const long long v = (long long)(int)a0 + 23 * (j * (j + 1) / 2) + 4093 * j * j + 1;
+
+// One thread per FRU of every cell (blocks map to cells through d.bmap).
+__device__ __forceinline__ int cell_fru(const Dev& d, int& cell) {
+  cell = d.bmap[blockIdx.x];
+  const CellInfo ci = d.cells[cell];
+  const int il = (blockIdx.x - ci.bb0) * blockDim.x + threadIdx.x;
+  return il < ci.nfru ? il : -1;
+}
+
+// Every cell starts from the same initial FRU set of nic records, FRU j taking
+// record j mod nic (initialize_state.c:38-41,62-65).
+__global__ void k_load_ic(const SimArgs a, const Dev d, const double* icc, const uint32_t* ich, int nic) {
+  int cell;
+  const int il = cell_fru(d, cell);
+  if (il < 0) return;
+  const int f = d.cells[cell].f0 + il, r = il % nic;
+  for (int k = 0; k < 5; ++k) d.conc[k * a.ld + f] = icc[k * nic + r];
+  for (int c = 0; c < 4; ++c) d.chan[c * a.ld + f] = ich[c * nic + r];
+}
+
+// initialize_ran.c:23-36 + sgrnd_local (mt19937_local.c:75-92) for every FRU
+// j of every cell, with the cell's master seed.
+__global__ void k_seed_cells(const SimArgs a, const Dev d) {
+  int cell;
+  const int il = cell_fru(d, cell);
+  if (il < 0) return;
+  const CellInfo ci = d.cells[cell];
+  const long long j = il;
+  const long long v = (long long)(int)ci.seed + 23 * (j * (j + 1) / 2) + 4093 * j * j + 1;
   uint32_t x = (uint32_t)(unsigned long long)v;
-  uint32_t* row = d.mt + (size_t)i * kMtStride;
+  const int f = ci.f0 + il;
+  uint32_t* row = d.mt + (size_t)f * kMtStride;
   row[0] = x;
   for (uint32_t k = 1; k < kMtN; ++k) {
     x = 1812433253u * (x ^ (x >> 30)) + k;
     row[k] = x;
   }
-  d.mtidx[i] = kMtN;  // the first draw regenerates word 0
-  d.nreg[i] = 0;
+  d.mtidx[f] = kMtN;  // the first draw regenerates word 0
+  d.nreg[f] = 0;
+}
+
+// Checkpoint of every FRU slot of the arrays (padding slots included).
+__global__ void k_save_all(const SimArgs a, const Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.ld) fru_checkpoint(a, d, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -1402,6 +1572,15 @@ struct grw_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> evpool;  // per-launch timing of device-driven runs
   std::vector<void*> allocs;
+  // cells: one for a context made by grw_create, many for grw_batch_create
+  int ncells = 1;
+  bool batch = false;
+  std::vector<CellInfo> cells;
+  int grid_b = 0, grid_s = 0;   // integrator grids: one thread per FRU / 64 FRUs per block
+  RkCtl* dctl = nullptr;        // [ncells] integrator control blocks, persistent across runs
+  std::vector<RkCtl> hctl;      // host copies
+  double* dbeats = nullptr;     // [ncells][beat_cap][kBeatStats]
+  int beat_cap = 0;
 };
 
 namespace {
@@ -1470,6 +1649,197 @@ int run_sim(grw_ctx* c, const SimArgs& a, bool reduce) {
   return 0;
 }
 
+// Device arrays for the FRUs of all cells (c->ld slots, c->grid_s Monte Carlo blocks).
+int alloc_fru_state(grw_ctx* c) {
+  int rc = 0;
+  const size_t ld = c->ld;
+  const size_t nslots = std::max<size_t>({(size_t)c->grid_sim, (size_t)c->grid_s, (size_t)c->grid_b,
+                                          (size_t)c->grid});
+  if ((rc = dalloc(c, &c->d.conc, 5 * ld)) || (rc = dalloc(c, &c->d.chan, 4 * ld)) ||
+      (rc = dalloc(c, &c->d.mtidx, ld)) || (rc = dalloc(c, &c->d.nreg, ld)) ||
+      (rc = dalloc(c, &c->d.mt, ld * kMtStride)) || (rc = dalloc(c, &c->d.mtw_h, ld)) ||
+      (rc = dalloc(c, &c->d.conc_h, 5 * ld)) || (rc = dalloc(c, &c->d.chan_h, 4 * ld)) ||
+      (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, nslots * 16)) ||
+      (rc = dalloc(c, &c->d.ticket, (size_t)c->ncells)) || (rc = dalloc(c, &c->d.out, 32)) ||
+      (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
+      (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
+      (rc = dalloc(c, &c->d.tailticks, 1)))
+    return rc;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_out, 32 * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) return fail(GRW_EDEVICE, std::string("stream/event setup failed: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Cell placement: cell k's FRUs start at a multiple of 64; block maps for the
+// two integrator grids; control blocks and beat records.  Sets c->ld, grids.
+int place_cells(grw_ctx* c, std::vector<CellInfo>& cells) {
+  int f0 = 0, sb = 0, bb = 0;
+  for (CellInfo& ci : cells) {
+    ci.f0 = f0;
+    ci.sb0 = sb;
+    ci.snb = (ci.nfru + 63) / 64;
+    ci.bb0 = bb;
+    ci.bnb = (ci.nfru + 255) / 256;
+    f0 += ci.snb * 64;
+    sb += ci.snb;
+    bb += ci.bnb;
+  }
+  c->cells = cells;
+  c->ncells = (int)cells.size();
+  c->ld = f0;
+  c->grid_s = sb;
+  c->grid_b = bb;
+  return 0;
+}
+
+int upload_cells(grw_ctx* c) {
+  const int nc = c->ncells;
+  std::vector<int> smap(c->grid_s), bmap(c->grid_b);
+  for (int k = 0; k < nc; ++k) {
+    const CellInfo& ci = c->cells[k];
+    for (int b = 0; b < ci.snb; ++b) smap[ci.sb0 + b] = k;
+    for (int b = 0; b < ci.bnb; ++b) bmap[ci.bb0 + b] = k;
+  }
+  CellInfo* dc = nullptr;
+  int* ds = nullptr;
+  int* db = nullptr;
+  int rc = 0;
+  if ((rc = dalloc(c, &dc, (size_t)nc)) || (rc = dalloc(c, &ds, (size_t)c->grid_s)) ||
+      (rc = dalloc(c, &db, (size_t)c->grid_b)) || (rc = dalloc(c, &c->dctl, (size_t)nc)) ||
+      (rc = dalloc(c, &c->dbeats, (size_t)nc * (size_t)c->beat_cap * kBeatStats)))
+    return rc;
+  HIPCHK(hipMemcpy(dc, c->cells.data(), sizeof(CellInfo) * nc, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ds, smap.data(), sizeof(int) * smap.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(db, bmap.data(), sizeof(int) * bmap.size(), hipMemcpyHostToDevice));
+  c->d.cells = dc;
+  c->d.smap = ds;
+  c->d.bmap = db;
+  c->hctl.assign(nc, RkCtl{});
+  return 0;
+}
+
+int refresh_cells(grw_ctx* c) {
+  HIPCHK(hipMemcpy((void*)c->d.cells, c->cells.data(), sizeof(CellInfo) * c->ncells, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Forget the integrator's clock and beat tracking (a new initial state).
+void reset_tracking(grw_ctx* c) {
+  for (int k = 0; k < c->ncells; ++k) {
+    RkCtl& h = c->hctl[k];
+    h.ms_done = 0;
+    h.beat_cap = c->beat_cap;
+    h.beat_n = 0;
+    h.beat_cur = -1;
+    h.pend_row = -1;
+    h.prev_t = h.prev_V = h.pend_t = h.pend_V = h.pend_Cai = h.pend_NSR = h.pend_JSR = NAN;
+    for (int q = 0; q < kBeatStats; ++q) h.bt[q] = NAN;
+    h.out_beats = c->dbeats + (size_t)k * c->beat_cap * kBeatStats;
+  }
+}
+
+// Enqueue RK attempts (k_dev_begin + five k_dev_stage) until every active cell
+// of c->hctl has reached its ms_target; the host synchronises once per batch of
+// attempts sized to ~25 simulated ms of the slowest cell, and launches after a
+// cell is done return at once for it.
+int run_device(grw_ctx* c) {
+  const int nc = c->ncells;
+  HIPCHK(hipMemcpyAsync(c->dctl, c->hctl.data(), sizeof(RkCtl) * nc, hipMemcpyHostToDevice, c->stream));
+  SimArgs base = base_args(c);
+  base.ncells = nc;
+  const bool mixed = c->cfg.precision == GRW_MIXED;
+  std::vector<long long> att_prev(nc, 0);
+  std::vector<int> ms_prev(nc);
+  std::vector<double> ms_per_att(nc, 1.0 / 40.0);
+  for (int k = 0; k < nc; ++k) ms_prev[k] = c->hctl[k].ms_done;
+  std::vector<hipEvent_t>& evs = c->evpool;
+  while (true) {
+    int K = 0;
+    for (int k = 0; k < nc; ++k) {
+      const RkCtl& h = c->hctl[k];
+      if (!h.active) continue;
+      const int chunk = std::min(h.ms_target - h.ms_done, 25);
+      K = std::max(K, (int)std::ceil(chunk / ms_per_att[k] * 1.1) + 2);
+    }
+    if (K == 0) break;
+    K = std::max(4, std::min(K, 4000));
+    if (c->timing) {
+      while ((int)evs.size() < 10 * K) {
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) break;
+        evs.push_back(ev);
+      }
+    }
+    const bool timed = c->timing && (int)evs.size() >= 10 * K;
+    for (int k = 0; k < K; ++k) {
+      hipLaunchKernelGGL(k_dev_begin, dim3(c->grid_b), dim3(256), 0, c->stream, base, c->d, c->dctl);
+      for (int s5 = 0; s5 < 5; ++s5) {
+        if (timed) (void)hipEventRecord(evs[10 * k + 2 * s5], c->stream);
+        if (mixed)
+          hipLaunchKernelGGL(k_dev_stage<float>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+        else
+          hipLaunchKernelGGL(k_dev_stage<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+        if (timed) (void)hipEventRecord(evs[10 * k + 2 * s5 + 1], c->stream);
+      }
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->hctl.data(), c->dctl, sizeof(RkCtl) * nc, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(GRW_EDEVICE, std::string("device run failed: ") + hipGetErrorString(e));
+    int kerr = 0;
+    HIPCHK(hipMemcpy(&kerr, c->d.err, sizeof(int), hipMemcpyDeviceToHost));
+    long long eff = 0;
+    for (int k = 0; k < nc; ++k) {
+      const RkCtl& h = c->hctl[k];
+      bool finite = true;
+      for (int q = 0; q < kNglobal; ++q) finite = finite && std::isfinite(h.state[q]);
+      const std::string who = nc > 1 ? " (cell " + std::to_string(k) + ")" : std::string();
+      if (!finite) return fail(GRW_ESTATE, "global state became non-finite" + who);
+      if (h.attempts > 20000LL * (long long)(h.ms_target - h.ms_base) + 100000LL)
+        return fail(GRW_ESTATE, "integrator made no progress (step size collapsed)" + who);
+      if (kerr || h.err)
+        return fail(GRW_ESTATE, "kernel met an unknown channel state, a non-finite rate or a runaway step count" + who);
+      const long long done = h.attempts - att_prev[k];
+      eff = std::max(eff, done);
+      c->stats.simfru_calls += (uint64_t)(5 * done) * (uint64_t)c->cells[k].nfru;
+      att_prev[k] = h.attempts;
+      if (h.ms_done > ms_prev[k] && done > 0) ms_per_att[k] = (double)(h.ms_done - ms_prev[k]) / (double)done;
+      ms_prev[k] = h.ms_done;
+    }
+    if (timed) {
+      for (long long k = 0; k < std::min<long long>(eff, K); ++k)
+        for (int s5 = 0; s5 < 5; ++s5) {
+          float ms = 0.f;
+          if (hipEventElapsedTime(&ms, evs[10 * k + 2 * s5], evs[10 * k + 2 * s5 + 1]) == hipSuccess)
+            c->stats.kernel_ms += ms;
+        }
+    }
+    c->stats.launches += (uint64_t)(5 * eff);
+  }
+  bool avg = false;
+  for (const RkCtl& h : c->hctl) avg = avg || h.avg_pending;
+  if (avg) {  // averages of the last output sample
+    hipLaunchKernelGGL(k_dev_begin, dim3(c->grid_b), dim3(256), 0, c->stream, base, c->d, c->dctl);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->hctl.data(), c->dctl, sizeof(RkCtl) * nc, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(GRW_EDEVICE, std::string("device run failed: ") + hipGetErrorString(e));
+  }
+  return 0;
+}
+
+// The per-FRU ABI calls act on one cell.
+int single(const grw_ctx* c) {
+  if (!c) return fail(GRW_EINVAL, "null context");
+  if (c->ncells != 1) return fail(GRW_EINVAL, "this call acts on a single-cell context (grw_create)");
+  return 0;
+}
+
 // Sub-interval bounds the kernel can finish: finite, ordered, at most 1 s long.
 int check_interval(double t0, double t1, const double* dep0, const double* depf) {
   if (!std::isfinite(t0) || !std::isfinite(t1) || t1 < t0 || t1 - t0 > 1000.0)
@@ -1532,32 +1902,24 @@ int grw_create(const grw_config* cfg, grw_ctx** out) {
   grw_ctx* c = new grw_ctx();
   c->cfg = *cfg;
   c->n = cfg->nfru;
-  c->ld = (cfg->nfru + 63) / 64 * 64;
   c->block = blk;
   c->grid = (c->n + blk - 1) / blk;
   c->grid_sim = cfg->lanes_per_fru == 4 ? (4 * c->n + blk - 1) / blk : c->grid;
-  int rc = 0;
-  const size_t ld = c->ld, n = c->n;
-  if ((rc = dalloc(c, &c->d.conc, 5 * ld)) || (rc = dalloc(c, &c->d.chan, 4 * ld)) ||
-      (rc = dalloc(c, &c->d.mtidx, ld)) || (rc = dalloc(c, &c->d.nreg, ld)) ||
-      (rc = dalloc(c, &c->d.mt, n * kMtStride)) || (rc = dalloc(c, &c->d.mtw_h, ld)) ||
-      (rc = dalloc(c, &c->d.conc_h, 5 * ld)) || (rc = dalloc(c, &c->d.chan_h, 4 * ld)) ||
-      (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, (size_t)c->grid_sim * 16)) ||
-      (rc = dalloc(c, &c->d.ticket, 1)) || (rc = dalloc(c, &c->d.out, 32)) ||
-      (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
-      (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
-      (rc = dalloc(c, &c->d.tailticks, 1))) {
+  std::vector<CellInfo> one(1);
+  one[0].nfru = c->n;
+  one[0].scale = cfg->nfru_scale;
+  one[0].k23c = cfg->k23_scale != 1.0 ? k23cf * cfg->k23_scale : k23cf;
+  one[0].gcf = cfg->gamma_scale != 1.0 ? gammacf * cfg->gamma_scale : gammacf;
+  one[0].seed = 0;
+  place_cells(c, one);
+  c->beat_cap = 4096;
+  int rc = alloc_fru_state(c);
+  if (!rc) rc = upload_cells(c);
+  if (rc) {
     grw_destroy(c);
     return rc;
   }
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_out, 32 * sizeof(double), hipHostMallocDefault);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
-  if (e != hipSuccess) {
-    grw_destroy(c);
-    return fail(GRW_EDEVICE, std::string("stream/event setup failed: ") + hipGetErrorString(e));
-  }
+  reset_tracking(c);
   *out = c;
   return 0;
 }
@@ -1577,7 +1939,8 @@ int grw_destroy(grw_ctx* c) {
 
 int grw_load_state(grw_ctx* c, const double* fru, const int32_t* ltype, const int32_t* ryr,
                    const int32_t* ito2, const int32_t* mti, const uint32_t* mt) {
-  if (!c || !fru || !ltype || !ryr || !ito2 || !mti || !mt) return fail(GRW_EINVAL, "null argument");
+  if (int rc = single(c)) return rc;
+  if (!fru || !ltype || !ryr || !ito2 || !mti || !mt) return fail(GRW_EINVAL, "null argument");
   const int n = c->n, ld = c->ld;
   std::vector<double> conc(5 * (size_t)ld, 0.0);
   std::vector<uint32_t> chan(4 * (size_t)ld, 0u);
@@ -1605,12 +1968,13 @@ int grw_load_state(grw_ctx* c, const double* fru, const int32_t* ltype, const in
   HIPCHK(hipMemsetAsync(c->d.nreg, 0, (size_t)ld * sizeof(int32_t), c->stream));
   HIPCHK(hipMemcpyAsync(c->d.mt, rows.data(), rows.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  reset_tracking(c);
   return grw_save_state(c);
 }
 
 int grw_get_state(grw_ctx* c, double* fru, int32_t* ltype, int32_t* ryr, int32_t* ito2,
                   int32_t* mti, uint32_t* mt) {
-  if (!c) return fail(GRW_EINVAL, "null context");
+  if (int rc = single(c)) return rc;
   const int n = c->n, ld = c->ld;
   std::vector<double> conc(5 * (size_t)ld);
   std::vector<uint32_t> chan(4 * (size_t)ld);
@@ -1661,16 +2025,20 @@ int grw_get_state(grw_ctx* c, double* fru, int32_t* ltype, int32_t* ryr, int32_t
 }
 
 int grw_seed(grw_ctx* c, uint32_t a0) {
-  if (!c) return fail(GRW_EINVAL, "null context");
+  if (int rc = single(c)) return rc;
+  c->cells[0].seed = a0;
+  if (int rc = refresh_cells(c)) return rc;
   SimArgs a = base_args(c);
-  hipLaunchKernelGGL(k_seed, dim3(c->grid), dim3(c->block), 0, c->stream, a, c->d, a0);
+  hipLaunchKernelGGL(k_seed_cells, dim3(c->grid_b), dim3(256), 0, c->stream, a, c->d);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
+  reset_tracking(c);
   return grw_save_state(c);
 }
 
 int grw_simfru_local(grw_ctx* c, double t0, double t1, const double dep0[3], const double depf[3]) {
-  if (!c || !dep0 || !depf) return fail(GRW_EINVAL, "null argument");
+  if (int rc = single(c)) return rc;
+  if (!dep0 || !depf) return fail(GRW_EINVAL, "null argument");
   if (int rc0 = check_interval(t0, t1, dep0, depf)) return rc0;
   SimArgs a = base_args(c);
   a.t0 = t0;
@@ -1686,7 +2054,8 @@ int grw_simfru_local(grw_ctx* c, double t0, double t1, const double dep0[3], con
 
 int grw_distrib_simfru(grw_ctx* c, double t0, double t1, const double dep0[3], const double depf[3],
                        double out4[4]) {
-  if (!c || !dep0 || !depf || !out4) return fail(GRW_EINVAL, "null argument");
+  if (int rc = single(c)) return rc;
+  if (!dep0 || !depf || !out4) return fail(GRW_EINVAL, "null argument");
   if (int rc0 = check_interval(t0, t1, dep0, depf)) return rc0;
   SimArgs a = base_args(c);
   a.t0 = t0;
@@ -1703,14 +2072,16 @@ int grw_distrib_simfru(grw_ctx* c, double t0, double t1, const double dep0[3], c
 }
 
 int grw_calc_flux(grw_ctx* c, const double dep[3], double out4[4]) {
-  if (!c || !dep || !out4) return fail(GRW_EINVAL, "null argument");
+  if (int rc = single(c)) return rc;
+  if (!dep || !out4) return fail(GRW_EINVAL, "null argument");
   int rc = flux_now(c, dep);
   if (rc) return rc;
   return fetch_out(c, 0, 4, out4);
 }
 
 int grw_calc_fru_flux_local(grw_ctx* c, double num5[5]) {
-  if (!c || !num5) return fail(GRW_EINVAL, "null argument");
+  if (int rc = single(c)) return rc;
+  if (!num5) return fail(GRW_EINVAL, "null argument");
   const double dep[3] = {-80.0, 1e-4, 1.0};
   int rc = flux_now(c, dep);
   if (rc) return rc;
@@ -1718,7 +2089,8 @@ int grw_calc_fru_flux_local(grw_ctx* c, double num5[5]) {
 }
 
 int grw_calc_fru_avg_local(grw_ctx* c, double num11[11]) {
-  if (!c || !num11) return fail(GRW_EINVAL, "null argument");
+  if (int rc = single(c)) return rc;
+  if (!num11) return fail(GRW_EINVAL, "null argument");
   SimArgs a = base_args(c);
   if (c->cfg.sequential_sums)
     hipLaunchKernelGGL(k_seq_sums, dim3(1), dim3(1), 0, c->stream, a, c->d);
@@ -1729,7 +2101,8 @@ int grw_calc_fru_avg_local(grw_ctx* c, double num11[11]) {
 }
 
 int grw_calc_avg(grw_ctx* c, const double state[37], double other[12]) {
-  if (!c || !state || !other) return fail(GRW_EINVAL, "null argument");
+  if (int rc = single(c)) return rc;
+  if (!state || !other) return fail(GRW_EINVAL, "null argument");
   double num[11];
   int rc = grw_calc_fru_avg_local(c, num);
   if (rc) return rc;
@@ -1738,7 +2111,7 @@ int grw_calc_avg(grw_ctx* c, const double state[37], double other[12]) {
 }
 
 int grw_save_state(grw_ctx* c) {
-  if (!c) return fail(GRW_EINVAL, "null context");
+  if (int rc = single(c)) return rc;
   SimArgs a = base_args(c);
   hipLaunchKernelGGL(k_save, dim3(c->grid), dim3(c->block), 0, c->stream, a, c->d);
   HIPCHK(hipGetLastError());
@@ -1746,7 +2119,7 @@ int grw_save_state(grw_ctx* c) {
 }
 
 int grw_resume_state(grw_ctx* c) {
-  if (!c) return fail(GRW_EINVAL, "null context");
+  if (int rc = single(c)) return rc;
   SimArgs a = base_args(c);
   hipLaunchKernelGGL(k_resume, dim3(c->grid), dim3(c->block), 0, c->stream, a, c->d);
   HIPCHK(hipGetLastError());
@@ -1790,22 +2163,18 @@ int grw_set_timing(grw_ctx* c, int on) {
 int grw_run_cell_device(grw_ctx* c, const GrwProtocol* p, const double errw[37], double t0,
                         int nsteps, double state[37], double* h, double* states, double* currents,
                         double* other, double* log, int log_cap, int* log_n, GrwRkStats* stt) {
-  if (!c || !p || !errw || !state || !h || nsteps < 0 || log_cap < 0)
+  if (int rc = single(c)) return rc;
+  if (!p || !errw || !state || !h || nsteps < 0 || log_cap < 0)
     return fail(GRW_EINVAL, "bad argument to grw_run_cell_device");
   if (nsteps == 0) return 0;
-  if (!std::isfinite(t0) || !(p->time_step > 0.0) || !(p->step_min > 0.0) || !(p->step_max >= p->step_min))
+  if (!std::isfinite(t0) || !(p->time_step > 0.0) || !(p->step_min > 0.0) || !(p->step_max >= p->step_min) ||
+      !(p->period > 0.0))
     return fail(GRW_EINVAL, "bad protocol time controls");
   const size_t ns = (size_t)nsteps, lc = (size_t)std::max(log_cap, 1);
-  RkCtl* dctl = nullptr;
   double* dbuf = nullptr;
-  HIPCHK(hipMalloc((void**)&dctl, sizeof(RkCtl)));
   hipError_t e0 = hipMalloc((void**)&dbuf, sizeof(double) * (ns * (kNglobal + kNcur + 12) + lc * 4));
-  if (e0 != hipSuccess) {
-    (void)hipFree(dctl);
-    return fail(GRW_EDEVICE, std::string("hipMalloc failed: ") + hipGetErrorString(e0));
-  }
-  RkCtl hc;
-  std::memset(&hc, 0, sizeof hc);
+  if (e0 != hipSuccess) return fail(GRW_EDEVICE, std::string("hipMalloc failed: ") + hipGetErrorString(e0));
+  RkCtl& hc = c->hctl[0];  // keeps the output clock and the beat tracking of earlier runs
   hc.prot = *p;
   for (int q = 0; q < kNglobal; ++q) {
     hc.errw[q] = errw[q];
@@ -1818,135 +2187,215 @@ int grw_run_cell_device(grw_ctx* c, const GrwProtocol* p, const double errw[37],
   h0 = (p->step_max < h0) ? p->step_max : h0;
   hc.h = h0;
   hc.oh = h0;
+  hc.stage = 0;
   hc.success = 1;
   hc.keepc = 1;
+  hc.resume = 0;
   hc.active = 1;
-  hc.ms_target = nsteps;
+  hc.avg_pending = 0;
+  hc.ms_base = hc.ms_done;
+  hc.ms_target = hc.ms_done + nsteps;
+  hc.log_n = 0;
   hc.log_cap = log ? log_cap : 0;
+  hc.err = 0;
+  hc.attempts = hc.rejects = hc.forced = 0;
   hc.out_states = dbuf;
   hc.out_cur = dbuf + ns * kNglobal;
   hc.out_other = hc.out_cur + ns * kNcur;
   hc.out_log = hc.out_other + ns * 12;
-  int rc = 0;
-  auto cleanup = [&]() {
-    (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(dctl);
-    (void)hipFree(dbuf);
-  };
-  hipError_t e = hipMemcpyAsync(dctl, &hc, sizeof hc, hipMemcpyHostToDevice, c->stream);
-  if (e != hipSuccess) {
-    cleanup();
-    return fail(GRW_EDEVICE, std::string("ctl upload failed: ") + hipGetErrorString(e));
-  }
-  const SimArgs base = base_args(c);
-  const int blk = 256;
-  const int grid_b = (c->n + blk - 1) / blk;
-  const int grid_s = (4 * c->n + blk - 1) / blk;
-  const bool mixed = c->cfg.precision == GRW_MIXED;
-  std::vector<hipEvent_t>& evs = c->evpool;
-  long long att_prev = 0;
-  double ms_per_attempt = 1.0 / 40.0;
-  int ms_prev = 0;
-  while (true) {
-    const int remaining = nsteps - ms_prev;
-    const int chunk = std::min(remaining, 25);
-    int K = (int)std::ceil(chunk / ms_per_attempt * 1.1) + 2;
-    K = std::max(4, std::min(K, 4000));
-    if (c->timing) {
-      while ((int)evs.size() < 10 * K) {
-        hipEvent_t ev;
-        if (hipEventCreate(&ev) != hipSuccess) break;
-        evs.push_back(ev);
+  hc.out_beats = c->dbeats;
+  hc.beat_cap = c->beat_cap;
+  int rc = run_device(c);
+  if (!rc) {
+    std::vector<double> hb(ns * (kNglobal + kNcur + 12) + (size_t)hc.log_n * 4);
+    hipError_t e = hipMemcpy(hb.data(), dbuf, sizeof(double) * hb.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = fail(GRW_EDEVICE, std::string("output download failed: ") + hipGetErrorString(e));
+    } else {
+      if (states) std::memcpy(states, hb.data(), sizeof(double) * ns * kNglobal);
+      if (currents) std::memcpy(currents, hb.data() + ns * kNglobal, sizeof(double) * ns * kNcur);
+      if (other) std::memcpy(other, hb.data() + ns * (kNglobal + kNcur), sizeof(double) * ns * 12);
+      if (log && log_n) {
+        const int add = std::min(hc.log_n, log_cap - *log_n);
+        if (add > 0)
+          std::memcpy(log + 4 * (size_t)(*log_n), hb.data() + ns * (kNglobal + kNcur + 12),
+                      sizeof(double) * 4 * (size_t)add);
+        *log_n += std::max(add, 0);
+      }
+      for (int q = 0; q < kNglobal; ++q) state[q] = hc.state[q];
+      *h = hc.oh;
+      if (stt) {
+        stt->attempts += hc.attempts;
+        stt->rejects += hc.rejects;
+        stt->forced += hc.forced;
       }
     }
-    const bool timed = c->timing && (int)evs.size() >= 10 * K;
-    for (int k = 0; k < K; ++k) {
-      hipLaunchKernelGGL(k_dev_begin, dim3(grid_b), dim3(blk), 0, c->stream, base, c->d, dctl);
-      for (int s5 = 0; s5 < 5; ++s5) {
-        if (timed) (void)hipEventRecord(evs[10 * k + 2 * s5], c->stream);
-        if (mixed)
-          hipLaunchKernelGGL(k_dev_stage<float>, dim3(grid_s), dim3(blk), 0, c->stream, base, c->d, dctl);
-        else
-          hipLaunchKernelGGL(k_dev_stage<double>, dim3(grid_s), dim3(blk), 0, c->stream, base, c->d, dctl);
-        if (timed) (void)hipEventRecord(evs[10 * k + 2 * s5 + 1], c->stream);
-      }
-    }
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(&hc, dctl, sizeof hc, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-      cleanup();
-      return fail(GRW_EDEVICE, std::string("device run failed: ") + hipGetErrorString(e));
-    }
-    int kerr = 0;
-    (void)hipMemcpy(&kerr, c->d.err, sizeof(int), hipMemcpyDeviceToHost);
-    bool finite = true;
-    for (int q = 0; q < kNglobal; ++q) finite = finite && std::isfinite(hc.state[q]);
-    if (!finite) {
-      cleanup();
-      return fail(GRW_ESTATE, "global state became non-finite");
-    }
-    if (hc.attempts > 20000LL * (long long)nsteps + 100000LL) {
-      cleanup();
-      return fail(GRW_ESTATE, "integrator made no progress (step size collapsed)");
-    }
-    if (kerr || hc.err) {
-      cleanup();
-      return fail(GRW_ESTATE, "kernel met an unknown channel state, a non-finite rate or a runaway step count");
-    }
-    const long long done_att = hc.attempts - att_prev;
-    if (timed) {
-      const long long eff = std::min<long long>(done_att, K);
-      for (long long k = 0; k < eff; ++k)
-        for (int s5 = 0; s5 < 5; ++s5) {
-          float ms = 0.f;
-          if (hipEventElapsedTime(&ms, evs[10 * k + 2 * s5], evs[10 * k + 2 * s5 + 1]) == hipSuccess)
-            c->stats.kernel_ms += ms;
-        }
-    }
-    c->stats.launches += (uint64_t)(5 * done_att);
-    c->stats.simfru_calls += (uint64_t)(5 * done_att) * (uint64_t)c->n;
-    att_prev = hc.attempts;
-    if (hc.ms_done > ms_prev && done_att > 0)
-      ms_per_attempt = (double)(hc.ms_done - ms_prev) / (double)done_att;
-    ms_prev = hc.ms_done;
-    if (!hc.active) break;
   }
-  if (hc.avg_pending) {
-    hipLaunchKernelGGL(k_dev_begin, dim3(grid_b), dim3(blk), 0, c->stream, base, c->d, dctl);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(&hc, dctl, sizeof hc, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-      cleanup();
-      return fail(GRW_EDEVICE, std::string("device run failed: ") + hipGetErrorString(e));
-    }
-  }
-  std::vector<double> hb(ns * (kNglobal + kNcur + 12) + (size_t)hc.log_n * 4);
-  e = hipMemcpy(hb.data(), dbuf, sizeof(double) * hb.size(), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    cleanup();
-    return fail(GRW_EDEVICE, std::string("output download failed: ") + hipGetErrorString(e));
-  }
-  if (states) std::memcpy(states, hb.data(), sizeof(double) * ns * kNglobal);
-  if (currents) std::memcpy(currents, hb.data() + ns * kNglobal, sizeof(double) * ns * kNcur);
-  if (other) std::memcpy(other, hb.data() + ns * (kNglobal + kNcur), sizeof(double) * ns * 12);
-  if (log && log_n) {
-    const int add = std::min(hc.log_n, log_cap - *log_n);
-    if (add > 0)
-      std::memcpy(log + 4 * (size_t)(*log_n), hb.data() + ns * (kNglobal + kNcur + 12),
-                  sizeof(double) * 4 * (size_t)add);
-    *log_n += std::max(add, 0);
-  }
-  for (int q = 0; q < kNglobal; ++q) state[q] = hc.state[q];
-  *h = hc.oh;
-  if (stt) {
-    stt->attempts += hc.attempts;
-    stt->rejects += hc.rejects;
-    stt->forced += hc.forced;
-  }
-  cleanup();
+  hc.active = 0;
+  hc.out_states = hc.out_cur = hc.out_other = hc.out_log = nullptr;
+  hc.log_cap = 0;
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(dbuf);
   return rc;
+}
+
+int grw_get_beats(grw_ctx* c, double* out, int cap, int* n, int include_current) {
+  return grw_batch_get_beats(c, 0, out, cap, n, include_current);
+}
+
+// ---- batches of independent cells (include/grw_batch.h)
+int grw_batch_create(const grw_batch_config* cfg, const grw_cell_spec* spec, grw_ctx** out) {
+  if (!cfg || !spec || !out) return fail(GRW_EINVAL, "null argument");
+  *out = nullptr;
+  if (cfg->ncells <= 0 || cfg->ncells > (1 << 16)) return fail(GRW_EINVAL, "ncells out of range");
+  if (cfg->precision != GRW_FP64 && cfg->precision != GRW_MIXED)
+    return fail(GRW_EINVAL, "precision must be GRW_FP64 or GRW_MIXED");
+  if (cfg->beat_cap < 0 || cfg->beat_cap > (1 << 20)) return fail(GRW_EINVAL, "beat_cap out of range");
+  std::vector<CellInfo> cells(cfg->ncells);
+  long long total = 0;
+  for (int k = 0; k < cfg->ncells; ++k) {
+    const grw_cell_spec& sp = spec[k];
+    if (sp.nfru <= 0 || sp.nfru > (1 << 24)) return fail(GRW_EINVAL, "cell " + std::to_string(k) + ": nfru out of range");
+    const GrwProtocol& p = sp.prot;
+    if (!(p.time_step > 0.0) || !(p.step_min > 0.0) || !(p.step_max >= p.step_min) || !(p.period > 0.0) ||
+        !(p.tolrk > 0.0) || !std::isfinite(p.shift))
+      return fail(GRW_EINVAL, "cell " + std::to_string(k) + ": bad protocol time controls");
+    if (!(sp.k23_scale > 0.0) || !(sp.gamma_scale > 0.0) || !(sp.nfru_scale >= 0.0) || !(p.serca_scale > 0.0))
+      return fail(GRW_EINVAL, "cell " + std::to_string(k) + ": scales must be positive");
+    cells[k].nfru = sp.nfru;
+    cells[k].scale = sp.nfru_scale > 0.0 ? sp.nfru_scale : 12500.0 / (double)sp.nfru;
+    cells[k].k23c = sp.k23_scale != 1.0 ? k23cf * sp.k23_scale : k23cf;
+    cells[k].gcf = sp.gamma_scale != 1.0 ? gammacf * sp.gamma_scale : gammacf;
+    cells[k].seed = sp.seed;
+    total += sp.nfru;
+  }
+  if (total > (1LL << 28)) return fail(GRW_EINVAL, "too many FRUs in the batch");
+  HIPCHK(hipSetDevice(cfg->device));
+  grw_ctx* c = new grw_ctx();
+  grw_config_default(&c->cfg);
+  c->cfg.nfru = (int)total;
+  c->cfg.precision = cfg->precision;
+  c->cfg.device = cfg->device;
+  c->batch = true;
+  c->n = (int)total;
+  c->block = 256;
+  place_cells(c, cells);
+  c->grid = c->grid_sim = c->grid_s;
+  c->beat_cap = cfg->beat_cap;
+  int rc = alloc_fru_state(c);
+  if (!rc) rc = upload_cells(c);
+  if (rc) {
+    grw_destroy(c);
+    return rc;
+  }
+  for (int k = 0; k < c->ncells; ++k) c->hctl[k].prot = spec[k].prot;
+  reset_tracking(c);
+  *out = c;
+  return 0;
+}
+
+int grw_batch_load_ic(grw_ctx* c, const double* states, const double errw[37], int nic, const double* fru,
+                      const int32_t* ltype, const int32_t* ryr, const int32_t* ito2) {
+  if (!c || !states || !errw || !fru || !ltype || !ryr || !ito2) return fail(GRW_EINVAL, "null argument");
+  if (nic <= 0) return fail(GRW_EINVAL, "empty initial FRU set");
+  std::vector<double> icc(5 * (size_t)nic);
+  std::vector<uint32_t> ich(4 * (size_t)nic);
+  for (int i = 0; i < nic; ++i) {
+    for (int k = 0; k < 5; ++k) icc[(size_t)k * nic + i] = fru[5 * (size_t)i + k];
+    for (int cl = 0; cl < 4; ++cl) {
+      const int lcc = ltype[8 * (size_t)i + 2 * cl], y = ltype[8 * (size_t)i + 2 * cl + 1];
+      const int32_t* rr = ryr + 20 * (size_t)i + 5 * cl;
+      const int it = ito2[4 * (size_t)i + cl];
+      if (lcc < 1 || lcc > 12 || (y != 1 && y != 2) || (it != 1 && it != 2))
+        return fail(GRW_EINVAL, "invalid LCC / gate / Ito2 state at FRU " + std::to_string(i));
+      for (int r = 0; r < 5; ++r)
+        if (rr[r] < 1 || rr[r] > 8) return fail(GRW_EINVAL, "invalid RyR state at FRU " + std::to_string(i));
+      ich[(size_t)cl * nic + i] = ch_pack(lcc, y, rr, it);
+    }
+  }
+  for (int k = 0; k < c->ncells; ++k)
+    for (int q = 0; q < kNglobal; ++q)
+      if (!std::isfinite(states[(size_t)k * kNglobal + q]))
+        return fail(GRW_EINVAL, "non-finite initial state of cell " + std::to_string(k));
+  double* dic = nullptr;
+  HIPCHK(hipMalloc((void**)&dic, icc.size() * sizeof(double) + ich.size() * sizeof(uint32_t)));
+  uint32_t* dich = (uint32_t*)(dic + icc.size());
+  hipError_t e = hipMemcpy(dic, icc.data(), icc.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dich, ich.data(), ich.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  SimArgs a = base_args(c);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_load_ic, dim3(c->grid_b), dim3(256), 0, c->stream, a, c->d, dic, dich, nic);
+    hipLaunchKernelGGL(k_seed_cells, dim3(c->grid_b), dim3(256), 0, c->stream, a, c->d);
+    hipLaunchKernelGGL(k_save_all, dim3((c->ld + 255) / 256), dim3(256), 0, c->stream, a, c->d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dic);
+  if (e != hipSuccess) return fail(GRW_EDEVICE, std::string("initial state upload failed: ") + hipGetErrorString(e));
+  reset_tracking(c);
+  for (int k = 0; k < c->ncells; ++k) {
+    RkCtl& hc = c->hctl[k];
+    const GrwProtocol& p = hc.prot;
+    for (int q = 0; q < kNglobal; ++q) {
+      hc.errw[q] = errw[q];
+      hc.state[q] = states[(size_t)k * kNglobal + q];
+    }
+    if (p.vclamp) hc.state[0] = p.vclamp_hold;  // initialize_state.c:222-224
+    hc.tcall = hc.start = 0.0;
+    hc.tstep = p.time_step;
+    hc.h = hc.oh = std::min(p.step_max, p.time_step);
+    hc.success = 1;
+    hc.keepc = 1;
+    hc.resume = 0;
+    hc.stage = 0;
+    hc.active = 0;
+  }
+  return 0;
+}
+
+int grw_batch_run(grw_ctx* c, const int32_t* nsteps, double* states, GrwRkStats* stt) {
+  if (!c || !nsteps) return fail(GRW_EINVAL, "null argument");
+  for (int k = 0; k < c->ncells; ++k) {
+    if (nsteps[k] < 0) return fail(GRW_EINVAL, "negative step count for cell " + std::to_string(k));
+    RkCtl& hc = c->hctl[k];
+    hc.active = nsteps[k] > 0;
+    hc.ms_base = hc.ms_done;
+    hc.ms_target = hc.ms_done + nsteps[k];
+    hc.attempts = hc.rejects = hc.forced = 0;
+    hc.err = 0;
+    hc.log_cap = hc.log_n = 0;
+    hc.out_states = hc.out_cur = hc.out_other = hc.out_log = nullptr;
+  }
+  int rc = run_device(c);
+  if (rc) return rc;
+  for (int k = 0; k < c->ncells; ++k) {
+    const RkCtl& hc = c->hctl[k];
+    if (states)
+      for (int q = 0; q < kNglobal; ++q) states[(size_t)k * kNglobal + q] = hc.state[q];
+    if (stt) {
+      stt[k].attempts += hc.attempts;
+      stt[k].rejects += hc.rejects;
+      stt[k].forced += hc.forced;
+    }
+  }
+  return 0;
+}
+
+int grw_batch_get_beats(grw_ctx* c, int cell, double* out, int cap, int* n, int include_current) {
+  if (!c || !n || (cap > 0 && !out)) return fail(GRW_EINVAL, "null argument");
+  if (cell < 0 || cell >= c->ncells) return fail(GRW_EINVAL, "cell index out of range");
+  const RkCtl& hc = c->hctl[cell];
+  const int done = std::min(std::min(hc.beat_n, c->beat_cap), std::max(cap, 0));
+  if (done > 0)
+    HIPCHK(hipMemcpy(out, c->dbeats + (size_t)cell * c->beat_cap * kBeatStats,
+                     sizeof(double) * (size_t)done * kBeatStats, hipMemcpyDeviceToHost));
+  int m = done;
+  if (include_current && hc.beat_cur >= 0 && m < cap) {
+    for (int q = 0; q < kBeatStats; ++q) out[(size_t)m * kBeatStats + q] = hc.bt[q];
+    ++m;
+  }
+  *n = m;
+  return 0;
 }
 
 }  // extern "C"

@@ -73,6 +73,26 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
                         double *currents, double *other, double *log, int log_cap, int *log_n,
                         GrwRkStats *st);
 
+/* Per-beat statistics of the device-driven runs of ctx since its state was
+ * last loaded or seeded (the reference writes only 1 ms samples,
+ * data_io.c:22-140; these come from every accepted RK step).  Beat b starts at
+ * the stimulus t_s = shift + b * period and holds the accepted steps ending in
+ * [t_s, t_s + period).  Each record is GRW_BEAT_STATS doubles:
+ *   0 t_s (ms)             1 V at the t_s output sample (rest, mV)
+ *   2 peak V               3 time of peak V - t_s
+ *   4 APD90: time - t_s at which V, after its peak, falls through
+ *     V_peak - 0.9 (V_peak - V_rest), linear between accepted steps (NaN if
+ *     not within the beat, or if the peak is under 20 mV above rest)
+ *   5 peak Cai (mM)        6 time of peak Cai - t_s
+ *   7 Cai at t_s           8 peak (most negative) ICa of the FRU flux (uA/uF)
+ *   9 CaNSR at t_s (SR load, mM)
+ *  10 mean CaJSR at t_s (JSR load, send_calc_fru_avg output 1, mM)
+ *  11 accepted steps in the beat
+ * Writes up to cap finished beats (and, if include_current, the beat in
+ * progress) to out and their count to *n. */
+#define GRW_BEAT_STATS 12
+int grw_get_beats(grw_ctx *ctx, double *out, int cap, int *n, int include_current);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,3 +109,46 @@ def test_total_ca_matches_reference_initialisation(golden):
     g = golden("ic_stable.npz")
     d = load_stable(1250, 10.0)
     assert d["state"][31] == g["state"][31]
+
+
+def test_beat_stats_closed_form():
+    """beat_stats on a piecewise-linear action potential whose APD90, peaks and
+    rest values are known in closed form."""
+    from greenstein2002_amd.beats import beat_stats
+
+    shift, period = 10.0, 100.0
+    # rest -90 mV; upstroke 10 -> 12 ms to +30; linear repolarisation to -90 at 92 ms
+    t = np.concatenate([np.arange(1.0, 10.01, 1.0), np.arange(10.5, 12.01, 0.5), np.arange(13.0, 92.01, 1.0),
+                        np.arange(93.0, 140.01, 1.0)])
+
+    def V(x):
+        if x <= 10.0:
+            return -90.0
+        if x <= 12.0:
+            return -90.0 + 60.0 * (x - 10.0)
+        if x <= 92.0:
+            return 30.0 - 1.5 * (x - 12.0)
+        return -90.0
+
+    Vs = np.array([V(x) for x in t])
+    cai = 1e-4 + 1e-3 * np.exp(-((t - 40.0) / 10.0) ** 2)
+    ica = -np.exp(-((t - 11.0) / 1.0) ** 2)
+    log = np.stack([t, Vs, cai, ica], axis=1)
+    rows_t = np.arange(1.0, 141.0)
+    states = np.zeros((len(rows_t), 37))
+    other = np.zeros((len(rows_t), 12))
+    states[:, 0] = [V(x) for x in rows_t]
+    states[:, 6] = 1e-4 + 1e-3 * np.exp(-((rows_t - 40.0) / 10.0) ** 2)
+    states[:, 7] = 0.7
+    other[:, 1] = 0.65
+    rec = beat_stats(log, rows_t, states, other, shift, period)
+    assert rec.shape == (2, 12)
+    b = rec[0]
+    assert b[0] == 10.0 and b[1] == -90.0 and b[2] == 30.0 and b[3] == 2.0
+    # threshold 30 - 0.9 * 120 = -78 mV reached at 12 + 108/1.5 = 84 ms
+    assert abs(b[4] - (84.0 - 10.0)) < 1e-12
+    assert b[5] == cai.max() and b[6] == 40.0 - 10.0
+    assert b[8] == ica.min() and b[9] == 0.7 and b[10] == 0.65
+    assert b[11] == np.count_nonzero((t >= 10.0) & (t < 110.0))
+    # the second (partial) beat has no action potential: no APD90
+    assert rec[1][0] == 110.0 and np.isnan(rec[1][4])

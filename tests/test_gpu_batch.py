@@ -1,0 +1,75 @@
+"""Batches of independent cells (include/grw_batch.h) and the per-beat
+statistics of the device integrator.
+
+A cell in a batch runs the same kernels on the same block structure as a
+single-cell context (its FRUs start at a multiple of 64 and its reductions span
+only its own blocks), so its trajectory must equal the single-cell device run
+with the same parameters bit for bit, whatever the other cells do.
+"""
+import numpy as np
+import pytest
+
+from greenstein2002_amd.batch import Batch, CellSpec
+from greenstein2002_amd.beats import beat_stats
+from greenstein2002_amd.cell import Cell, Protocol
+
+pytestmark = pytest.mark.gpu
+
+
+def single_run(sp: CellSpec, ms: int, precision="fp64"):
+    with Cell(sp.nfru, seed=sp.seed, precision=precision, protocol=sp.protocol,
+              k23_scale=sp.k23_scale, gamma_scale=sp.gamma_scale, nfru_scale=sp.nfru_scale) as c:
+        r = c.run(ms, log_cap=0)
+        return r.states[-1], r.stats["attempts"], r.stats["rejects"], c.beats()
+
+
+def test_batch_cells_equal_single_cell_runs():
+    """Uneven cells with their own seeds, pacing, SERCA, RyR and LCC scales."""
+    specs = [
+        CellSpec(250, seed=11),
+        CellSpec(300, seed=12, k23_scale=1.6, protocol=Protocol(period=300.0, shift=5.0)),
+        CellSpec(64, seed=13, gamma_scale=0.6, protocol=Protocol(serca_scale=0.8)),
+        CellSpec(513, seed=14, nfru_scale=30.0, protocol=Protocol(shift=2.0)),
+    ]
+    ms = 22
+    with Batch(specs) as b:
+        b.run(10)
+        b.run([12, 12, 0, 12])   # cell 2 idles, then catches up alone
+        b.run([0, 0, 12, 0])
+        got = b.state.copy()
+        att = b.attempts.copy()
+        rej = b.rejects.copy()
+        beats = [b.beats(k) for k in range(len(specs))]
+    for k, sp in enumerate(specs):
+        st, a, r, bt = single_run(sp, ms)
+        assert np.array_equal(got[k], st), f"cell {k} state differs"
+        assert att[k] == a and rej[k] == r, f"cell {k} attempts {att[k]}/{a} rejects {rej[k]}/{r}"
+        assert np.array_equal(beats[k], bt, equal_nan=True), f"cell {k} beat records differ"
+
+
+def test_mixed_batch_runs():
+    specs = [CellSpec(200, seed=s) for s in range(3)]
+    with Batch(specs, precision="mixed") as b:
+        st = b.run(15)
+    assert np.all(np.isfinite(st)) and np.all(st[:, 0] > 0.0)  # all three fired at 10 ms
+
+
+def test_device_beat_records_match_host_reduction():
+    """The online per-beat statistics equal beat_stats() of the same run's
+    accepted-step log and 1 ms samples (three short beats, run in chunks)."""
+    prot = Protocol(period=120.0, shift=10.0)
+    with Cell(250, seed=5, protocol=prot) as c:
+        parts = [c.run(n, log_cap=200000) for n in (50, 130, 200)]
+        dev = c.beats()
+    log = np.concatenate([p.log for p in parts])
+    t = np.concatenate([p.t for p in parts])
+    states = np.concatenate([p.states for p in parts])
+    other = np.concatenate([p.other for p in parts])
+    host = beat_stats(log, t, states, other, prot.shift, prot.period)
+    assert dev.shape == (4, 12)
+    assert np.array_equal(dev, host, equal_nan=True)
+    # three full beats with an action potential each; the first one, started from
+    # the 1 Hz steady state, outlasts the 120 ms cycle, the paced ones repolarise
+    assert np.all(dev[:3, 2] > 0.0)
+    assert np.all((dev[1:3, 4] > 20.0) & (dev[1:3, 4] < 120.0))
+    assert np.all(dev[:3, 8] < 0.0)  # inward ICa
