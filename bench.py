@@ -12,10 +12,20 @@ stimulus, the Ca release and diastole.
     python bench.py [--gpus 1] [--steps 1000] [--warmup 5]
 
 Prints one JSON line.  ``roofline`` is measured for the dominant kernel (the
-FRU Monte Carlo sub-interval kernel) from HIP events recorded around every
-launch on the library's stream; ``cpu_baseline`` times the CPU oracle (a
-single-threaded restatement of the reference, bit-exact with it) on a bounded
-sample of the same workload on this host.
+FRU Monte Carlo sub-interval kernel with the integrator stage in its tail) from
+HIP events recorded around every launch on the library's stream;
+``cpu_baseline`` times the CPU oracle (a single-threaded restatement of the
+reference, bit-exact with it) on a bounded sample of the same workload on this
+host.  Two further measurements share the line:
+
+* ``mixed``: the same beat with the mixed-precision form (FP32 rates and local
+  Ca right-hand side), and its speed relative to FP64;
+* ``batched``: 256 independent cells in one batch, FRU counts uniform in
+  [1250, 12500] and pacing cycle lengths uniform in [300, 2000] ms (host PRNG
+  seed 1234, master seeds 1234 + k), over the first --batch-ms simulated ms
+  (each cell's first stimulus at 10 ms included); beats/s counts simulated
+  cycle lengths completed per wall second, and a sample of the same cells run
+  one after another through the single-cell driver gives the sequential rate.
 """
 from __future__ import annotations
 
@@ -52,6 +62,14 @@ def parse():
                     help="simulated ms of the CPU-oracle sample (0 disables)")
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--batch-cells", type=int, default=256, help="cells of the batched run (0 skips it)")
+    ap.add_argument("--batch-ms", type=int, default=30, help="simulated ms of the batched run")
+    ap.add_argument("--batch-seq-sample", type=int, default=8,
+                    help="cells of the batch also run one at a time for the sequential rate")
+    ap.add_argument("--summarize", default="",
+                    help="only summarise a rocprofv3 output folder (kernel trace and / or counter "
+                         "CSVs) per kernel and grid size, as markdown")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (5 times): the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -59,18 +77,65 @@ def parse():
 
 
 def calibrate_copy(mib: int) -> None:
-    """A plain copy kernel of known size whose HBM counters calibrate the
-    profiler's FETCH_SIZE / WRITE_SIZE (reads and writes mib MiB per launch)."""
-    import torch
+    """A plain device-to-device copy of known size (hipMemcpy, five times) whose
+    HBM counters calibrate the profiler's FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
 
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
-    b = torch.empty_like(a)
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib * (1 << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, C.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMemset failed")
     for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"calibration": "copy", "bytes_read_per_launch": n * 4,
-                      "bytes_written_per_launch": n * 4, "launches": 5}), flush=True)
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration": "copy", "bytes_read_per_launch": nbytes,
+                      "bytes_written_per_launch": nbytes, "launches": 5}), flush=True)
+
+
+def summarize(folder: str) -> None:
+    """Per (kernel, grid size): dispatches and mean duration from
+    *kernel_trace.csv, mean counter values per dispatch from
+    *counter_collection.csv (rocprofv3 --output-format csv)."""
+    import csv
+    import glob
+    from collections import defaultdict
+
+    def short(name):
+        name = name.replace("(anonymous namespace)::", "")
+        return name.split("(")[0].replace("void ", "")
+
+    for path in sorted(glob.glob(os.path.join(folder, "**", "*kernel_trace.csv"), recursive=True)):
+        acc = defaultdict(list)
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                grid = int(r["Grid_Size_X"]) // max(int(r["Workgroup_Size_X"]), 1)
+                acc[(short(r["Kernel_Name"]), grid)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        print(f"\n{os.path.relpath(path, folder)}\n")
+        print("| kernel | workgroups | dispatches | mean us | min us | max us | total ms |")
+        print("|---|---|---|---|---|---|---|")
+        for (k, g), v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+            print(f"| {k} | {g} | {len(v)} | {sum(v) / len(v):.2f} | {min(v):.2f} | {max(v):.2f} | {sum(v) / 1e3:.2f} |")
+    for path in sorted(glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True)):
+        acc = defaultdict(lambda: defaultdict(list))
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                grid = int(r["Grid_Size"]) // max(int(r["Workgroup_Size"]), 1)
+                acc[(short(r["Kernel_Name"]), grid)][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        print(f"\n{os.path.relpath(path, folder)}\n")
+        names = sorted({c for v in acc.values() for c in v})
+        print("| kernel | workgroups | dispatches | " + " | ".join(f"mean {c}" for c in names) + " |")
+        print("|---|---|---|" + "---|" * len(names))
+        for (k, g), v in sorted(acc.items()):
+            n = max(len(x) for x in v.values())
+            print(f"| {k} | {g} | {n} | " + " | ".join(
+                f"{sum(v[c]) / len(v[c]):.1f}" if v.get(c) else "" for c in names) + " |")
 
 
 def cpu_baseline(nfru: int, ms: float, seed: int):
@@ -110,8 +175,95 @@ def cpu_baseline(nfru: int, ms: float, seed: int):
                        f"{steps} FRU-steps in {dt:.1f} s = {dt / max(steps, 1) * 1e9:.0f} ns/FRU-step")
 
 
+def run_cell(args, precision, timing):
+    """The single-cell beat: warmup, then args.steps ms timed in 100 ms chunks."""
+    from greenstein2002_amd.cell import Cell, Protocol
+
+    prot = Protocol()
+    cell = Cell(args.nfru, seed=args.seed, precision=precision, protocol=prot, block=args.block,
+                lanes_per_fru=args.lanes)
+    if args.warmup:
+        cell.run(args.warmup, outputs=False, log_cap=0, driver=args.driver)
+    cell.fru.set_timing(timing)
+    cell.fru.reset_stats()
+    chunks = []
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        n = min(100, args.steps - done)
+        chunks.append(cell.run(n, log_cap=0, driver=args.driver))
+        done += n
+        print(f"[bench] {precision}: t = {args.warmup + done} ms simulated, {time.perf_counter() - t0:.1f} s wall",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    st = cell.fru.stats()
+    beats = cell.beats() if args.driver == "device" else None
+    cell.close()
+    return wall, chunks, st, beats
+
+
+def batched(args):
+    """256 uneven cells in one batch vs the same cells one at a time."""
+    import numpy as np
+
+    from greenstein2002_amd.batch import Batch, CellSpec
+    from greenstein2002_amd.cell import Cell, Protocol
+
+    n = args.batch_cells
+    rng = np.random.default_rng(1234)
+    nf = rng.integers(1250, 12501, n)
+    cl = rng.uniform(300.0, 2000.0, n)
+    specs = [CellSpec(int(nf[k]), seed=1234 + k, protocol=Protocol(period=float(cl[k]))) for k in range(n)]
+    b = Batch(specs, precision=args.precision)
+    b.set_timing(True)
+    b.reset_stats()
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.batch_ms:
+        m = min(10, args.batch_ms - done)
+        b.run(m)
+        done += m
+        print(f"[bench] batch of {n}: t = {done} ms simulated, {time.perf_counter() - t0:.1f} s wall",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    st = b.stats()
+    att = int(b.attempts.sum())
+    b.close()
+    alg = BYTES_PER_STEP * st["fru_steps"] + BYTES_PER_CALL * st["simfru_calls"]
+    achieved = alg / (st["kernel_ms"] / 1e3) / 1e9 if st["kernel_ms"] else 0.0
+    # the same cells one at a time (an evenly spaced sample of them)
+    pick = np.unique(np.linspace(0, n - 1, max(1, min(args.batch_seq_sample, n))).astype(int))
+    seq = []
+    for k in pick:
+        c = Cell(int(nf[k]), seed=1234 + int(k), precision=args.precision, protocol=Protocol(period=float(cl[k])))
+        ts = time.perf_counter()
+        c.run(args.batch_ms, outputs=False, log_cap=0)
+        seq.append(time.perf_counter() - ts)
+        c.close()
+    seq_wall = float(np.mean(seq)) * n
+    beats = float(np.sum(args.batch_ms / cl))
+    return dict(
+        workload=f"{n} cells, FRUs uniform in [1250, 12500] (total {int(nf.sum())}), cycle lengths uniform "
+                 f"in [300, 2000] ms, t = 0..{args.batch_ms} ms from ic/stable, {args.precision}",
+        cells=n, frus=int(nf.sum()), window_ms=args.batch_ms, wall_s=round(wall, 3),
+        cell_ms_per_s=round(n * args.batch_ms / wall, 2), beats_per_s=round(beats / wall, 4),
+        rk_attempts=att, fru_steps=st["fru_steps"],
+        fru_steps_per_s=round(st["fru_steps"] / wall, 1),
+        roofline=dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=round(achieved / HBM_PEAK_GBS, 6),
+                      kernel="k_dev_stage<%s> over all cells" % ("double" if args.precision == "fp64" else "float"),
+                      avg_launch_us=round(st["kernel_ms"] * 1e3 / max(st["launches"], 1), 3),
+                      launches=st["launches"]),
+        sequential=dict(sample_cells=[int(k) for k in pick], sample_walls_s=[round(x, 3) for x in seq],
+                        est_wall_s=round(seq_wall, 2), beats_per_s=round(beats / seq_wall, 4)),
+        speedup_vs_sequential=round(seq_wall / wall, 2))
+
+
 def main():
     args = parse()
+    if args.summarize:
+        summarize(args.summarize)
+        return
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
         return
@@ -121,35 +273,16 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from greenstein2002_amd.cell import Cell, Protocol
-
-    prot = Protocol()
-    cell = Cell(args.nfru, seed=args.seed, precision=args.precision, protocol=prot, block=args.block,
-                lanes_per_fru=args.lanes)
-    if args.warmup:
-        cell.run(args.warmup, outputs=False, log_cap=0, driver=args.driver)
-    if not args.no_timing:
-        cell.fru.set_timing(True)
-    cell.fru.reset_stats()
     import numpy as np
 
-    chunks = []
-    t0 = time.perf_counter()
-    done = 0
-    while done < args.steps:
-        n = min(100, args.steps - done)
-        chunks.append(cell.run(n, log_cap=0, driver=args.driver))
-        done += n
-        print(f"[bench] t = {args.warmup + done} ms simulated, {time.perf_counter() - t0:.1f} s wall",
-              file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t0
+    from greenstein2002_amd.beats import as_dicts
+
+    wall, chunks, st, beats = run_cell(args, args.precision, not args.no_timing)
     states = np.concatenate([c.states for c in chunks])
     attempts = sum(c.stats["attempts"] for c in chunks)
     rejects = sum(c.stats["rejects"] for c in chunks)
-    st = cell.fru.stats()
-    cell.close()
 
-    value = args.steps * prot.time_step / wall
+    value = args.steps / wall
     roof = None
     if not args.no_timing and st["launches"]:
         alg_bytes = BYTES_PER_STEP * st["fru_steps"] + BYTES_PER_CALL * st["simfru_calls"]
@@ -193,6 +326,21 @@ def main():
                    "peak_Cai_uM": round(float(states[:, 6].max()) * 1e3, 4)},
         "roofline": roof,
     }
+    if beats is not None and len(beats):
+        line["config"]["beat"] = {k: (round(v, 6) if v == v else None) for k, v in as_dicts(beats[:1])[0].items()}
+    if not args.no_mixed and args.precision == "fp64":
+        mw, mchunks, mst, mbeats = run_cell(args, "mixed", False)
+        mbeat = as_dicts(mbeats[:1])[0] if mbeats is not None and len(mbeats) else {}
+        fbeat = as_dicts(beats[:1])[0] if beats is not None and len(beats) else {}
+        line["mixed"] = dict(value=round(args.steps / mw, 4), unit="simulated ms / s",
+                             speedup_vs_fp64=round(wall / mw, 4),
+                             fru_steps=mst["fru_steps"],
+                             beat_vs_fp64={k: (round(mbeat[k] - fbeat[k], 6) if mbeat[k] == mbeat[k]
+                                               and fbeat[k] == fbeat[k] else None)
+                                           for k in ("APD90", "V_peak", "Cai_peak", "ICa_peak")}
+                             if mbeat and fbeat else None)
+    if args.batch_cells > 0:
+        line["batched"] = batched(args)
     if args.cpu_ms > 0:
         line["cpu_baseline"] = cpu_baseline(args.nfru, args.cpu_ms, args.seed)
         line["cpu_baseline"]["value"] = round(line["cpu_baseline"]["value"], 5)
