@@ -1086,6 +1086,24 @@ __device__ __forceinline__ void stage_bounds(int s, double start, double h, doub
   }
 }
 
+// fcn (fcn.c:29-1072) for the integrator tails, by a whole block: thread 0
+// prepares the arguments of its transcendental calls, the lanes of the first
+// wavefront evaluate them side by side (one exp, one log and one pow call for
+// all), thread 0 finishes.  The values are those of the one-thread fcn.  Call
+// from every thread; J, st and cur are read / written by thread 0 only.
+__device__ __forceinline__ void fcn_block(double t, double* st, double* F, double* cur, bool keepc, const double* J,
+                          const GrwProtocol& p, FcnTrans* q) {
+  const int j = threadIdx.x;
+  if (j == 0) fcn_prep(t, st, p, *q);
+  __syncthreads();
+  if (j < kFcnExp) q->ex[j] = exp(q->ex_arg[j]);
+  if (j < kFcnLog) q->lg[j] = log(q->lg_arg[j]);
+  if (j < kFcnPow) q->pw[j] = pow(q->pw_base[j], q->pw_exp[j]);
+  __syncthreads();
+  if (j == 0) fcn_finish(t, st, F, cur, keepc, J, p, *q);
+  __syncthreads();
+}
+
 // Attempt start of every cell: rollback or checkpoint, the flux sums (and the
 // average sums of a finished output sample), then in the cell's last block
 // fcn at the attempt start and the first stage (rk_pd54m.c:93-138).  One
@@ -1142,7 +1160,12 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
   // ---- last block of the cell: outputs of the finished call, fcn at the attempt start
   __shared__ double F1[kNglobal];
   __shared__ double st[kNglobal];
+  __shared__ double Jsh[4], cursh[kNcur];
+  __shared__ FcnTrans fq;
   const int j = threadIdx.x;
+  const bool eval = active && !resume && success;  // rk_pd54m.c:93-108
+  const bool keepc = ctl->keepc != 0;
+  const double tstart = ctl->start;
   if (j < kNglobal) st[j] = ctl->state[j];
   __syncthreads();
   if (j == 0) {
@@ -1157,21 +1180,20 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
       }
       ctl->avg_pending = 0;
     }
-    if (active) {
-      if (resume) {
-        ctl->resume = 0;
-      } else if (success) {  // rk_pd54m.c:93-108
-        double J[4];
-        flux_currents(tot, st[0], st[6], st[7], ci.nfru, ci.scale, J);
-        double cur[kNcur];
-        fcn(ctl->start, st, F1, cur, ctl->keepc != 0, J, ctl->prot);
-        if (ctl->keepc)
-          for (int q = 0; q < kNcur; ++q) ctl->cur[q] = cur[q];
-        ctl->keepc = 0;
-        for (int q = 0; q < 4; ++q) ctl->J[q] = J[q];
-      }
-      ctl->attempts++;
+    if (eval) flux_currents(tot, st[0], st[6], st[7], ci.nfru, ci.scale, Jsh);
+  }
+  __syncthreads();
+  if (eval) fcn_block(tstart, st, F1, cursh, keepc, Jsh, ctl->prot, &fq);
+  if (j == 0 && active) {
+    if (resume) {
+      ctl->resume = 0;
+    } else if (success) {
+      if (keepc)
+        for (int q = 0; q < kNcur; ++q) ctl->cur[q] = cursh[q];
+      ctl->keepc = 0;
+      for (int q = 0; q < 4; ++q) ctl->J[q] = Jsh[q];
     }
+    ctl->attempts++;
   }
   __syncthreads();
   if (!active) return;
@@ -1230,6 +1252,8 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   if (threadIdx.x == 0 && base.ncells == 1) close_launch_stats(d);
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
   __shared__ double st[kNglobal], yv[kNglobal], F[kNglobal], kk[6][kNglobal], ev[kNglobal];
+  __shared__ double Jsh[4];
+  __shared__ FcnTrans fq;
   __shared__ int decision;
   const int j = threadIdx.x;
   const int s = ctl->stage;
@@ -1247,13 +1271,11 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     for (int q = 0; q < s; ++q) kk[q][j] = ctl->k[q][j];
   }
   __syncthreads();
-  if (j == 0) {
-    double J[4];
-    flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], ci.nfru, ci.scale, J);  // mpi_master.c:401-468
-    fcn(a.t1, yv, F, nullptr, false, J, ctl->prot);
-    for (int q = 0; q < 4; ++q) ctl->J[q] = J[q];
-  }
+  if (j == 0) flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], ci.nfru, ci.scale, Jsh);  // mpi_master.c:401-468
   __syncthreads();
+  fcn_block(a.t1, yv, F, nullptr, false, Jsh, ctl->prot, &fq);
+  if (j == 0)
+    for (int q = 0; q < 4; ++q) ctl->J[q] = Jsh[q];
   if (j < kNglobal) {
     kk[s][j] = h * F[j];
     ctl->k[s][j] = kk[s][j];

@@ -69,10 +69,93 @@ GRW_HD void ito_pack(const double dx[2][5], int i0, double* F) {
   F[i0 + 9] = -dx[1][0] - dx[1][1] - dx[1][2] - dx[1][3] - dx[0][4] - dx[0][0] - dx[0][1] - dx[0][2] - dx[0][3];
 }
 
-// fcn.c:29-1072.  J = {Jxfer, Jtr, ICa, Ito2} from the release units.  Mutates
-// state[1] below -90 mV (:663-669) and state[0] under voltage clamp (:811-814).
-GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, const double J[4],
-                const GrwProtocol& p) {
+// Transcendental calls of fcn.c:29-1072 whose arguments depend on the state
+// or time: their arguments (fcn_prep) and values (fcn_trans).  The host
+// evaluates them one by one; the device integrator evaluates each kind as one
+// call across the lanes of a wavefront (same function, same argument, same
+// value).  Calls with constant arguments and the one that needs EK stay in
+// fcn_finish.
+constexpr int kFcnLog = 4, kFcnPow = 3, kFcnExp = 33;
+struct FcnTrans {
+  double lg_arg[kFcnLog], pw_base[kFcnPow], pw_exp[kFcnPow], ex_arg[kFcnExp];
+  double lg[kFcnLog], pw[kFcnPow], ex[kFcnExp];
+};
+
+// The arguments, with the clamp and the floors of fcn.c:448-483, :552, :562.
+GRW_HD void fcn_prep(double t, const double* state, const GrwProtocol& p, FcnTrans& q) {
+  const double Nai = state[4], Ki = state[5];
+  double Cai = state[6];
+  const double CaNSR = state[7];
+  double V = state[0];
+  if (p.vclamp) {
+    const double von = floor((t - p.vclamp_shift) / p.vclamp_period) * p.vclamp_period;
+    const double voff = von + p.vclamp_duration;
+    if (((t - p.vclamp_shift) >= von) && (von >= 0.0) && ((t - p.vclamp_shift) < voff))
+      V = p.vclamp_set;
+    else
+      V = p.vclamp_hold;
+  }
+  if (Cai <= 0.0) Cai = 1.e-15;
+  q.lg_arg[0] = Nao / Nai;
+  q.lg_arg[1] = Ko / Ki;
+  q.lg_arg[2] = (Ko + 0.01833 * Nao) / (Ki + 0.01833 * Nai);
+  q.lg_arg[3] = Cao / Cai;
+  const double KmNai = 10.0, Kfb = 0.26e-3, Krb = 1.8, Nfb = 0.75, Nrb = 0.75;
+  q.pw_base[0] = KmNai / Nai;
+  q.pw_exp[0] = 1.5;
+  q.pw_base[1] = Cai / Kfb;
+  q.pw_exp[1] = Nfb;
+  q.pw_base[2] = CaNSR / Krb;
+  q.pw_exp[2] = Nrb;
+  const double VF_over_RT = V / RT_over_F;
+  const double eta = 0.35;
+  double* e = q.ex_arg;
+  e[0] = VF_over_RT;
+  e[1] = (7.488 - V) / 5.98;
+  e[2] = -0.1 * VF_over_RT;
+  e[3] = eta * VF_over_RT;
+  e[4] = -0.1 * (V + 47.13);
+  e[5] = -V / 11.0;
+  e[6] = (80 + V) / -6.8;
+  e[7] = 0.079 * V;
+  e[8] = 0.35 * V;
+  e[9] = (V + 10.66) / -11.1;
+  e[10] = 0.2444 * V;
+  e[11] = -0.04391 * V;
+  e[12] = 0.311 * (V + 79.23);
+  e[13] = -0.01052 * V;
+  e[14] = -0.1378 * (V + 40.14);
+  e[15] = -2.535E-7 * V;
+  e[16] = -0.1 * (V + 32);
+  e[17] = 0.03304608038835 * V;
+  e[18] = -0.04306054163980 * V;
+  e[19] = 0.02617412715118 * V;
+  e[20] = -0.02691385498399 * V;
+  e[21] = 0.00568908859717 * V;
+  e[22] = -0.04536642959543 * V;
+  e[23] = 0.00000069808924 * V;
+  e[24] = -(V - 24.7) / 13.6;
+  e[25] = -0.148 * (V - 10.0);
+  e[26] = 0.0687 * (V - 10.0);
+  e[27] = 0.028983 * V;
+  e[28] = -0.0468437 * V;
+  e[29] = -0.000373016 * V;
+  e[30] = 0.00000005374 * V;
+  e[31] = 0.00695 * V;
+  e[32] = -0.08527 * V;
+}
+
+GRW_HD void fcn_trans(FcnTrans& q) {
+  for (int k = 0; k < kFcnLog; ++k) q.lg[k] = log(q.lg_arg[k]);
+  for (int k = 0; k < kFcnPow; ++k) q.pw[k] = pow(q.pw_base[k], q.pw_exp[k]);
+  for (int k = 0; k < kFcnExp; ++k) q.ex[k] = exp(q.ex_arg[k]);
+}
+
+// fcn.c:29-1072 after its transcendental calls: every exp / log / pow whose
+// argument depends on the state is taken from q (fcn_prep + fcn_trans), every
+// other operation is done here in the reference's order.
+GRW_HD void fcn_finish(double t, double* state, double* F, double* cur, bool keepc, const double J[4],
+                       const GrwProtocol& p, const FcnTrans& q) {
   const double Jxfer = J[0], Jtr = J[1], ICa = J[2], Ito2 = J[3];
   const double GKr = 0.029, GKs = 0.035, GK1 = 3.0, GKp = 1.2 * 0.002216, GNa = 12.8;
   const double kNaCa = 0.9 * 0.30, KmNa = 87.5, KmCa = 1.38, KmK1 = 13.0, ksat = 0.2, eta = 0.35;
@@ -116,11 +199,11 @@ GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, con
       V = p.vclamp_hold;
   }
 
-  const double ENa = RT_over_F * log(Nao / Nai);
-  const double EK = RT_over_F * log(Ko / Ki);
-  const double EKs = RT_over_F * log((Ko + 0.01833 * Nao) / (Ki + 0.01833 * Nai));
+  const double ENa = RT_over_F * q.lg[0];
+  const double EK = RT_over_F * q.lg[1];
+  const double EKs = RT_over_F * q.lg[2];
   if (Cai <= 0.0) Cai = 1.e-15;
-  const double ECa = 0.5 * RT_over_F * log(Cao / Cai);
+  const double ECa = 0.5 * RT_over_F * q.lg[3];
 
   if (mNa <= 0.0) mNa = 1.e-15;
   const double INa = GNa * (mNa * mNa * mNa) * hNa * jNa * (V - ENa);
@@ -130,7 +213,7 @@ GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, con
   double IKv43 = GKv43 * k43x[0][4] * (V - EK);
   double VF_over_RT = V / RT_over_F;
   const double VFsq_over_RT = (1000.0 * Faraday) * VF_over_RT;
-  const double exp_VFRT = exp(VF_over_RT);
+  const double exp_VFRT = q.ex[0];
   const double OK14 = k14x[0][4];
   double a1, a2, a3, a4, a5;
   a1 = Ki * exp_VFRT - Ko;
@@ -145,18 +228,18 @@ GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, con
   const double K1_inf = 1.0 / (0.94 + exp(1.76 / RT_over_F * (V - EK)));
   double IK1 = GK1 * Ko / (Ko + KmK1) * K1_inf * (V - EK);
   const double INab = GNab * (V - ENa);
-  const double KpV = 1.0 / (1.0 + exp((7.488 - V) / 5.98));
+  const double KpV = 1.0 / (1.0 + q.ex[1]);
   const double IKp = GKp * KpV * (V - EK);
 
   VF_over_RT = V / RT_over_F;
   const double sigma = (exp(Nao / 67.3) - 1.0) / 7.0;
-  a1 = 1.0 + 0.1245 * exp(-0.1 * VF_over_RT);
+  a1 = 1.0 + 0.1245 * q.ex[2];
   a2 = 0.0365 * sigma / exp_VFRT;
   const double fNaK = 1.0 / (a1 + a2);
   a1 = Ko / (Ko + KmKo);
-  a2 = 1.0 + pow(KmNai / Nai, 1.5);
+  a2 = 1.0 + q.pw[0];
   const double INaK = INaKmax * fNaK * (a1 / a2);
-  const double exp_eta = exp(eta * VF_over_RT);
+  const double exp_eta = q.ex[3];
   a1 = exp_eta * (Nai * Nai * Nai) * Cao;
   a2 = exp_eta / exp_VFRT * (Nao * Nao * Nao) * Cai;
   a3 = 1.0 + ksat * exp_eta / exp_VFRT;
@@ -168,8 +251,8 @@ GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, con
 
   double alpha_m;
   if (fabs(V + 47.13) <= 1.e-4) alpha_m = 0.32 / (0.1 - 0.005 * (V + 47.13));
-  else alpha_m = (0.32 * (V + 47.13)) / (1.0 - exp(-0.1 * (V + 47.13)));
-  const double beta_m = 0.08 * exp(-V / 11.0);
+  else alpha_m = (0.32 * (V + 47.13)) / (1.0 - q.ex[4]);
+  const double beta_m = 0.08 * q.ex[5];
   double dmNa;
   if (V >= -90.0) {
     dmNa = alpha_m * (1.0 - mNa) - beta_m * mNa;
@@ -178,29 +261,29 @@ GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, con
     mNa = alpha_m / (alpha_m + beta_m);
     state[1] = mNa;
   }
-  const double alpha_h = 0.135 * exp((80 + V) / -6.8);
-  const double beta_h = (V < -38.73809636838782) ? 3.56 * exp(0.079 * V) + 310000 * exp(0.35 * V)
-                                                 : 1 / (0.13 * (1 + exp((V + 10.66) / -11.1)));
+  const double alpha_h = 0.135 * q.ex[6];
+  const double beta_h = (V < -38.73809636838782) ? 3.56 * q.ex[7] + 310000 * q.ex[8]
+                                                 : 1 / (0.13 * (1 + q.ex[9]));
   double alpha_j = 0;
   if (V < -37.78) {
-    a1 = -127140 * exp(0.2444 * V);
-    a2 = 3.474E-5 * exp(-0.04391 * V);
-    a3 = 1.0 + exp(0.311 * (V + 79.23));
+    a1 = -127140 * q.ex[10];
+    a2 = 3.474E-5 * q.ex[11];
+    a3 = 1.0 + q.ex[12];
     alpha_j = (a1 - a2) * (V + 37.78) / a3;
   }
   const double beta_j = (V < -39.82600037702883)
-                            ? 0.1212 * exp(-0.01052 * V) / (1.0 + exp(-0.1378 * (V + 40.14)))
-                            : 0.3 * exp(-2.535E-7 * V) / (1 + exp(-0.1 * (V + 32)));
+                            ? 0.1212 * q.ex[13] / (1.0 + q.ex[14])
+                            : 0.3 * q.ex[15] / (1 + q.ex[16]);
   const double dhNa = alpha_h * (1.0 - hNa) - beta_h * hNa;
   const double djNa = alpha_j * (1.0 - jNa) - beta_j * jNa;
 
-  const double C1H_to_C2H = TH * 0.017147641733086 * exp(0.03304608038835 * V);
-  const double C2H_to_C1H = TH * 0.03969328381141 * exp(-0.04306054163980 * V);
-  const double C3H_to_OH = TH * 0.02057448605977 * exp(0.02617412715118 * V);
-  const double OH_to_C3H = TH * 0.00134366604423 * exp(-0.02691385498399 * V);
-  const double OH_to_IH = TH * 0.10666316491288 * exp(0.00568908859717 * V);
-  const double IH_to_OH = TH * 0.00646393910049 * exp(-0.04536642959543 * V);
-  const double C3H_to_IH = TH * 0.00008039374403 * exp(0.00000069808924 * V);
+  const double C1H_to_C2H = TH * 0.017147641733086 * q.ex[17];
+  const double C2H_to_C1H = TH * 0.03969328381141 * q.ex[18];
+  const double C3H_to_OH = TH * 0.02057448605977 * q.ex[19];
+  const double OH_to_C3H = TH * 0.00134366604423 * q.ex[20];
+  const double OH_to_IH = TH * 0.10666316491288 * q.ex[21];
+  const double IH_to_OH = TH * 0.00646393910049 * q.ex[22];
+  const double C3H_to_IH = TH * 0.00008039374403 * q.ex[23];
   const double IH_to_C3H = (OH_to_C3H * IH_to_OH * C3H_to_IH) / (C3H_to_OH * OH_to_IH);
   const double dC1H = C2H_to_C1H * C2H - C1H_to_C2H * C1H;
   const double dC2H = (C1H_to_C2H * C1H + C3H_to_C2H * C3H) - ((C2H_to_C1H + C2H_to_C3H) * C2H);
@@ -208,19 +291,19 @@ GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, con
                       ((C3H_to_IH + C3H_to_OH + C3H_to_C2H) * C3H);
   const double dOH = (C3H_to_OH * C3H + IH_to_OH * IH) - ((OH_to_C3H + OH_to_IH) * OH);
 
-  const double xKs_inf = 1.0 / (1.0 + exp(-(V - 24.7) / 13.6));
+  const double xKs_inf = 1.0 / (1.0 + q.ex[24]);
   if (fabs(V - 10) < 1.e-6) {
     a1 = (7.19e-5) / 0.148;
     a2 = (1.31e-4) / 0.0687;
   } else {
-    a1 = 7.19e-5 * (V - 10.0) / (1.0 - exp(-0.148 * (V - 10.0)));
-    a2 = 1.31e-4 * (V - 10.0) / (exp(0.0687 * (V - 10.0)) - 1.0);
+    a1 = 7.19e-5 * (V - 10.0) / (1.0 - q.ex[25]);
+    a2 = 1.31e-4 * (V - 10.0) / (q.ex[26] - 1.0);
   }
   const double tau_xKs = 1.0 / (a1 + a2);
   const double dxKs = (xKs_inf - xKs) / tau_xKs;
 
-  const double fb = pow(Cai / Kfb, Nfb);
-  const double rb = pow(CaNSR / Krb, Nrb);
+  const double fb = q.pw[1];
+  const double rb = q.pw[2];
   double Jup = KSR * (vmaxf * fb - vmaxr * rb) / (1.0 + fb + rb);
   const double dLTRPNCa = kltrpn_plus * Cai * (1.0 - LTRPNCa) - kltrpn_minus * LTRPNCa;
   const double dHTRPNCa = khtrpn_plus * Cai * (1.0 - HTRPNCa) - khtrpn_minus * HTRPNCa;
@@ -258,14 +341,14 @@ GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, con
     dV = -Itot;
   }
 
-  ItoRates q;
+  ItoRates iq;
   double d43[2][5], d14[2][5];
-  ito_rates(0.543708 * exp(0.028983 * V), 0.080185 * exp(-0.0468437 * V),
-            0.0498424 * exp(-0.000373016 * V), 0.000819482 * exp(0.00000005374 * V), f43, b43, q);
-  ito_chain(k43x, q, d43);
-  ito_rates(6.0 * 0.31551 * exp(0.00695 * V), 6.0 * 0.001966 * exp(-0.08527 * V), 6.0 * 0.0004938,
-            6.0 * 0.0000176, f14, b14, q);
-  ito_chain(k14x, q, d14);
+  ito_rates(0.543708 * q.ex[27], 0.080185 * q.ex[28], 0.0498424 * q.ex[29], 0.000819482 * q.ex[30], f43,
+            b43, iq);
+  ito_chain(k43x, iq, d43);
+  ito_rates(6.0 * 0.31551 * q.ex[31], 6.0 * 0.001966 * q.ex[32], 6.0 * 0.0004938, 6.0 * 0.0000176, f14, b14,
+            iq);
+  ito_chain(k14x, iq, d14);
 
   F[0] = dV; F[1] = dmNa; F[2] = dhNa; F[3] = djNa; F[4] = dNai; F[5] = dKi; F[6] = dCai;
   F[7] = dCaNSR; F[8] = dxKs; F[9] = dLTRPNCa; F[10] = dHTRPNCa;
@@ -282,6 +365,17 @@ GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, con
     cur[17] = IKv43; cur[18] = IKv14; cur[19] = IKv14_K; cur[20] = IKv14_Na; cur[21] = Ito2;
     cur[22] = Istim; cur[23] = Itot;
   }
+}
+
+
+// fcn.c:29-1072.  J = {Jxfer, Jtr, ICa, Ito2} from the release units.  Mutates
+// state[1] below -90 mV (:663-669) and state[0] under voltage clamp (:811-814).
+GRW_HD void fcn(double t, double* state, double* F, double* cur, bool keepc, const double J[4],
+                const GrwProtocol& p) {
+  FcnTrans q;
+  fcn_prep(t, state, p, q);
+  fcn_trans(q);
+  fcn_finish(t, state, F, cur, keepc, J, p, q);
 }
 
 }  // namespace grw
