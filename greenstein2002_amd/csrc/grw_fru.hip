@@ -703,14 +703,14 @@ __device__ __forceinline__ double unif(uint32_t y) { return ((double)y) / 429496
 template <class R>
 __device__ __forceinline__ void fcn_fru4(R CaJSR, R s, R V, R Cai, R CaNSR, bool lo, int no, R& dj,
                                          R& ds) {
-  const R JDHconstant = R(1.0) / (R(2.0) * R(VSS) * R(Faraday) * R(1000.0));
-  const R Jtr = (CaNSR - CaJSR) / R(tautr);
+  const R JDHconstant = rdiv(R(1.0), R(2.0) * R(VSS) * R(Faraday) * R(1000.0));
+  const R Jtr = rdiv(CaNSR - CaJSR, R(tautr));
   const R JRyR = R(JRyRmax) * ((R)no) * (CaJSR - s);
   const R JRyRtot = qperm<QB0>(JRyR) + qperm<QB1>(JRyR) + qperm<QB2>(JRyR) + qperm<QB3>(JRyR);
-  const R Jxfer = (s - Cai) / R(tauxfer);
-  const R Jss = (s - qperm<QNEXT>(s)) / R(tauss2ss);
+  const R Jxfer = rdiv(s - Cai, R(tauxfer));
+  const R Jss = rdiv(s - qperm<QNEXT>(s), R(tauss2ss));
   const R Jss_prev = qperm<QPREV>(Jss);
-  const R VF_over_RT = V / R(RT_over_F);
+  const R VF_over_RT = rdiv(V, R(RT_over_F));
   const R VFsq_over_RT = (R(1000.0) * R(Faraday)) * VF_over_RT;
   const R e = xexp(R(2.0) * VF_over_RT);
   const R open = lo ? R(1.0) : R(0.0);
@@ -718,15 +718,15 @@ __device__ __forceinline__ void fcn_fru4(R CaJSR, R s, R V, R Cai, R CaNSR, bool
   if (fabs(V) < R(1.e-6)) {
     JDHPR = -R(PCa) * R(2.0) * R(1000.0) * R(Faraday) * (s - R(Cao) * R(0.341)) * open * JDHconstant;
   } else {
-    const R a2 = R(PCa) * R(4.0) * VFsq_over_RT / (e - R(1.0)) * JDHconstant;
+    const R a2 = rdiv(R(PCa) * R(4.0) * VFsq_over_RT, e - R(1.0)) * JDHconstant;
     JDHPR = -(s * e - R(Cao) * R(0.341)) * a2 * open;
   }
-  const R a1s = R(BSRtot) * R(KBSR) / ((s + R(KBSR)) * (s + R(KBSR)));
-  const R a2s = R(BSLtot) * R(KBSL) / ((s + R(KBSL)) * (s + R(KBSL)));
-  const R bSS = R(1.0) / (R(1.0) + a1s + a2s);
-  const R a1j = R(CSQNtot) * R(KmCSQN) / ((CaJSR + R(KmCSQN)) * (CaJSR + R(KmCSQN)));
-  const R bJSR = R(1.0) / (R(1.0) + a1j);
-  dj = bJSR * (Jtr - R(VSS) / R(VJSR) * JRyRtot);
+  const R a1s = rdiv(R(BSRtot) * R(KBSR), (s + R(KBSR)) * (s + R(KBSR)));
+  const R a2s = rdiv(R(BSLtot) * R(KBSL), (s + R(KBSL)) * (s + R(KBSL)));
+  const R bSS = rdiv(R(1.0), R(1.0) + a1s + a2s);
+  const R a1j = rdiv(R(CSQNtot) * R(KmCSQN), (CaJSR + R(KmCSQN)) * (CaJSR + R(KmCSQN)));
+  const R bJSR = rdiv(R(1.0), R(1.0) + a1j);
+  dj = bJSR * (Jtr - rdiv(R(VSS), R(VJSR)) * JRyRtot);
   ds = bSS * (JDHPR + JRyR - Jxfer - Jss + Jss_prev);
 }
 
@@ -767,19 +767,19 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       const R V = (R)dV;
       const R vm35 = V - R(35.0);
       const R e0 = R(0.012) * vm35, e1 = R(-0.05) * vm35;
-      const R e2 = (V + R(12.5)) / R(5.0), e3 = (V + R(30.0)) / R(12.0);
+      const R e2 = rdiv(V + R(12.5), R(5.0)), e3 = rdiv(V + R(30.0), R(12.0));
       const R ex = xexp(c == 0 ? e0 : c == 1 ? e1 : c == 2 ? e2 : e3);  // one exp per lane
       const R alpha = R(alphacf) * qperm<QB0>(ex);
       const R beta = R(betacf) * qperm<QB1>(ex);
-      const R yinf = R(yCa_frac) / (R(1.0) + qperm<QB2>(ex)) + (R(1.0) - R(yCa_frac));
-      const R tau = R(60.0) + R(340.0) / (R(1.0) + qperm<QB3>(ex));
-      const R bp = beta / R(bL * 1.0);
+      const R yinf = rdiv(R(yCa_frac), R(1.0) + qperm<QB2>(ex)) + (R(1.0) - R(yCa_frac));
+      const R tau = R(60.0) + rdiv(R(340.0), R(1.0) + qperm<QB3>(ex));
+      const R bp = rdiv(beta, R(bL * 1.0));
       const R cc = (R)css;
       const int ls = ch_lcc(w);
       bad |= (ls < 1 || ls > 12);
       R lb, lf, lm, lt;
       lcc_arith<R>(ls, alpha, beta, bp, gcf * cc, lb, lf, lm, lt);
-      const R vr = (ch_y(w) == 2 ? (R(1.0) - yinf) : yinf) / tau;
+      const R vr = rdiv(ch_y(w) == 2 ? (R(1.0) - yinf) : yinf, tau);
       const R ir = (ch_ito2(w) == 1) ? R(kfIto2) * cc : R(kbIto2);
       R maxr = rmax(rmax(rmax(R(0.0), lt), vr), ir);
       const RyrK<R> k = ryr_k(cc, k23c);
@@ -810,9 +810,9 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       const R T2 = (R(k21) + k.k23 + k.k25);
       const R T3 = (R(k32) + k.k34);
       const R T5 = (R(k52) + k.k54 + k.k56);
-      const R a7 = k.k34 / (R(k43) + k.k34);
+      const R a7 = rdiv(k.k34, R(k43) + k.k34);
       const R T7 = ((R(1.0) - a7) * R(k32)) + (a7 * R(k45));
-      const R a8 = R(k65) / (k.k56 + R(k65));
+      const R a8 = rdiv(R(k65), k.k56 + R(k65));
       const R T8 = (a8 * R(k52)) + (a8 * k.k54);
       R tr[kNryr];
 #pragma unroll

@@ -67,8 +67,14 @@ constexpr double threshMAXCa = 0.0504410547074504;
 template <class T> GRW_HD T rmax(T a, T b) { return (a < b) ? b : a; }
 template <class T> GRW_HD T rmin(T a, T b) { return (a < b) ? a : b; }
 
+// Division and exp in the rate and local-Ca arithmetic: IEEE in FP64 (the
+// drop-in keeps the reference's operations bit for bit); in the FP32 mixed form
+// (device only) the hardware reciprocal and exp2 (v_rcp_f32, v_exp_f32, about
+// 1 ulp), whose error is far below the FP32 rounding the mixed form accepts.
+GRW_HD double rdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float rdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 GRW_HD double xexp(double x) { return exp(x); }
-GRW_HD float xexp(float x) { return expf(x); }
+__device__ __forceinline__ float xexp(float x) { return __expf(x); }
 
 // ---- packed channel word, one per cleft
 //   bits 0-3  LCC state 1..12      bit 4  voltage gate (0: y=1 inactivated, 1: y=2)
@@ -145,8 +151,8 @@ struct RyrK {
 template <class R>
 GRW_HD RyrK<R> ryr_k(R ca, R k23c) {
   RyrK<R> k;
-  const R sat = rmin(R(threshMAX), (ca * ca) / R(k_rate));
-  const R sat2 = rmin(R(threshMAX), (ca * ca) / R(k_rate2));
+  const R sat = rmin(R(threshMAX), rdiv(ca * ca, R(k_rate)));
+  const R sat2 = rmin(R(threshMAX), rdiv(ca * ca, R(k_rate2)));
   k.k12 = R(k12cf) * sat2;
   k.k23 = k23c * sat;
   k.k34 = R(k34cf) * sat;
@@ -173,13 +179,13 @@ GRW_HD void ryr_exits(int s, const RyrK<R>& k, Exits<R>& e) {
     case 5: e.tot = (R(k52) + k.k54 + k.k56); e.r[0] = R(k52); e.d[0] = 2; e.r[1] = k.k54; e.d[1] = 4; e.r[2] = k.k56; e.d[2] = 6; e.n = 3; return;
     case 6: e.tot = R(k65); e.r[0] = R(k65); e.d[0] = 5; e.n = 1; return;
     case 7: {
-      const R a1 = k.k34 / (R(k43) + k.k34);
+      const R a1 = rdiv(k.k34, R(k43) + k.k34);
       const R a2 = (R(1.0) - a1) * R(k32);
       const R a3 = a1 * R(k45);
       e.tot = (a2 + a3); e.r[0] = a2; e.d[0] = 2; e.r[1] = a3; e.d[1] = 8; e.n = 2; return;
     }
     default: {  // 8
-      const R a1 = R(k65) / (k.k56 + R(k65));
+      const R a1 = rdiv(R(k65), k.k56 + R(k65));
       const R a2 = a1 * R(k52);
       const R a3 = a1 * k.k54;
       e.tot = (a2 + a3); e.r[0] = a2; e.d[0] = 2; e.r[1] = a3; e.d[1] = (g == 3) ? 7 : 4; e.n = 2; return;
@@ -201,8 +207,8 @@ GRW_HD int ryr_collapse(int s, int regime) {
 // Split of a merged state with uniform u: 7 -> 4|3, 8 -> 6|5.
 template <class R>
 GRW_HD int ryr_split(int s, double u, const RyrK<R>& k) {
-  if (s == 7) return (u < (double)(k.k34 / (k.k34 + R(k43)))) ? 4 : 3;
-  return (u < (double)(k.k56 / (k.k56 + R(k65)))) ? 6 : 5;
+  if (s == 7) return (u < (double)rdiv(k.k34, k.k34 + R(k43))) ? 4 : 3;
+  return (u < (double)rdiv(k.k56, k.k56 + R(k65))) ? 6 : 5;
 }
 
 // ---- local Ca right-hand side: fcn_fru.c:23-137
