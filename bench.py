@@ -67,6 +67,11 @@ def parse():
     ap.add_argument("--batch-ms", type=int, default=30, help="simulated ms of the batched run")
     ap.add_argument("--batch-seq-sample", type=int, default=8,
                     help="cells of the batch also run one at a time for the sequential rate")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="only run the seed ensemble: this many master seeds from 1234, one cell each, "
+                         "FP64 and mixed batches, per-beat statistics vs each other and vs the CPU "
+                         "reference fixture when one exists for the shape")
+    ap.add_argument("--ensemble-ms", type=int, default=1011, help="simulated ms of each ensemble cell")
     ap.add_argument("--summarize", default="",
                     help="only summarise a rocprofv3 output folder (kernel trace and / or counter "
                          "CSVs) per kernel and grid size, as markdown")
@@ -259,8 +264,66 @@ def batched(args):
         speedup_vs_sequential=round(seq_wall / wall, 2))
 
 
+def ensemble(args) -> dict:
+    """Seeds 1234.. as one batch per precision; beat 0 statistics of each cell,
+    ensemble means, mixed - FP64 differences, and FP64 vs the CPU oracle's runs
+    on the same seeds (tests/golden/ensemble_*.npz) when the shape matches."""
+    import glob
+
+    import numpy as np
+
+    from greenstein2002_amd.batch import Batch, CellSpec
+    from greenstein2002_amd.beats import FIELDS
+
+    seeds = np.arange(1234, 1234 + args.ensemble)
+    keys = ["APD90", "V_peak", "Cai_peak", "ICa_peak", "CaNSR_rest", "CaJSR_rest"]
+    idx = [FIELDS.index(k) for k in keys]
+    res, walls = {}, {}
+    for prec in ("fp64", "mixed"):
+        specs = [CellSpec(args.nfru, seed=int(s)) for s in seeds]
+        b = Batch(specs, precision=prec)
+        t0 = time.perf_counter()
+        done = 0
+        while done < args.ensemble_ms:
+            m = min(100, args.ensemble_ms - done)
+            b.run(m)
+            done += m
+            print(f"[bench] ensemble {prec}: t = {done} ms, {time.perf_counter() - t0:.1f} s wall",
+                  file=sys.stderr, flush=True)
+        walls[prec] = time.perf_counter() - t0
+        res[prec] = np.array([b.beats(k)[0] for k in range(len(seeds))])
+        b.close()
+    out = dict(metric="beat statistics of a seed ensemble, FP64 vs mixed vs CPU reference",
+               workload=f"{len(seeds)} cells x {args.nfru} FRUs, seeds {seeds[0]}..{seeds[-1]}, 1 Hz, "
+                        f"t = 0..{args.ensemble_ms} ms from ic/stable, beat at 10 ms",
+               wall_s={k: round(v, 2) for k, v in walls.items()}, stats={},
+               beats={k: v.tolist() for k, v in res.items()}, fields=list(FIELDS))
+    f64, mx = res["fp64"][:, idx], res["mixed"][:, idx]
+    for j, k in enumerate(keys):
+        a, m = f64[:, j], mx[:, j]
+        out["stats"][k] = dict(fp64_mean=float(np.nanmean(a)), fp64_sd=float(np.nanstd(a)),
+                               mixed_mean=float(np.nanmean(m)), mixed_sd=float(np.nanstd(m)),
+                               mixed_rel_err=float((np.nanmean(m) - np.nanmean(a)) / abs(np.nanmean(a))),
+                               sem_rel=float(np.nanstd(a) / np.sqrt(len(a)) / abs(np.nanmean(a))))
+    for path in sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "ensemble_*.npz"))):
+        g = np.load(path)
+        if int(g["nfru"]) == args.nfru and int(g["ms"]) <= args.ensemble_ms and \
+                np.array_equal(g["seeds"][: len(seeds)], seeds):
+            ref = g["beats"][: len(seeds), 0][:, idx]
+            same = np.isclose(f64, ref, rtol=1e-6, atol=0).all(axis=1)
+            out["cpu_reference"] = dict(
+                fixture=os.path.relpath(path, ROOT), seeds_equal_1e6=int(same.sum()),
+                mean_rel_diff={k: float((np.nanmean(f64[:, j]) - np.nanmean(ref[:, j])) / abs(np.nanmean(ref[:, j])))
+                               for j, k in enumerate(keys)})
+            break
+    return out
+
+
 def main():
     args = parse()
+    if args.ensemble:
+        print(json.dumps(ensemble(args)), flush=True)
+        return
     if args.summarize:
         summarize(args.summarize)
         return
