@@ -72,6 +72,12 @@ def parse():
                          "FP64 and mixed batches, per-beat statistics vs each other and vs the CPU "
                          "reference fixture when one exists for the shape")
     ap.add_argument("--ensemble-ms", type=int, default=1011, help="simulated ms of each ensemble cell")
+    ap.add_argument("--clamp", type=int, default=0,
+                    help="only run the voltage-clamp set: this many seeds per test potential "
+                         "(-40..+60 mV step 10, hold -80 mV), FP64 and mixed batches; peak ICa, "
+                         "Ca entry, Ca release and EC-coupling gain of the step")
+    ap.add_argument("--clamp-shift", type=float, default=100.0, help="ms held before the step")
+    ap.add_argument("--clamp-duration", type=float, default=200.0, help="ms of the step")
     ap.add_argument("--summarize", default="",
                     help="only summarise a rocprofv3 output folder (kernel trace and / or counter "
                          "CSVs) per kernel and grid size, as markdown")
@@ -319,8 +325,61 @@ def ensemble(args) -> dict:
     return out
 
 
+def clamp(args) -> dict:
+    """Voltage clamp from -80 mV to Vt for --clamp-duration ms (fcn.c:448-483),
+    seeds 1234.. per test potential, one batch per precision; statistics of the
+    clamp period containing the step (beat records 8, 12, 13)."""
+    import numpy as np
+
+    from greenstein2002_amd.batch import Batch, CellSpec
+    from greenstein2002_amd.cell import Protocol
+
+    vts = np.arange(-40.0, 61.0, 10.0)
+    seeds = np.arange(1234, 1234 + args.clamp)
+    ms = int(args.clamp_shift + args.clamp_duration + 50.0)
+    specs = []
+    for vt in vts:
+        for sd in seeds:
+            prot = Protocol(vclamp=True, vclamp_hold=-80.0, vclamp_set=float(vt), vclamp_shift=args.clamp_shift,
+                            vclamp_duration=args.clamp_duration, vclamp_period=1000.0)
+            specs.append(CellSpec(args.nfru, seed=int(sd), protocol=prot))
+    res, walls = {}, {}
+    for prec in ("fp64", "mixed"):
+        b = Batch(specs, precision=prec)
+        t0 = time.perf_counter()
+        done = 0
+        while done < ms:
+            m = min(50, ms - done)
+            b.run(m)
+            done += m
+            print(f"[bench] clamp {prec}: t = {done} ms, {time.perf_counter() - t0:.1f} s wall", file=sys.stderr,
+                  flush=True)
+        walls[prec] = time.perf_counter() - t0
+        res[prec] = np.array([b.beats(k)[0] for k in range(len(specs))]).reshape(len(vts), len(seeds), -1)
+        b.close()
+    out = dict(metric="voltage clamp, FP64 vs mixed",
+               workload=f"{len(vts)} test potentials x {len(seeds)} seeds x {args.nfru} FRUs, hold -80 mV, "
+                        f"step at {args.clamp_shift} ms for {args.clamp_duration} ms, t = 0..{ms} ms",
+               wall_s={k: round(v, 2) for k, v in walls.items()}, rows=[])
+    for i, vt in enumerate(vts):
+        row = dict(Vt=float(vt))
+        for name, col in (("ICa_peak", 8), ("Ca_in", 12), ("Ca_rel", 13)):
+            a, m = res["fp64"][i, :, col], res["mixed"][i, :, col]
+            row[name] = dict(fp64=float(a.mean()), sd=float(a.std()), mixed=float(m.mean()),
+                             rel_err=float((m.mean() - a.mean()) / abs(a.mean())) if a.mean() else None)
+        ga = res["fp64"][i, :, 13] / res["fp64"][i, :, 12]
+        gm = res["mixed"][i, :, 13] / res["mixed"][i, :, 12]
+        row["gain"] = dict(fp64=float(ga.mean()), sd=float(ga.std()), mixed=float(gm.mean()),
+                           rel_err=float((gm.mean() - ga.mean()) / abs(ga.mean())))
+        out["rows"].append(row)
+    return out
+
+
 def main():
     args = parse()
+    if args.clamp:
+        print(json.dumps(clamp(args)), flush=True)
+        return
     if args.ensemble:
         print(json.dumps(ensemble(args)), flush=True)
         return

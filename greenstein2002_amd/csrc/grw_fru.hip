@@ -733,7 +733,7 @@ __device__ __forceinline__ void fcn_fru4(R CaJSR, R s, R V, R Cai, R CaNSR, bool
 // One sub-interval of every FRU with four lanes per FRU; returns this lane's
 // FRU-step count (nonzero only in lane 0 of a quad) and its flux terms in acc.
 template <class R>
-__device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[5]) {
+__device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[6]) {
   const int gid = (blockIdx.x - a.b0) * blockDim.x + threadIdx.x;
   const int f = a.f0 + (gid >> 2);
   const int c = gid & 3;
@@ -904,6 +904,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     acc[2] = lopen ? css : 0.0;
     acc[3] = lopen ? 1.0 : 0.0;
     acc[4] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
+    acc[5] = JRyRmax * ((double)nopen) * (cj - css);  // this cleft's release flux (calc_fru_avg_local.c)
   }
   return steps;
 }
@@ -930,7 +931,7 @@ __device__ __forceinline__ void close_launch_stats(const Dev& d) {
 
 template <class R, bool REDUCE>
 __global__ void __launch_bounds__(256) k_simfru4(const SimArgs a, const Dev d) {
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   count_steps(d, simfru4_body<R>(a, d, acc));
   if (REDUCE) {
     double tot[5];
@@ -986,8 +987,12 @@ __device__ __forceinline__ void fru_rollback(const SimArgs& a, const Dev& d, int
 //   5 peak Cai, 6 its time - t_s, 7 Cai at the t_s sample, 8 peak (most
 //   negative) ICa of the FRU flux conversion, 9 CaNSR at the t_s sample (SR
 //   load), 10 mean CaJSR at the t_s sample (JSR load, calc_fru_avg output 1),
-//   11 accepted steps in the beat.
-constexpr int kBeatStats = 12;
+//   11 accepted steps in the beat, 12 Ca entered through the LCCs and 13 Ca
+//   released by the RyRs over the beat (mM of myoplasm, trapezoids over the
+//   accepted steps, the first one from the step before t_s), 14 peak release
+//   flux (mM/ms).  Under voltage clamp the beats are the clamp periods
+//   (vclamp_shift, vclamp_period).
+constexpr int kBeatStats = 15;
 
 struct RkCtl {
   GrwProtocol prot;
@@ -1011,7 +1016,7 @@ struct RkCtl {
   double* out_beats;   // [beat_cap][kBeatStats] or null
   int beat_cap, beat_n, beat_cur, pend_row;
   double bt[kBeatStats];
-  double prev_t, prev_V;
+  double prev_t, prev_V, prev_jin, prev_jrel;
   double pend_t, pend_V, pend_Cai, pend_NSR, pend_JSR;  // the last output sample
 };
 
@@ -1034,16 +1039,21 @@ __device__ __forceinline__ void beat_take_sample(RkCtl* c) {
   }
 }
 
-__device__ void beat_update(RkCtl* c, double t, double V, double Cai, double ICa) {
+__device__ void beat_update(RkCtl* c, double t, double V, double Cai, double ICa, double jin, double jrel) {
   if (!c->out_beats) return;
   const GrwProtocol& p = c->prot;
-  const int b = (t >= p.shift) ? (int)floor((t - p.shift) / p.period) : -1;
+  const double shift = p.vclamp ? p.vclamp_shift : p.shift;
+  const double period = p.vclamp ? p.vclamp_period : p.period;
+  const int b = (t >= shift) ? (int)floor((t - shift) / period) : -1;
+  const bool seg = c->prev_t == c->prev_t;  // a previous accepted step exists
+  const double din = seg ? 0.5 * (c->prev_jin + jin) * (t - c->prev_t) : 0.0;
+  const double drel = seg ? 0.5 * (c->prev_jrel + jrel) * (t - c->prev_t) : 0.0;
   if (b != c->beat_cur) {
     beat_flush(c);
     c->beat_cur = b;
     if (b >= 0) {
       double* bt = c->bt;
-      bt[0] = p.shift + b * p.period;
+      bt[0] = shift + b * period;
       bt[1] = bt[4] = bt[7] = bt[9] = bt[10] = __builtin_nan("");
       bt[2] = V;
       bt[3] = t - bt[0];
@@ -1051,6 +1061,9 @@ __device__ void beat_update(RkCtl* c, double t, double V, double Cai, double ICa
       bt[6] = t - bt[0];
       bt[8] = ICa;
       bt[11] = 1.0;
+      bt[12] = din;
+      bt[13] = drel;
+      bt[14] = jrel;
       beat_take_sample(c);
     }
   } else if (b >= 0) {
@@ -1065,6 +1078,9 @@ __device__ void beat_update(RkCtl* c, double t, double V, double Cai, double ICa
     }
     if (ICa < bt[8]) bt[8] = ICa;
     bt[11] += 1.0;
+    bt[12] += din;
+    bt[13] += drel;
+    if (jrel > bt[14]) bt[14] = jrel;
     if (bt[4] != bt[4] && bt[1] == bt[1] && bt[2] - bt[1] > 20.0 && t - bt[0] > bt[3]) {
       const double thr = bt[2] - 0.9 * (bt[2] - bt[1]);
       if (c->prev_V >= thr && V < thr)
@@ -1073,6 +1089,8 @@ __device__ void beat_update(RkCtl* c, double t, double V, double Cai, double ICa
   }
   c->prev_t = t;
   c->prev_V = V;
+  c->prev_jin = jin;
+  c->prev_jrel = jrel;
 }
 
 // Stage s (1..5) sub-interval bounds, rk_pd54m.c:124,145-146,166-167,187-188,208-209.
@@ -1244,10 +1262,10 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   a.b0 = ci.sb0;
   a.k23c = ci.k23c;
   a.gcf = ci.gcf;
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   count_steps(d, simfru4_body<R>(a, d, acc));
-  __shared__ double tot[5];
-  if (!reduce_range<5>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)) return;
+  __shared__ double tot[6];
+  if (!reduce_range<6>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)) return;
   const unsigned long long tail0 = wall_clock64();
   if (threadIdx.x == 0 && base.ncells == 1) close_launch_stats(d);
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
@@ -1380,7 +1398,11 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       L[0] = ctl->start; L[1] = yv[0]; L[2] = yv[6]; L[3] = ctl->J[2];
       ctl->log_n++;
     }
-    beat_update(ctl, ctl->start, yv[0], yv[6], ctl->J[2]);
+    // Ca entering through the LCCs and released by the RyRs, both per litre of
+    // myoplasm (fcn.c:1058 and the scaling of send_calc_fru_avg, mpi_master.c:178)
+    const double jin = ctl->J[2] * Acap / (-2.0 * Vmyo * Faraday * 1000.0);
+    const double jrel = tot[5] * ci.scale * VSS / Vmyo;
+    beat_update(ctl, ctl->start, yv[0], yv[6], ctl->J[2], jin, jrel);
   }
   if (decision != 2) return;
   // ---- the caf.c loop: record this call's outputs, set up the next call
@@ -1757,7 +1779,8 @@ void reset_tracking(grw_ctx* c) {
     h.beat_n = 0;
     h.beat_cur = -1;
     h.pend_row = -1;
-    h.prev_t = h.prev_V = h.pend_t = h.pend_V = h.pend_Cai = h.pend_NSR = h.pend_JSR = NAN;
+    h.prev_t = h.prev_V = h.prev_jin = h.prev_jrel = NAN;
+    h.pend_t = h.pend_V = h.pend_Cai = h.pend_NSR = h.pend_JSR = NAN;
     for (int q = 0; q < kBeatStats; ++q) h.bt[q] = NAN;
     h.out_beats = c->dbeats + (size_t)k * c->beat_cap * kBeatStats;
   }

@@ -88,9 +88,16 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
  *   9 CaNSR at t_s (SR load, mM)
  *  10 mean CaJSR at t_s (JSR load, send_calc_fru_avg output 1, mM)
  *  11 accepted steps in the beat
- * Writes up to cap finished beats (and, if include_current, the beat in
- * progress) to out and their count to *n. */
-#define GRW_BEAT_STATS 12
+ *  12 Ca entered through the LCCs over the beat, -ICa Acap / (2 Vmyo F) (mM of
+ *     myoplasm; fcn.c:1058), 13 Ca released by the RyRs over the beat, the
+ *     whole-cell JRyR of send_calc_fru_avg (mpi_master.c:178; mM of myoplasm),
+ *     both trapezoids over the accepted steps (the first from the step before
+ *     t_s); 13 / 12 is the EC-coupling gain
+ *  14 peak JRyR (mM/ms)
+ * Under voltage clamp the beats are the clamp periods: t_s = vclamp_shift +
+ * b * vclamp_period.  Writes up to cap finished beats (and, if
+ * include_current, the beat in progress) to out and their count to *n. */
+#define GRW_BEAT_STATS 15
 int grw_get_beats(grw_ctx *ctx, double *out, int cap, int *n, int include_current);
 
 #ifdef __cplusplus

@@ -112,9 +112,9 @@ def test_total_ca_matches_reference_initialisation(golden):
 
 
 def test_beat_stats_closed_form():
-    """beat_stats on a piecewise-linear action potential whose APD90, peaks and
-    rest values are known in closed form."""
-    from greenstein2002_amd.beats import beat_stats
+    """beat_stats on a piecewise-linear action potential, ICa and release flux
+    whose APD90, peaks, rest values and integrals are known in closed form."""
+    from greenstein2002_amd.beats import NSTAT, beat_stats, ca_influx
 
     shift, period = 10.0, 100.0
     # rest -90 mV; upstroke 10 -> 12 ms to +30; linear repolarisation to -90 at 92 ms
@@ -130,10 +130,14 @@ def test_beat_stats_closed_form():
             return 30.0 - 1.5 * (x - 12.0)
         return -90.0
 
+    def ICa(x):  # triangle of area 4 (uA/uF ms) between 10 and 14 ms, peak -2 at 12 ms
+        return -(x - 10.0) if 10.0 <= x <= 12.0 else (-(14.0 - x) if 12.0 < x <= 14.0 else 0.0)
+
     Vs = np.array([V(x) for x in t])
     cai = 1e-4 + 1e-3 * np.exp(-((t - 40.0) / 10.0) ** 2)
-    ica = -np.exp(-((t - 11.0) / 1.0) ** 2)
+    ica = np.array([ICa(x) for x in t])
     log = np.stack([t, Vs, cai, ica], axis=1)
+    jrel = np.full(len(t), 2e-3)
     rows_t = np.arange(1.0, 141.0)
     states = np.zeros((len(rows_t), 37))
     other = np.zeros((len(rows_t), 12))
@@ -141,14 +145,20 @@ def test_beat_stats_closed_form():
     states[:, 6] = 1e-4 + 1e-3 * np.exp(-((rows_t - 40.0) / 10.0) ** 2)
     states[:, 7] = 0.7
     other[:, 1] = 0.65
-    rec = beat_stats(log, rows_t, states, other, shift, period)
-    assert rec.shape == (2, 12)
+    rec = beat_stats(log, rows_t, states, other, shift, period, jrel=jrel)
+    assert rec.shape == (2, NSTAT)
     b = rec[0]
     assert b[0] == 10.0 and b[1] == -90.0 and b[2] == 30.0 and b[3] == 2.0
     # threshold 30 - 0.9 * 120 = -78 mV reached at 12 + 108/1.5 = 84 ms
     assert abs(b[4] - (84.0 - 10.0)) < 1e-12
     assert b[5] == cai.max() and b[6] == 40.0 - 10.0
-    assert b[8] == ica.min() and b[9] == 0.7 and b[10] == 0.65
+    assert b[8] == -2.0 and b[9] == 0.7 and b[10] == 0.65
     assert b[11] == np.count_nonzero((t >= 10.0) & (t < 110.0))
+    # trapezoids are exact on piecewise-linear data with kinks on the grid: Ca in
+    # = -4 uA/uF ms converted, Ca released = 2e-3 mM/ms x (109 - 9) ms
+    assert abs(b[12] - ca_influx(-4.0)) < 1e-15 and ca_influx(-4.0) > 0
+    assert abs(b[13] - 2e-3 * 100.0) < 1e-15 and b[14] == 2e-3
     # the second (partial) beat has no action potential: no APD90
     assert rec[1][0] == 110.0 and np.isnan(rec[1][4])
+    # without a release-flux column the release fields are NaN
+    assert np.isnan(beat_stats(log, rows_t, states, other, shift, period)[0][13])

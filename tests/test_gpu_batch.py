@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from greenstein2002_amd.batch import Batch, CellSpec
-from greenstein2002_amd.beats import beat_stats
+from greenstein2002_amd.beats import NSTAT, beat_stats
 from greenstein2002_amd.cell import Cell, Protocol
 
 pytestmark = pytest.mark.gpu
@@ -66,8 +66,17 @@ def test_device_beat_records_match_host_reduction():
     states = np.concatenate([p.states for p in parts])
     other = np.concatenate([p.other for p in parts])
     host = beat_stats(log, t, states, other, prot.shift, prot.period)
-    assert dev.shape == (4, 12)
-    assert np.array_equal(dev, host, equal_nan=True)
+    assert dev.shape == (4, NSTAT)
+    assert np.array_equal(dev[:, :13], host[:, :13], equal_nan=True)  # host log has no release flux
+    # release flux: the peak over accepted steps bounds the 1 ms samples of the
+    # same whole-cell JRyR (send_calc_fru_avg output 2) from above, and the
+    # integral is close to the trapezoid over those samples
+    for k in range(3):
+        m = (t >= dev[k, 0]) & (t < dev[k, 0] + prot.period)
+        assert dev[k, 14] >= other[m, 2].max() * (1.0 - 1e-9)
+        approx = np.trapezoid(other[m, 2], t[m])
+        assert abs(dev[k, 13] - approx) < 0.1 * abs(approx)
+        assert dev[k, 12] > 0.0 and dev[k, 13] > dev[k, 12]  # gain above 1
     # three full beats with an action potential each; the first one, started from
     # the 1 Hz steady state, outlasts the 120 ms cycle, the paced ones repolarise
     assert np.all(dev[:3, 2] > 0.0)
@@ -90,7 +99,7 @@ def test_ensemble_matches_cpu_oracle():
     specs = [CellSpec(int(g["nfru"]), seed=int(s)) for s in seeds]
     with Batch(specs) as b:
         b.run(ms)
-        got = np.array([b.beats(k)[: ref.shape[1]] for k in range(len(seeds))])
+        got = np.array([b.beats(k)[: ref.shape[1], : ref.shape[2]] for k in range(len(seeds))])
     assert got.shape == ref.shape
     cols = [2, 4, 5, 8, 9, 10]  # V_peak, APD90, Cai_peak, ICa_peak, CaNSR, CaJSR
     close = np.isclose(got[:, :, cols], ref[:, :, cols], rtol=1e-6, atol=0)
