@@ -149,6 +149,8 @@ struct SimArgs {
   int nfru, ld;
   int f0, b0;       // the cell's first FRU in the arrays and first block of the grid
   int ncells;
+  int pro;          // before the sub-interval: 1 checkpoint, 2 roll back each FRU (first RK stage)
+  int avg;          // also reduce the average sums of the FRU state at the end
 };
 
 // One cell of a batch: its FRUs are [f0, f0 + nfru) of the shared arrays (f0 a
@@ -203,10 +205,14 @@ __device__ __forceinline__ int pick(double u, double dt, const Exits<R>& e) {
 // Deterministic reduction of NV doubles per lane: xor-butterfly inside the
 // wavefront, waves in order through LDS, block partials in fixed slots, and the
 // last block to finish (ticket) sums the slots in a fixed order.  Returns true
-// in thread 0 of the last block, with the totals in tot[].
+// in thread 0 of the last block, with the totals in tot[].  Block b owns
+// d.part[b * kSlot, (b + 1) * kSlot); slot_off + NV must not exceed kSlot.
+constexpr int kSlot = 32;
+
 template <int NV>
-__device__ bool reduce_range(double v[NV], const Dev& d, unsigned int* ticket, int b0, int nb,
+__device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigned int* ticket, int b0, int nb,
                              int slot_off, double* tot_sh) {
+  static_assert(NV <= kSlot, "block partials overflow their slot row");
   __shared__ double wsum[4][NV];
   __shared__ bool last;
   for (int off = 32; off >= 1; off >>= 1)
@@ -221,7 +227,7 @@ __device__ bool reduce_range(double v[NV], const Dev& d, unsigned int* ticket, i
     for (int k = 0; k < NV; ++k) {
       double s = wsum[0][k];
       for (int w = 1; w < nw; ++w) s += wsum[w][k];
-      d.part[(size_t)blockIdx.x * 16 + slot_off + k] = s;
+      d.part[(size_t)blockIdx.x * kSlot + slot_off + k] = s;
     }
     __threadfence();
     const unsigned t = atomicAdd(ticket, 1u);
@@ -237,7 +243,7 @@ __device__ bool reduce_range(double v[NV], const Dev& d, unsigned int* ticket, i
   for (int b = b0 + (int)threadIdx.x; b < b0 + nb; b += blockDim.x)
 #pragma unroll
     for (int k = 0; k < NV; ++k)
-      s[k] += __hip_atomic_load(&d.part[(size_t)b * 16 + slot_off + k], __ATOMIC_RELAXED,
+      s[k] += __hip_atomic_load(&d.part[(size_t)b * kSlot + slot_off + k], __ATOMIC_RELAXED,
                                 __HIP_MEMORY_SCOPE_AGENT);
   for (int off = 32; off >= 1; off >>= 1)
 #pragma unroll
@@ -260,13 +266,13 @@ __device__ bool reduce_range(double v[NV], const Dev& d, unsigned int* ticket, i
 }
 
 template <int NV>
-__device__ bool reduce_grid_block(double v[NV], const Dev& d, int slot_off, double* tot_sh) {
+__device__ __forceinline__ bool reduce_grid_block(double v[NV], const Dev& d, int slot_off, double* tot_sh) {
   return reduce_range<NV>(v, d, d.ticket, 0, gridDim.x, slot_off, tot_sh);
 }
 
 // Thread 0 of the last block gets the totals in tot[] (returns true there only).
 template <int NV>
-__device__ bool reduce_grid(double v[NV], const Dev& d, int slot_off, double tot[NV]) {
+__device__ __forceinline__ bool reduce_grid(double v[NV], const Dev& d, int slot_off, double tot[NV]) {
   __shared__ double tsh[NV];
   if (!reduce_grid_block<NV>(v, d, slot_off, tsh)) return false;
   if (threadIdx.x != 0) return false;
@@ -671,7 +677,7 @@ __device__ __forceinline__ uint32_t mt_untwist(const uint32_t* row, int q) {
   return (yh << 1) | b;
 }
 
-__device__ void mt_rollback(uint32_t* row, int idx_h, int nreg, uint32_t w_first) {
+__device__ __forceinline__ void mt_rollback(uint32_t* row, int idx_h, int nreg, uint32_t w_first) {
   if (nreg <= 0) return;
   const int q0 = idx_h >= kMtN ? idx_h - kMtN : 0;  // row position of the first regenerated word
   int hi = nreg - 1;                                // offsets from q0, restored newest first
@@ -733,18 +739,42 @@ __device__ __forceinline__ void fcn_fru4(R CaJSR, R s, R V, R Cai, R CaNSR, bool
 // One sub-interval of every FRU with four lanes per FRU; returns this lane's
 // FRU-step count (nonzero only in lane 0 of a quad) and its flux terms in acc.
 template <class R>
-__device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[6]) {
+__device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[17]) {
   const int gid = (blockIdx.x - a.b0) * blockDim.x + threadIdx.x;
   const int f = a.f0 + (gid >> 2);
   const int c = gid & 3;
   unsigned steps = 0;
   if ((gid >> 2) < a.nfru) {  // quads are active or inactive as a whole
-    double cj = d.conc[f];
-    double css = d.conc[(c + 1) * a.ld + f];
-    uint32_t w = d.chan[c * a.ld + f];
     uint32_t* const row = d.mt + (size_t)f * kMtStride;
-    int idx = d.mtidx[f];
-    int nreg = d.nreg[f];
+    double cj, css;
+    uint32_t w;
+    int idx, nreg;
+    if (a.pro == 2) {
+      // rollback (mpi_slave.c:150-184): start from the checkpoint; lane 0
+      // restores the MT words, the fence makes them visible to its quad
+      cj = d.conc_h[f];
+      css = d.conc_h[(c + 1) * a.ld + f];
+      w = d.chan_h[c * a.ld + f];
+      idx = d.mtidx_h[f];
+      nreg = 0;
+      if (c == 0) mt_rollback(row, idx, d.nreg[f], d.mtw_h[f]);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    } else {
+      cj = d.conc[f];
+      css = d.conc[(c + 1) * a.ld + f];
+      w = d.chan[c * a.ld + f];
+      idx = d.mtidx[f];
+      nreg = a.pro ? 0 : d.nreg[f];
+      if (a.pro == 1) {  // checkpoint (mpi_slave.c:192-226)
+        d.conc_h[(c + 1) * a.ld + f] = css;
+        d.chan_h[c * a.ld + f] = w;
+        if (c == 0) {
+          d.conc_h[f] = cj;
+          d.mtidx_h[f] = idx;
+          d.mtw_h[f] = row[mt_first_pos(idx)];
+        }
+      }
+    }
     const double span0 = a.depf[0] - a.dep0[0];
     const double span1 = a.depf[1] - a.dep0[1];
     const double span2 = a.depf[2] - a.dep0[2];
@@ -905,6 +935,25 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     acc[3] = lopen ? 1.0 : 0.0;
     acc[4] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
     acc[5] = JRyRmax * ((double)nopen) * (cj - css);  // this cleft's release flux (calc_fru_avg_local.c)
+    if (a.avg) {  // calc_fru_avg_local.c:135-176, this cleft's terms (CaJSR ones in lane 0)
+      int nr = 0;
+#pragma unroll
+      for (int r = 0; r < kNryr; ++r) {
+        const int st = ch_ryr(w, r);
+        nr += (st == 1 || st == 2);
+      }
+      acc[6] = css;
+      acc[7] = (c == 0) ? cj : 0.0;
+      acc[8] = JRyRmax * ((double)nopen) * (cj - css);
+      acc[9] = (double)nopen;
+      acc[10] = (double)nr;
+      acc[11] = css * (1.0 + BSRtot / (KBSR + css) + BSLtot / (KBSL + css));
+      acc[12] = (c == 0) ? cj * (1.0 + CSQNtot / (KmCSQN + cj)) : 0.0;
+      acc[13] = (ch_lcc(w) <= 6) ? 1.0 : 0.0;
+      acc[14] = (ch_y(w) == 2) ? 1.0 : 0.0;
+      acc[15] = lopen ? 1.0 : 0.0;
+      acc[16] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
+    }
   }
   return steps;
 }
@@ -931,7 +980,7 @@ __device__ __forceinline__ void close_launch_stats(const Dev& d) {
 
 template <class R, bool REDUCE>
 __global__ void __launch_bounds__(256) k_simfru4(const SimArgs a, const Dev d) {
-  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double acc[17] = {};
   count_steps(d, simfru4_body<R>(a, d, acc));
   if (REDUCE) {
     double tot[5];
@@ -1006,6 +1055,7 @@ struct RkCtl {
   double dep0[3], depf[3];
   double start, h, oh, tcall, tstep, ta, tb;
   int stage, success, keepc, resume;
+  int pro;  // per-FRU work of the next first stage: 1 checkpoint, 2 rollback
   int active, ms_done, ms_target, ms_base, avg_pending, avg_row;  // rows count from ms_base
   int log_n, log_cap, err;
   long long attempts, rejects, forced;
@@ -1039,7 +1089,7 @@ __device__ __forceinline__ void beat_take_sample(RkCtl* c) {
   }
 }
 
-__device__ void beat_update(RkCtl* c, double t, double V, double Cai, double ICa, double jin, double jrel) {
+__device__ __forceinline__ void beat_update(RkCtl* c, double t, double V, double Cai, double ICa, double jin, double jrel) {
   if (!c->out_beats) return;
   const GrwProtocol& p = c->prot;
   const double shift = p.vclamp ? p.vclamp_shift : p.shift;
@@ -1122,66 +1172,19 @@ __device__ __forceinline__ void fcn_block(double t, double* st, double* F, doubl
   __syncthreads();
 }
 
-// Attempt start of every cell: rollback or checkpoint, the flux sums (and the
-// average sums of a finished output sample), then in the cell's last block
-// fcn at the attempt start and the first stage (rk_pd54m.c:93-138).  One
-// thread per FRU; blocks map to cells through d.bmap.
-__global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev d, RkCtl* ctls) {
-  const int cell = d.bmap[blockIdx.x];
-  const CellInfo ci = d.cells[cell];
-  RkCtl* ctl = ctls + cell;
-  const int active = ctl->active, avgp = ctl->avg_pending;
-  if (!active && !avgp) return;
-  SimArgs a = base;
-  a.nfru = ci.nfru;
-  const int resume = ctl->resume, success = ctl->success;
-  const int il = (blockIdx.x - ci.bb0) * blockDim.x + threadIdx.x;
-  const int i = ci.f0 + il;
-  double v[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = 0.0;
-  if (il < ci.nfru) {
-    if (active && resume) fru_rollback(a, d, i);
-    else if (active && success) fru_checkpoint(a, d, i);
-    double conc[5];
-    uint32_t ch[4];
-    for (int q = 0; q < 5; ++q) conc[q] = d.conc[q * a.ld + i];
-    for (int q = 0; q < 4; ++q) ch[q] = d.chan[q * a.ld + i];
-    flux_terms(conc, ch, v);
-    if (avgp) {  // calc_fru_avg_local.c:135-176
-      const double cj = conc[0];
-      v[6] = cj;
-      v[11] = cj * (1.0 + CSQNtot / (KmCSQN + cj));
-      for (int q = 0; q < 4; ++q) {
-        const double s = conc[q + 1];
-        const uint32_t w = ch[q];
-        v[5] += s;
-        v[10] += s * (1.0 + BSRtot / (KBSR + s) + BSLtot / (KBSL + s));
-        int no = 0, nr = 0;
-        for (int r = 0; r < kNryr; ++r) {
-          const int st = ch_ryr(w, r);
-          no += (st == 3 || st == 4 || st == 7);
-          nr += (st == 1 || st == 2);
-        }
-        v[7] += JRyRmax * ((double)no) * (cj - s);
-        v[8] += (double)no;
-        v[9] += (double)nr;
-        v[12] += (ch_lcc(w) <= 6) ? 1.0 : 0.0;
-        v[13] += (ch_y(w) == 2) ? 1.0 : 0.0;
-        v[14] += ch_lcc_open(w) ? 1.0 : 0.0;
-        v[15] += (ch_ito2(w) == 2) ? 1.0 : 0.0;
-      }
-    }
-  }
-  __shared__ double tot[16];
-  if (!reduce_range<16>(v, d, d.ticket + cell, ci.bb0, ci.bnb, 0, tot)) return;
-  // ---- last block of the cell: outputs of the finished call, fcn at the attempt start
-  __shared__ double F1[kNglobal];
-  __shared__ double st[kNglobal];
-  __shared__ double Jsh[4], cursh[kNcur];
-  __shared__ FcnTrans fq;
+// The start of an RK attempt, in the cell's last block (rk_pd54m.c:93-138):
+// the averages of a finished output sample (av: the 11 sums of
+// calc_fru_avg_local.c over the FRU state), on success fcn at the attempt start
+// with the fluxes of the FRU sums fl, then k0, the first stage's end point and
+// its bounds.  Call from every thread of the block, after the control block's
+// flags are final.
+__device__ __forceinline__ void attempt_start(RkCtl* ctl, const CellInfo& ci, const double* fl,
+                                              const double* av, double* st, double* F1, double* Jsh,
+                                              double* cursh, FcnTrans* fq) {
   const int j = threadIdx.x;
-  const bool eval = active && !resume && success;  // rk_pd54m.c:93-108
+  const int active = ctl->active, avgp = ctl->avg_pending;
+  const int resume = ctl->resume, success = ctl->success;
+  const bool eval = active && !resume && success;
   const bool keepc = ctl->keepc != 0;
   const double tstart = ctl->start;
   if (j < kNglobal) st[j] = ctl->state[j];
@@ -1189,7 +1192,7 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
   if (j == 0) {
     if (avgp) {
       double other[12];
-      avg_outputs(tot + 5, st, ci.nfru, ci.scale, other);
+      avg_outputs(av, st, ci.nfru, ci.scale, other);
       if (ctl->out_other)
         for (int q = 0; q < 12; ++q) ctl->out_other[(size_t)(ctl->avg_row - ctl->ms_base) * 12 + q] = other[q];
       if (ctl->avg_row == ctl->pend_row) {
@@ -1198,10 +1201,10 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
       }
       ctl->avg_pending = 0;
     }
-    if (eval) flux_currents(tot, st[0], st[6], st[7], ci.nfru, ci.scale, Jsh);
+    if (eval) flux_currents(fl, st[0], st[6], st[7], ci.nfru, ci.scale, Jsh);
   }
   __syncthreads();
-  if (eval) fcn_block(tstart, st, F1, cursh, keepc, Jsh, ctl->prot, &fq);
+  if (eval) fcn_block(tstart, st, F1, cursh, keepc, Jsh, ctl->prot, fq);
   if (j == 0 && active) {
     if (resume) {
       ctl->resume = 0;
@@ -1239,6 +1242,41 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
   }
 }
 
+// Attempt start of every cell at the beginning of a run (later attempts start
+// in the tail of the previous attempt's last stage): a sub-interval of length
+// zero through the stage kernel's per-FRU code, which rolls every FRU back or
+// checkpoints it and sums the FRU state with the lanes, blocks and order of a
+// stage tail, so that a run continued in pieces adds the same numbers as one
+// unbroken run; then attempt_start in the cell's last block.
+template <class R>
+__global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev d, RkCtl* ctls) {
+  const int cell = d.smap[blockIdx.x];
+  const CellInfo ci = d.cells[cell];
+  RkCtl* ctl = ctls + cell;
+  const int active = ctl->active, avgp = ctl->avg_pending;
+  if (!active && !avgp) return;
+  SimArgs a = base;
+  a.t0 = a.t1 = ctl->start;
+  a.dep0[0] = a.depf[0] = ctl->state[0];
+  a.dep0[1] = a.depf[1] = ctl->state[6];
+  a.dep0[2] = a.depf[2] = ctl->state[7];
+  a.nfru = ci.nfru;
+  a.f0 = ci.f0;
+  a.b0 = ci.sb0;
+  a.k23c = ci.k23c;
+  a.gcf = ci.gcf;
+  a.pro = !active ? 0 : ctl->resume ? 2 : ctl->success ? 1 : 0;  // mpi_slave.c:150-226
+  a.avg = avgp;
+  double acc[17] = {};
+  (void)simfru4_body<R>(a, d, acc);
+  __shared__ double tot[17];
+  if (!reduce_range<17>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)) return;
+  __shared__ double st[kNglobal], F1[kNglobal], Jsh[4], cursh[kNcur];
+  __shared__ FcnTrans fq;
+  if (threadIdx.x == 0) ctl->pro = 0;  // this launch did the per-FRU work
+  attempt_start(ctl, ci, tot, tot + 6, st, F1, Jsh, cursh, &fq);
+}
+
 // One RK stage of every cell: the Monte Carlo sub-interval of its FRUs (four
 // lanes per FRU, 64 FRUs per block, blocks map to cells through d.smap), the
 // flux reduction, and in the cell's last block fcn for this stage and then
@@ -1262,10 +1300,13 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   a.b0 = ci.sb0;
   a.k23c = ci.k23c;
   a.gcf = ci.gcf;
-  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int s0 = ctl->stage;
+  a.pro = (s0 == 1) ? ctl->pro : 0;
+  a.avg = (s0 == 5);  // the attempt's end state: averages for an output sample
+  double acc[17] = {};
   count_steps(d, simfru4_body<R>(a, d, acc));
-  __shared__ double tot[6];
-  if (!reduce_range<6>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)) return;
+  __shared__ double tot[17];
+  if (!reduce_range<17>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)) return;
   const unsigned long long tail0 = wall_clock64();
   if (threadIdx.x == 0 && base.ncells == 1) close_launch_stats(d);
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
@@ -1321,6 +1362,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       ctl->ta = ta;
       ctl->tb = tb;
       ctl->stage = s + 1;
+      if (s == 1) ctl->pro = 0;  // every FRU did its checkpoint / rollback
       atomicAdd(d.tailticks, wall_clock64() - tail0);
     }
     return;
@@ -1381,10 +1423,9 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     }
     ctl->h = hn;
     ctl->stage = 0;
-    atomicAdd(d.tailticks, wall_clock64() - tail0);
   }
   __syncthreads();
-  if (decision == 0) return;
+  if (decision != 0) {
   if (j < kNglobal) {
     double y = yv[j];
     if (fabs(y) < 1.e-10) y = 0.0;
@@ -1404,7 +1445,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     const double jrel = tot[5] * ci.scale * VSS / Vmyo;
     beat_update(ctl, ctl->start, yv[0], yv[6], ctl->J[2], jin, jrel);
   }
-  if (decision != 2) return;
+  if (decision == 2) {
   // ---- the caf.c loop: record this call's outputs, set up the next call
   const int row = ctl->ms_done;
   const size_t orow = (size_t)(row - ctl->ms_base);
@@ -1437,6 +1478,16 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     ctl->keepc = 1;
     if (row + 1 >= ctl->ms_target) ctl->active = 0;
   }
+  }
+  }
+  // ---- the next attempt starts here (rk_pd54m.c:93-138): its first stage
+  // checkpoints every FRU after an accepted step and rolls it back after a
+  // rejected one; the FRU sums of this launch are those of the state it starts from
+  if (j == 0) ctl->pro = (decision == 0) ? 2 : 1;
+  __syncthreads();
+  __shared__ double F1[kNglobal], cursh[kNcur];
+  attempt_start(ctl, ci, tot, tot + 6, st, F1, Jsh, cursh, &fq);
+  if (j == 0) atomicAdd(d.tailticks, wall_clock64() - tail0);
 }
 
 // Average-output sums of calc_fru_avg_local.c:135-176 (tree order).
@@ -1703,7 +1754,7 @@ int alloc_fru_state(grw_ctx* c) {
       (rc = dalloc(c, &c->d.mtidx, ld)) || (rc = dalloc(c, &c->d.nreg, ld)) ||
       (rc = dalloc(c, &c->d.mt, ld * kMtStride)) || (rc = dalloc(c, &c->d.mtw_h, ld)) ||
       (rc = dalloc(c, &c->d.conc_h, 5 * ld)) || (rc = dalloc(c, &c->d.chan_h, 4 * ld)) ||
-      (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, nslots * 16)) ||
+      (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, nslots * kSlot)) ||
       (rc = dalloc(c, &c->d.ticket, (size_t)c->ncells)) || (rc = dalloc(c, &c->d.out, 32)) ||
       (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
       (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
@@ -1786,7 +1837,7 @@ void reset_tracking(grw_ctx* c) {
   }
 }
 
-// Enqueue RK attempts (k_dev_begin + five k_dev_stage) until every active cell
+// Enqueue RK attempts (five k_dev_stage each, after one k_dev_begin) until every active cell
 // of c->hctl has reached its ms_target; the host synchronises once per batch of
 // attempts sized to ~25 simulated ms of the slowest cell, and launches after a
 // cell is done return at once for it.
@@ -1801,6 +1852,7 @@ int run_device(grw_ctx* c) {
   std::vector<double> ms_per_att(nc, 1.0 / 40.0);
   for (int k = 0; k < nc; ++k) ms_prev[k] = c->hctl[k].ms_done;
   std::vector<hipEvent_t>& evs = c->evpool;
+  bool first = true;
   while (true) {
     int K = 0;
     for (int k = 0; k < nc; ++k) {
@@ -1819,8 +1871,16 @@ int run_device(grw_ctx* c) {
       }
     }
     const bool timed = c->timing && (int)evs.size() >= 10 * K;
+    std::vector<char> was_active(nc);
+    for (int k = 0; k < nc; ++k) was_active[k] = c->hctl[k].active != 0;
+    if (first) {  // the run's first attempt; later ones start in the last stage's tail
+      if (mixed)
+        hipLaunchKernelGGL(k_dev_begin<float>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+      else
+        hipLaunchKernelGGL(k_dev_begin<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+    }
+    first = false;
     for (int k = 0; k < K; ++k) {
-      hipLaunchKernelGGL(k_dev_begin, dim3(c->grid_b), dim3(256), 0, c->stream, base, c->d, c->dctl);
       for (int s5 = 0; s5 < 5; ++s5) {
         if (timed) (void)hipEventRecord(evs[10 * k + 2 * s5], c->stream);
         if (mixed)
@@ -1849,6 +1909,10 @@ int run_device(grw_ctx* c) {
       if (kerr || h.err)
         return fail(GRW_ESTATE, "kernel met an unknown channel state, a non-finite rate or a runaway step count" + who);
       const long long done = h.attempts - att_prev[k];
+      // a cell still active after a batch of at least four attempts' launches
+      // started attempts in it (one finishing its last attempt may start none)
+      if (was_active[k] && h.active && done == 0)
+        return fail(GRW_EDEVICE, "a batch of launches completed no RK attempt" + who);
       eff = std::max(eff, done);
       c->stats.simfru_calls += (uint64_t)(5 * done) * (uint64_t)c->cells[k].nfru;
       att_prev[k] = h.attempts;
@@ -1868,7 +1932,10 @@ int run_device(grw_ctx* c) {
   bool avg = false;
   for (const RkCtl& h : c->hctl) avg = avg || h.avg_pending;
   if (avg) {  // averages of the last output sample
-    hipLaunchKernelGGL(k_dev_begin, dim3(c->grid_b), dim3(256), 0, c->stream, base, c->d, c->dctl);
+    if (mixed)
+      hipLaunchKernelGGL(k_dev_begin<float>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+    else
+      hipLaunchKernelGGL(k_dev_begin<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
       e = hipMemcpyAsync(c->hctl.data(), c->dctl, sizeof(RkCtl) * nc, hipMemcpyDeviceToHost, c->stream);
