@@ -543,10 +543,20 @@ __device__ __forceinline__ int lcc_move(int s, double u, double dt, R back, R fw
 }
 
 // Total exit rate of a RyR in (resolved) state s: fru_rates_local.c:390-727.
+// Independent selects: a chained conditional on s becomes a branch tree, and
+// since the lanes of a wavefront hold different states every arm of it runs,
+// each behind its own exec-mask handling.
 template <class R>
 __device__ __forceinline__ R ryr_total(int s, R T1, R T2, R T3, R T5, R T7, R T8) {
-  return s == 1 ? T1 : s == 2 ? T2 : s == 3 ? T3 : s == 4 ? R(k43 + k45) : s == 5 ? T5
-       : s == 6 ? R(k65) : s == 7 ? T7 : T8;
+  R t = T8;
+  t = (s == 1) ? T1 : t;
+  t = (s == 2) ? T2 : t;
+  t = (s == 3) ? T3 : t;
+  t = (s == 4) ? R(k43 + k45) : t;
+  t = (s == 5) ? T5 : t;
+  t = (s == 6) ? R(k65) : t;
+  t = (s == 7) ? T7 : t;
+  return t;
 }
 
 // Row position of absolute stream position g in [0, 1872): positions below 624
@@ -640,6 +650,9 @@ __device__ __forceinline__ void mt_select(const uint4& x0, const uint4& x1, cons
 }
 
 // Regeneration and stores for stream positions g0 .. g0+7 from a fetched window.
+// Branch-free: a position of the loaded generation (g < 624) stores its word
+// back unchanged (cur[k] is the word at row position g), so every word takes
+// the same instructions whatever the lane's stream position.
 __device__ __forceinline__ void mt_finish8(uint32_t* row, int g0, const MtWin8& w, uint32_t out[8]) {
   const int q = mt_row_pos(g0);
   uint32_t cur[9], far[8];
@@ -648,13 +661,9 @@ __device__ __forceinline__ void mt_finish8(uint32_t* row, int g0, const MtWin8& 
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int g = g0 + k;
-    if (g >= kMtN) {
-      const uint32_t wv = mt_twist(cur[k], cur[k + 1], far[k]);
-      row[mt_row_pos(g)] = wv;
-      out[k] = MtLane::temper(wv);
-    } else {
-      out[k] = MtLane::temper(cur[k]);
-    }
+    const uint32_t wv = (g >= kMtN) ? mt_twist(cur[k], cur[k + 1], far[k]) : cur[k];
+    row[mt_row_pos(g)] = wv;
+    out[k] = MtLane::temper(wv);
   }
 }
 
@@ -704,11 +713,34 @@ __device__ __forceinline__ int mt_first_pos(int idx) { return idx >= kMtN ? idx 
 
 __device__ __forceinline__ double unif(uint32_t y) { return ((double)y) / 4294967296.0; }
 
-// fcn_fru.c:23-137 for one cleft of the quad: returns d(CaJSR) (same in all
-// four lanes) and d(CaSS) of this lane's cleft.
+// The terms of fcn_fru.c that depend on the membrane potential alone: the
+// LCC flux factor exp(2VF/RT) and PCa 4 V F^2/RT / (exp(2VF/RT) - 1) / (2 VSS F)
+// (fcn_fru.c:60-75), with the reference's operations.  The second evaluation of
+// a Heun step and the first of the next step share V, so a step computes them
+// once.
 template <class R>
-__device__ __forceinline__ void fcn_fru4(R CaJSR, R s, R V, R Cai, R CaNSR, bool lo, int no, R& dj,
-                                         R& ds) {
+struct Ghk {
+  R e, a2;
+  bool zero;  // |V| < 1e-6: the limit form
+};
+
+template <class R>
+__device__ __forceinline__ Ghk<R> ghk_terms(R V) {
+  const R JDHconstant = rdiv(R(1.0), R(2.0) * R(VSS) * R(Faraday) * R(1000.0));
+  const R VF_over_RT = rdiv(V, R(RT_over_F));
+  const R VFsq_over_RT = (R(1000.0) * R(Faraday)) * VF_over_RT;
+  Ghk<R> g;
+  g.e = xexp(R(2.0) * VF_over_RT);
+  g.zero = fabs(V) < R(1.e-6);
+  g.a2 = rdiv(R(PCa) * R(4.0) * VFsq_over_RT, g.e - R(1.0)) * JDHconstant;  // unused when zero
+  return g;
+}
+
+// fcn_fru.c:23-137 for one cleft of the quad: returns d(CaJSR) (same in all
+// four lanes) and d(CaSS) of this lane's cleft; g holds the V terms.
+template <class R>
+__device__ __forceinline__ void fcn_fru4(R CaJSR, R s, const Ghk<R>& g, R Cai, R CaNSR, bool lo, int no,
+                                         R& dj, R& ds) {
   const R JDHconstant = rdiv(R(1.0), R(2.0) * R(VSS) * R(Faraday) * R(1000.0));
   const R Jtr = rdiv(CaNSR - CaJSR, R(tautr));
   const R JRyR = R(JRyRmax) * ((R)no) * (CaJSR - s);
@@ -716,17 +748,9 @@ __device__ __forceinline__ void fcn_fru4(R CaJSR, R s, R V, R Cai, R CaNSR, bool
   const R Jxfer = rdiv(s - Cai, R(tauxfer));
   const R Jss = rdiv(s - qperm<QNEXT>(s), R(tauss2ss));
   const R Jss_prev = qperm<QPREV>(Jss);
-  const R VF_over_RT = rdiv(V, R(RT_over_F));
-  const R VFsq_over_RT = (R(1000.0) * R(Faraday)) * VF_over_RT;
-  const R e = xexp(R(2.0) * VF_over_RT);
   const R open = lo ? R(1.0) : R(0.0);
-  R JDHPR;
-  if (fabs(V) < R(1.e-6)) {
-    JDHPR = -R(PCa) * R(2.0) * R(1000.0) * R(Faraday) * (s - R(Cao) * R(0.341)) * open * JDHconstant;
-  } else {
-    const R a2 = rdiv(R(PCa) * R(4.0) * VFsq_over_RT, e - R(1.0)) * JDHconstant;
-    JDHPR = -(s * e - R(Cao) * R(0.341)) * a2 * open;
-  }
+  const R JDHPR = g.zero ? -R(PCa) * R(2.0) * R(1000.0) * R(Faraday) * (s - R(Cao) * R(0.341)) * open * JDHconstant
+                         : -(s * g.e - R(Cao) * R(0.341)) * g.a2 * open;
   const R a1s = rdiv(R(BSRtot) * R(KBSR), (s + R(KBSR)) * (s + R(KBSR)));
   const R a2s = rdiv(R(BSLtot) * R(KBSL), (s + R(KBSL)) * (s + R(KBSL)));
   const R bSS = rdiv(R(1.0), R(1.0) + a1s + a2s);
@@ -792,6 +816,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     // waits for all outstanding loads, prefetch included.
     __builtin_amdgcn_s_waitcnt(0);
     if (t < a.t1) mt_fetch8(row, idx + 8 * c, win);
+    Ghk<R> gv = ghk_terms<R>((R)dV);
     while (t < a.t1) {
       // ---- rates at the step start (fru_rates_local.c:126-777)
       const R V = (R)dV;
@@ -892,14 +917,15 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       mt_fetch8(row, idx + 8 * c, win);
       // ---- Heun step (simfru_local.c:149-173)
       R djs, dss;
-      fcn_fru4<R>((R)cj, cc, (R)dV, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
+      fcn_fru4<R>((R)cj, cc, gv, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
       const double k1j = dt * (double)djs, k1s = dt * (double)dss;
       const double y1j = cj + k1j, y1s = css + k1s;
       t = t + dt;
       dV = a.dep0[0] + (t - a.t0) / tint * span0;
       dCai = a.dep0[1] + (t - a.t0) / tint * span1;
       dNSR = a.dep0[2] + (t - a.t0) / tint * span2;
-      fcn_fru4<R>((R)y1j, (R)y1s, (R)dV, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
+      gv = ghk_terms<R>((R)dV);  // also the next step's first evaluation
+      fcn_fru4<R>((R)y1j, (R)y1s, gv, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
       cj = cj + (k1j + dt * (double)djs) / 2.0;
       css = css + (k1s + dt * (double)dss) / 2.0;
       // ---- transitions of this cleft (simfru_local.c:215-276)
