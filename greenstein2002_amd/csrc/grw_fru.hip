@@ -650,19 +650,22 @@ __device__ __forceinline__ void mt_select(const uint4& x0, const uint4& x1, cons
 }
 
 // Regeneration and stores for stream positions g0 .. g0+7 from a fetched window.
-// Branch-free: a position of the loaded generation (g < 624) stores its word
-// back unchanged (cur[k] is the word at row position g), so every word takes
-// the same instructions whatever the lane's stream position.
+// Branch-free: every word is twisted and a mask keeps the loaded word for a
+// position of the loaded generation (g < 624), which is stored back unchanged
+// (cur[k] is the word at row position g); row positions q + k wrap to the row
+// start past 623, the store address selecting between two bases.
 __device__ __forceinline__ void mt_finish8(uint32_t* row, int g0, const MtWin8& w, uint32_t out[8]) {
   const int q = mt_row_pos(g0);
   uint32_t cur[9], far[8];
   mt_select<9>(w.a0, w.a1, w.a2, q & 3, cur);
   mt_select<8>(w.b0, w.b1, w.b2, (q + 1) & 3, far);  // 397 = 1 mod 4
+  uint32_t* const lo = row + q;
+  uint32_t* const hi = row + (q - kMtN);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const int g = g0 + k;
-    const uint32_t wv = (g >= kMtN) ? mt_twist(cur[k], cur[k + 1], far[k]) : cur[k];
-    row[mt_row_pos(g)] = wv;
+    const uint32_t m = 0u - (uint32_t)(g0 + k >= kMtN);
+    const uint32_t wv = (mt_twist(cur[k], cur[k + 1], far[k]) & m) | (cur[k] & ~m);
+    (q + k < kMtN ? lo : hi)[k] = wv;
     out[k] = MtLane::temper(wv);
   }
 }
@@ -894,13 +897,13 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
         break;
       }
       // ---- this cleft's 8 of the step's 32 uniforms (simfru_local.c:211), from
-      // the words fetched during the previous step when no merged-state draw
-      // moved the window
+      // the words fetched during the previous step; when merged-state draws
+      // moved the stream (E > 0, rare) the window is fetched again at the new
+      // position first, so that both cases run the same code and the common
+      // one waits for nothing but its window
       uint32_t dw[8];
-      if (E == 0)
-        mt_finish8(row, idx + 8 * c, win, dw);
-      else
-        mt_run<8>(row, idx + E + 8 * c, 8, dw);
+      if (E != 0) mt_fetch8(row, idx + E + 8 * c, win);
+      mt_finish8(row, idx + E + 8 * c, win, dw);
       {
         const int idn = idx + E + 32;
         const int adv = idn - lazy0;
