@@ -730,7 +730,7 @@ struct Ghk {
 template <class R>
 __device__ __forceinline__ Ghk<R> ghk_terms(R V) {
   const R JDHconstant = rdiv(R(1.0), R(2.0) * R(VSS) * R(Faraday) * R(1000.0));
-  const R VF_over_RT = rdiv(V, R(RT_over_F));
+  const R VF_over_RT = rdiv(V, divk(RT_over_F));
   const R VFsq_over_RT = (R(1000.0) * R(Faraday)) * VF_over_RT;
   Ghk<R> g;
   g.e = xexp(R(2.0) * VF_over_RT);
@@ -745,11 +745,11 @@ template <class R>
 __device__ __forceinline__ void fcn_fru4(R CaJSR, R s, const Ghk<R>& g, R Cai, R CaNSR, bool lo, int no,
                                          R& dj, R& ds) {
   const R JDHconstant = rdiv(R(1.0), R(2.0) * R(VSS) * R(Faraday) * R(1000.0));
-  const R Jtr = rdiv(CaNSR - CaJSR, R(tautr));
+  const R Jtr = rdiv(CaNSR - CaJSR, divk(tautr));
   const R JRyR = R(JRyRmax) * ((R)no) * (CaJSR - s);
   const R JRyRtot = qperm<QB0>(JRyR) + qperm<QB1>(JRyR) + qperm<QB2>(JRyR) + qperm<QB3>(JRyR);
-  const R Jxfer = rdiv(s - Cai, R(tauxfer));
-  const R Jss = rdiv(s - qperm<QNEXT>(s), R(tauss2ss));
+  const R Jxfer = rdiv(s - Cai, divk(tauxfer));
+  const R Jss = rdiv(s - qperm<QNEXT>(s), divk(tauss2ss));
   const R Jss_prev = qperm<QPREV>(Jss);
   const R open = lo ? R(1.0) : R(0.0);
   const R JDHPR = g.zero ? -R(PCa) * R(2.0) * R(1000.0) * R(Faraday) * (s - R(Cao) * R(0.341)) * open * JDHconstant
@@ -806,6 +806,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     const double span1 = a.depf[1] - a.dep0[1];
     const double span2 = a.depf[2] - a.dep0[2];
     const double tint = a.t1 - a.t0;
+    const DivK kint{tint, 1.0 / tint};  // the sub-interval length, fixed for the launch
     double dV = a.dep0[0], dCai = a.dep0[1], dNSR = a.dep0[2];
     bool lopen = ch_lcc_open(w);
     int nopen = ch_ryr_open(w);
@@ -825,13 +826,13 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       const R V = (R)dV;
       const R vm35 = V - R(35.0);
       const R e0 = R(0.012) * vm35, e1 = R(-0.05) * vm35;
-      const R e2 = rdiv(V + R(12.5), R(5.0)), e3 = rdiv(V + R(30.0), R(12.0));
+      const R e2 = rdiv(V + R(12.5), divk(5.0)), e3 = rdiv(V + R(30.0), divk(12.0));
       const R ex = xexp(c == 0 ? e0 : c == 1 ? e1 : c == 2 ? e2 : e3);  // one exp per lane
       const R alpha = R(alphacf) * qperm<QB0>(ex);
       const R beta = R(betacf) * qperm<QB1>(ex);
       const R yinf = rdiv(R(yCa_frac), R(1.0) + qperm<QB2>(ex)) + (R(1.0) - R(yCa_frac));
       const R tau = R(60.0) + rdiv(R(340.0), R(1.0) + qperm<QB3>(ex));
-      const R bp = rdiv(beta, R(bL * 1.0));
+      const R bp = rdiv(beta, divk(bL * 1.0));
       const R cc = (R)css;
       const int ls = ch_lcc(w);
       bad |= (ls < 1 || ls > 12);
@@ -852,6 +853,15 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       const int n0 = qperm<QB0>(nd), n1 = qperm<QB1>(nd), n2 = qperm<QB2>(nd), n3 = qperm<QB3>(nd);
       const int E = n0 + n1 + n2 + n3;
       const int lazy0 = idx > kMtN ? idx : kMtN;
+      // totals of the resolved RyR states (fru_rates_local.c:390-727); they
+      // depend on CaSS alone, so they are computed ahead of the merged draws
+      const R T2 = (R(k21) + k.k23 + k.k25);
+      const R T3 = (R(k32) + k.k34);
+      const R T5 = (R(k52) + k.k54 + k.k56);
+      const R a7 = rdiv(k.k34, R(k43) + k.k34);
+      const R T7 = ((R(1.0) - a7) * R(k32)) + (a7 * R(k45));
+      const R a8 = rdiv(R(k65), k.k56 + R(k65));
+      const R T8 = (a8 * R(k52)) + (a8 * k.k54);
       if (nd) {
         const int Ec = (c > 0 ? n0 : 0) + (c > 1 ? n1 : 0) + (c > 2 ? n2 : 0);
         uint32_t vw[kNryr];
@@ -865,13 +875,6 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
           }
         }
       }
-      const R T2 = (R(k21) + k.k23 + k.k25);
-      const R T3 = (R(k32) + k.k34);
-      const R T5 = (R(k52) + k.k54 + k.k56);
-      const R a7 = rdiv(k.k34, R(k43) + k.k34);
-      const R T7 = ((R(1.0) - a7) * R(k32)) + (a7 * R(k45));
-      const R a8 = rdiv(R(k65), k.k56 + R(k65));
-      const R T8 = (a8 * R(k52)) + (a8 * k.k54);
       R tr[kNryr];
 #pragma unroll
       for (int r = 0; r < kNryr; ++r) {
@@ -883,7 +886,6 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       maxr = rmax(maxr, qperm<QX2>(maxr));
       bad |= qperm<QX1>(bad);
       bad |= qperm<QX2>(bad);
-      if (bad) break;
       // ---- step length (simfru_local.c:135-146)
       double dt = 0.1 / (double)maxr;
       dt = rmin(dt, 10.001e-3);
@@ -892,7 +894,8 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       const double cmax = rmax(0.e-3, cm);
       if (cmax > 1.5e-3) dt = rmin(dt, 5.001e-3);
       dt = rmin(dt, a.t1 - t);
-      if (!(t + dt > t) || ++nguard > kStepCap) {  // non-finite rates or runaway (quad-uniform)
+      // an unknown channel state, non-finite rates or a runaway (quad-uniform)
+      if (bad || !(t + dt > t) || ++nguard > kStepCap) {
         bad = 1;
         break;
       }
@@ -918,34 +921,48 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       // idx-32 .. idx-1) or any store before their use touches.  The 16-byte
       // chunks also bring up to three unused words on either side.
       mt_fetch8(row, idx + 8 * c, win);
+      // ---- transition decisions of this cleft (simfru_local.c:215-276); they
+      // use only the step-start state, so they are taken ahead of the Heun step
+      // in the same straight-line code and the two interleave
+      const double u0 = unif(dw[0]);
+      const bool mv_lcc = u0 > 1.0 - dt * (double)lt;
+      const bool mv_y = unif(dw[1]) >= 1.0 - dt * (double)vr;
+      const bool mv_ito = unif(dw[7]) >= 1.0 - dt * (double)ir;
+      double ur[kNryr];
+      bool mv_r[kNryr];
+      bool moved = false;
+#pragma unroll
+      for (int r = 0; r < kNryr; ++r) {
+        ur[r] = unif(dw[2 + r]);
+        mv_r[r] = ur[r] > 1.0 - dt * (double)tr[r];
+        moved |= mv_r[r];
+      }
       // ---- Heun step (simfru_local.c:149-173)
       R djs, dss;
       fcn_fru4<R>((R)cj, cc, gv, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
       const double k1j = dt * (double)djs, k1s = dt * (double)dss;
       const double y1j = cj + k1j, y1s = css + k1s;
       t = t + dt;
-      dV = a.dep0[0] + (t - a.t0) / tint * span0;
-      dCai = a.dep0[1] + (t - a.t0) / tint * span1;
-      dNSR = a.dep0[2] + (t - a.t0) / tint * span2;
+      const double frac = rdiv(t - a.t0, kint);  // (t - t0) / tint, correctly rounded
+      dV = a.dep0[0] + frac * span0;
+      dCai = a.dep0[1] + frac * span1;
+      dNSR = a.dep0[2] + frac * span2;
       gv = ghk_terms<R>((R)dV);  // also the next step's first evaluation
       fcn_fru4<R>((R)y1j, (R)y1s, gv, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
       cj = cj + (k1j + dt * (double)djs) / 2.0;
       css = css + (k1s + dt * (double)dss) / 2.0;
-      // ---- transitions of this cleft (simfru_local.c:215-276)
-      if (unif(dw[0]) > 1.0 - dt * (double)lt) w = ch_set_lcc(w, lcc_move<R>(ls, unif(dw[0]), dt, lb, lf, lm, lt));
-      if (unif(dw[1]) >= 1.0 - dt * (double)vr) w = ch_flip_y(w);
-      bool moved = false;
+      // ---- the transitions taken
+      if (mv_lcc) w = ch_set_lcc(w, lcc_move<R>(ls, u0, dt, lb, lf, lm, lt));
+      if (mv_y) w = ch_flip_y(w);
 #pragma unroll
       for (int r = 0; r < kNryr; ++r) {
-        const double u = unif(dw[2 + r]);
-        if (u > 1.0 - dt * (double)tr[r]) {
+        if (mv_r[r]) {
           Exits<R> e;
           ryr_exits(st[r], k, e);
-          w = ch_set_ryr(w, r, pick<R>(u, dt, e));
-          moved = true;
+          w = ch_set_ryr(w, r, pick<R>(ur[r], dt, e));
         }
       }
-      if (unif(dw[7]) >= 1.0 - dt * (double)ir) w = ch_flip_ito2(w);
+      if (mv_ito) w = ch_flip_ito2(w);
       lopen = ch_lcc_open(w);
       if (moved) nopen = ch_ryr_open(w);
       steps += (c == 0);
