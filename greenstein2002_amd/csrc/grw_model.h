@@ -71,7 +71,30 @@ template <class T> GRW_HD T rmin(T a, T b) { return (a < b) ? a : b; }
 // drop-in keeps the reference's operations bit for bit); in the FP32 mixed form
 // (device only) the hardware reciprocal and exp2 (v_rcp_f32, v_exp_f32, about
 // 1 ulp), whose error is far below the FP32 rounding the mixed form accepts.
-GRW_HD double rdiv(double a, double b) { return a / b; }
+//
+// On the device the FP64 division is the hardware's IEEE sequence (v_rcp_f64,
+// two Newton steps, the quotient and one fma correction) without its scaling
+// and fix-up instructions: v_div_scale leaves both operands unchanged and
+// v_div_fmas is a plain fma unless an operand is zero, infinite or denormal,
+// their exponents lie 768 or more apart, or the numerator is within 2^53 of
+// the smallest normal, and v_div_fixup then returns the quotient with the sign
+// it already has.  The rate and buffer terms divided here are finite, nonzero
+// denominators of moderate size (1 + exp(.), tau >= 60, (c + K)^2, rate sums
+// with a positive constant) and numerators that are moderate or +0, so the
+// eight operations give IEEE a / b bit for bit.  (The one quotient that can
+// see a zero denominator, the GHK factor at V = 0, is replaced by its limit
+// form there and never used.)
+GRW_HD double rdiv(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r0 = __builtin_amdgcn_rcp(b);
+  const double r1 = fma(r0, fma(-b, r0, 1.0), r0);
+  const double r2 = fma(r1, fma(-b, r1, 1.0), r1);
+  const double q0 = a * r2;
+  return fma(fma(-b, q0, a), r2, q0);
+#else
+  return a / b;
+#endif
+}
 __device__ __forceinline__ float rdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // Division by a model constant b, with rb = RN(1/b) fixed at compile time.  On
