@@ -1200,11 +1200,14 @@ __device__ __forceinline__ void stage_bounds(int s, double start, double h, doub
   }
 }
 
-// fcn (fcn.c:29-1072) for the integrator tails, by a whole block: thread 0
-// prepares the arguments of its transcendental calls, the lanes of the first
-// wavefront evaluate them side by side (one exp, one log and one pow call for
-// all), thread 0 finishes.  The values are those of the one-thread fcn.  Call
-// from every thread; J, st and cur are read / written by thread 0 only.
+// fcn (fcn.c:29-1072) for the integrator tails, by a whole block of 256
+// threads: thread 0 prepares the arguments of its transcendental calls, the
+// lanes of the first wavefront evaluate them side by side (one exp, one log
+// and one pow call for all), then the four independent groups of fcn_finish
+// run on the four wavefronts (lane 0 of each: currents, Ito1 lattices, gates,
+// Ca handling) and thread 0 assembles the derivatives.  The values are those
+// of the one-thread fcn.  Call from every thread; J and cur are read / written
+// by thread 0 only, st is read by all and written by thread 0.
 __device__ __forceinline__ void fcn_block(double t, double* st, double* F, double* cur, bool keepc, const double* J,
                           const GrwProtocol& p, FcnTrans* q) {
   const int j = threadIdx.x;
@@ -1214,7 +1217,27 @@ __device__ __forceinline__ void fcn_block(double t, double* st, double* F, doubl
   if (j < kFcnLog) q->lg[j] = log(q->lg_arg[j]);
   if (j < kFcnPow) q->pw[j] = pow(q->pw_base[j], q->pw_exp[j]);
   __syncthreads();
-  if (j == 0) fcn_finish(t, st, F, cur, keepc, J, p, *q);
+  // fcn_finish with its four independent groups on the block's four wavefronts
+  __shared__ FcnCa ca_sh;
+  __shared__ double mset_sh;
+  double V, Istim;
+  FcnCurrents c;
+  if (j == 0) {
+    fcn_voltage(t, st, p, V, Istim);
+    fcn_currents(st, V, *q, c);
+  } else if (j == 64) {
+    fcn_ito(st, *q, F);
+  } else if (j == 128) {
+    fcn_voltage(t, st, p, V, Istim);
+    fcn_gates(st, V, *q, F, &mset_sh);
+  } else if (j == 192) {
+    fcn_ca(st, p, *q, F, ca_sh);
+  }
+  __syncthreads();
+  if (j == 0) {
+    if (mset_sh >= 0.0) st[1] = mset_sh;
+    fcn_assemble(st, V, Istim, c, ca_sh, F, cur, keepc, J, p);
+  }
   __syncthreads();
 }
 

@@ -153,43 +153,22 @@ GRW_HD void fcn_trans(FcnTrans& q) {
 
 // fcn.c:29-1072 after its transcendental calls: every exp / log / pow whose
 // argument depends on the state is taken from q (fcn_prep + fcn_trans), every
-// other operation is done here in the reference's order.
-GRW_HD void fcn_finish(double t, double* state, double* F, double* cur, bool keepc, const double J[4],
-                       const GrwProtocol& p, const FcnTrans& q) {
-  const double Jxfer = J[0], Jtr = J[1], ICa = J[2], Ito2 = J[3];
-  const double GKr = 0.029, GKs = 0.035, GK1 = 3.0, GKp = 1.2 * 0.002216, GNa = 12.8;
-  const double kNaCa = 0.9 * 0.30, KmNa = 87.5, KmCa = 1.38, KmK1 = 13.0, ksat = 0.2, eta = 0.35;
-  const double INaKmax = 1.3 * 0.693, KmNai = 10.0, KmKo = 1.5, IpCamax = 0.6 * 0.05, KmpCa = 0.0005;
-  const double GCab = 3.3 * 7.684e-5, GNab = 0.001;
-  const double Kfb = 0.26e-3, Krb = 1.8, KSR = 1.0, Nfb = 0.75, Nrb = 0.75;
-  const double vmax0 = 1.53 * 137.0e-6;
-  const double vmaxf = (p.serca_scale != 1.0) ? vmax0 * p.serca_scale : vmax0;
-  const double vmaxr = vmaxf;
-  const double KvScale = 1.13 * 1.03 * 1.55, Kv43Frac = 0.77;
-  const double GKv43 = Kv43Frac * KvScale * 0.1;
-  const double PKv14 = (1.0 - Kv43Frac) * KvScale * 4.792933e-7;
-  const double f43[4] = {1.8936, 14.224647456, 158.574378389, 142.936645351};
-  const double b43[4] = {6.77348, 15.6212705152, 28.7532603313, 524.576206679};
-  const double f14[4] = {0.20005, 0.320280, 13.50909223, 1151.7651385};
-  const double b14[4] = {2.2300, 12.000299, 5.3701338025, 5.2396395511};
-  const double TH = 5.320000001;
-  const double C2H_to_C3H = TH * 0.02608362043337, C3H_to_C2H = TH * 0.14832978132145;
+// other operation is done in the reference's order.  The work falls into four
+// groups that share no intermediate values -- the membrane currents, the two
+// Ito1 lattices, the Na / HERG / IKs gates and the myoplasmic Ca handling --
+// and an assembly of the concentration and voltage derivatives from them;
+// fcn_finish runs them in turn, the device integrator runs the four groups on
+// the four wavefronts of a block (fcn_block in grw_fru.hip), with the same
+// values.
 
-  double mNa = state[1];
-  const double hNa = state[2], jNa = state[3], Nai = state[4], Ki = state[5];
-  double Cai = state[6];
-  const double CaNSR = state[7], xKs = state[8], LTRPNCa = state[9], HTRPNCa = state[10];
-  double k43x[2][5], k14x[2][5];
-  ito_unpack(state, 11, k43x);
-  ito_unpack(state, 21, k14x);
-  const double C1H = state[32], C2H = state[33], C3H = state[34], OH = state[35];
-  const double IH = 1.0 - OH - C1H - C2H - C3H;
-  double V = state[0];
-
+// Membrane potential (voltage clamp applied, fcn.c:448-483) and stimulus
+// current (fcn.c:434-443) at time t.
+GRW_HD void fcn_voltage(double t, const double* state, const GrwProtocol& p, double& V, double& Istim) {
   const double ton = floor((t - p.shift) / p.period) * p.period;  // :434-443
   const double toff = ton + p.pulse_duration;
-  double Istim = 0.0;
+  Istim = 0.0;
   if ((t - p.shift) >= ton && (t - p.shift) <= toff) Istim = Istim + p.pulse_amplitude;
+  V = state[0];
   if (p.vclamp) {  // :448-483
     const double von = floor((t - p.vclamp_shift) / p.vclamp_period) * p.vclamp_period;
     const double voff = von + p.vclamp_duration;
@@ -198,6 +177,25 @@ GRW_HD void fcn_finish(double t, double* state, double* F, double* cur, bool kee
     else
       V = p.vclamp_hold;
   }
+}
+
+struct FcnCurrents {
+  double INa, IKr, IKs, IKv43, IKv14, IKv14_K, IKv14_Na, IK1, INab, IKp, INaK, INaCa, ICab, IpCa;
+};
+
+// The sarcolemmal currents before the CHF scaling (fcn.c:485-746).
+GRW_HD void fcn_currents(const double* state, double V, const FcnTrans& q, FcnCurrents& c) {
+  const double GKr = 0.029, GKs = 0.035, GK1 = 3.0, GKp = 1.2 * 0.002216, GNa = 12.8;
+  const double kNaCa = 0.9 * 0.30, KmNa = 87.5, KmCa = 1.38, KmK1 = 13.0, ksat = 0.2;
+  const double INaKmax = 1.3 * 0.693, KmKo = 1.5, IpCamax = 0.6 * 0.05, KmpCa = 0.0005;
+  const double GCab = 3.3 * 7.684e-5, GNab = 0.001;
+  const double KvScale = 1.13 * 1.03 * 1.55, Kv43Frac = 0.77;
+  const double GKv43 = Kv43Frac * KvScale * 0.1;
+  const double PKv14 = (1.0 - Kv43Frac) * KvScale * 4.792933e-7;
+  double mNa = state[1];
+  const double hNa = state[2], jNa = state[3], Nai = state[4], Ki = state[5];
+  double Cai = state[6];
+  const double xKs = state[8], OH = state[35];
 
   const double ENa = RT_over_F * q.lg[0];
   const double EK = RT_over_F * q.lg[1];
@@ -206,48 +204,82 @@ GRW_HD void fcn_finish(double t, double* state, double* F, double* cur, bool kee
   const double ECa = 0.5 * RT_over_F * q.lg[3];
 
   if (mNa <= 0.0) mNa = 1.e-15;
-  const double INa = GNa * (mNa * mNa * mNa) * hNa * jNa * (V - ENa);
+  c.INa = GNa * (mNa * mNa * mNa) * hNa * jNa * (V - ENa);
   const double fKo = pow(Ko / 4.0, 0.5);
-  const double IKr = GKr * fKo * OH * (V - EK);
-  const double IKs = GKs * (xKs * xKs) * (V - EKs);
-  double IKv43 = GKv43 * k43x[0][4] * (V - EK);
-  double VF_over_RT = V / RT_over_F;
+  c.IKr = GKr * fKo * OH * (V - EK);
+  c.IKs = GKs * (xKs * xKs) * (V - EKs);
+  c.IKv43 = GKv43 * state[11 + 4] * (V - EK);
+  const double VF_over_RT = V / RT_over_F;
   const double VFsq_over_RT = (1000.0 * Faraday) * VF_over_RT;
   const double exp_VFRT = q.ex[0];
-  const double OK14 = k14x[0][4];
+  const double OK14 = state[21 + 4];
   double a1, a2, a3, a4, a5;
   a1 = Ki * exp_VFRT - Ko;
   a2 = exp_VFRT - 1.0;
-  const double IKv14_K = (fabs(V) < 1.e-6) ? PKv14 * OK14 * 1000.0 * Faraday * (Ki - Ko)
-                                            : PKv14 * OK14 * VFsq_over_RT * (a1 / a2);
+  c.IKv14_K = (fabs(V) < 1.e-6) ? PKv14 * OK14 * 1000.0 * Faraday * (Ki - Ko)
+                                : PKv14 * OK14 * VFsq_over_RT * (a1 / a2);
   a1 = Nai * exp_VFRT - Nao;
-  const double IKv14_Na = (fabs(V) < 1.e-6) ? 0.02 * PKv14 * OK14 * 1000.0 * Faraday * (Nai - Nao)
-                                             : 0.02 * PKv14 * OK14 * VFsq_over_RT * (a1 / a2);
-  const double IKv14 = IKv14_K + IKv14_Na;
-  double Ito1 = IKv43 + IKv14;
+  c.IKv14_Na = (fabs(V) < 1.e-6) ? 0.02 * PKv14 * OK14 * 1000.0 * Faraday * (Nai - Nao)
+                                 : 0.02 * PKv14 * OK14 * VFsq_over_RT * (a1 / a2);
+  c.IKv14 = c.IKv14_K + c.IKv14_Na;
   const double K1_inf = 1.0 / (0.94 + exp(1.76 / RT_over_F * (V - EK)));
-  double IK1 = GK1 * Ko / (Ko + KmK1) * K1_inf * (V - EK);
-  const double INab = GNab * (V - ENa);
+  c.IK1 = GK1 * Ko / (Ko + KmK1) * K1_inf * (V - EK);
+  c.INab = GNab * (V - ENa);
   const double KpV = 1.0 / (1.0 + q.ex[1]);
-  const double IKp = GKp * KpV * (V - EK);
+  c.IKp = GKp * KpV * (V - EK);
 
-  VF_over_RT = V / RT_over_F;
   const double sigma = (exp(Nao / 67.3) - 1.0) / 7.0;
   a1 = 1.0 + 0.1245 * q.ex[2];
   a2 = 0.0365 * sigma / exp_VFRT;
   const double fNaK = 1.0 / (a1 + a2);
   a1 = Ko / (Ko + KmKo);
   a2 = 1.0 + q.pw[0];
-  const double INaK = INaKmax * fNaK * (a1 / a2);
+  c.INaK = INaKmax * fNaK * (a1 / a2);
   const double exp_eta = q.ex[3];
   a1 = exp_eta * (Nai * Nai * Nai) * Cao;
   a2 = exp_eta / exp_VFRT * (Nao * Nao * Nao) * Cai;
   a3 = 1.0 + ksat * exp_eta / exp_VFRT;
   a4 = KmCa + Cao;
   a5 = 5000.0 / (KmNa * KmNa * KmNa + Nao * Nao * Nao);
-  double INaCa = kNaCa * a5 * (a1 - a2) / (a4 * a3);
-  const double ICab = GCab * (V - ECa);
-  const double IpCa = IpCamax * Cai / (KmpCa + Cai);
+  c.INaCa = kNaCa * a5 * (a1 - a2) / (a4 * a3);
+  c.ICab = GCab * (V - ECa);
+  c.IpCa = IpCamax * Cai / (KmpCa + Cai);
+}
+
+// The two Ito1 lattices (fcn.c:860-998): F[11..30].
+GRW_HD void fcn_ito(const double* state, const FcnTrans& q, double* F) {
+  const double f43[4] = {1.8936, 14.224647456, 158.574378389, 142.936645351};
+  const double b43[4] = {6.77348, 15.6212705152, 28.7532603313, 524.576206679};
+  const double f14[4] = {0.20005, 0.320280, 13.50909223, 1151.7651385};
+  const double b14[4] = {2.2300, 12.000299, 5.3701338025, 5.2396395511};
+  double k43x[2][5], k14x[2][5];
+  ito_unpack(state, 11, k43x);
+  ito_unpack(state, 21, k14x);
+  ItoRates iq;
+  double d43[2][5], d14[2][5];
+  ito_rates(0.543708 * q.ex[27], 0.080185 * q.ex[28], 0.0498424 * q.ex[29], 0.000819482 * q.ex[30], f43,
+            b43, iq);
+  ito_chain(k43x, iq, d43);
+  ito_rates(6.0 * 0.31551 * q.ex[31], 6.0 * 0.001966 * q.ex[32], 6.0 * 0.0004938, 6.0 * 0.0000176, f14, b14,
+            iq);
+  ito_chain(k14x, iq, d14);
+  ito_pack(d43, 11, F);
+  ito_pack(d14, 21, F);
+}
+
+// The INa gates, the HERG chain and xKs (fcn.c:640-778): F[1..3], F[8],
+// F[32..36].  Below -90 mV the m gate is set to its steady state
+// (fcn.c:663-669): *m_set receives that value, else -1.
+GRW_HD void fcn_gates(const double* state, double V, const FcnTrans& q, double* F, double* m_set) {
+  const double TH = 5.320000001;
+  const double C2H_to_C3H = TH * 0.02608362043337, C3H_to_C2H = TH * 0.14832978132145;
+  double mNa = state[1];
+  const double hNa = state[2], jNa = state[3], xKs = state[8];
+  const double C1H = state[32], C2H = state[33], C3H = state[34], OH = state[35];
+  const double IH = 1.0 - OH - C1H - C2H - C3H;
+  if (mNa <= 0.0) mNa = 1.e-15;
+  double a1, a2, a3;
+  *m_set = -1.0;
 
   double alpha_m;
   if (fabs(V + 47.13) <= 1.e-4) alpha_m = 0.32 / (0.1 - 0.005 * (V + 47.13));
@@ -258,8 +290,7 @@ GRW_HD void fcn_finish(double t, double* state, double* F, double* cur, bool kee
     dmNa = alpha_m * (1.0 - mNa) - beta_m * mNa;
   } else {
     dmNa = 0.0;
-    mNa = alpha_m / (alpha_m + beta_m);
-    state[1] = mNa;
+    *m_set = alpha_m / (alpha_m + beta_m);
   }
   const double alpha_h = 0.135 * q.ex[6];
   const double beta_h = (V < -38.73809636838782) ? 3.56 * q.ex[7] + 310000 * q.ex[8]
@@ -302,35 +333,64 @@ GRW_HD void fcn_finish(double t, double* state, double* F, double* cur, bool kee
   const double tau_xKs = 1.0 / (a1 + a2);
   const double dxKs = (xKs_inf - xKs) / tau_xKs;
 
+  F[1] = dmNa; F[2] = dhNa; F[3] = djNa; F[8] = dxKs;
+  F[32] = dC1H; F[33] = dC2H; F[34] = dC3H; F[35] = dOH;
+  F[36] = -dC1H - dC2H - dC3H - dOH;
+}
+
+struct FcnCa {
+  double Jup, Jtrpn, beta_i;
+};
+
+// SERCA uptake before the CHF scaling, troponin and the myoplasmic buffer
+// factor (fcn.c:748-778): F[9], F[10].
+GRW_HD void fcn_ca(const double* state, const GrwProtocol& p, const FcnTrans& q, double* F, FcnCa& ca) {
+  const double KSR = 1.0;
+  const double vmax0 = 1.53 * 137.0e-6;
+  const double vmaxf = (p.serca_scale != 1.0) ? vmax0 * p.serca_scale : vmax0;
+  const double vmaxr = vmaxf;
+  double Cai = state[6];
+  const double LTRPNCa = state[9], HTRPNCa = state[10];
+  if (Cai <= 0.0) Cai = 1.e-15;
   const double fb = q.pw[1];
   const double rb = q.pw[2];
-  double Jup = KSR * (vmaxf * fb - vmaxr * rb) / (1.0 + fb + rb);
+  ca.Jup = KSR * (vmaxf * fb - vmaxr * rb) / (1.0 + fb + rb);
   const double dLTRPNCa = kltrpn_plus * Cai * (1.0 - LTRPNCa) - kltrpn_minus * LTRPNCa;
   const double dHTRPNCa = khtrpn_plus * Cai * (1.0 - HTRPNCa) - khtrpn_minus * HTRPNCa;
-  const double Jtrpn = LTRPNtot * dLTRPNCa + HTRPNtot * dHTRPNCa;
-  a1 = CMDNtot * KmCMDN / (pow(Cai + KmCMDN, 2.0));
-  a2 = EGTAtot * KmEGTA / (pow(Cai + KmEGTA, 2.0));
-  const double beta_i = 1.0 / (1.0 + a1 + a2);
+  ca.Jtrpn = LTRPNtot * dLTRPNCa + HTRPNtot * dHTRPNCa;
+  const double a1 = CMDNtot * KmCMDN / (pow(Cai + KmCMDN, 2.0));
+  const double a2 = EGTAtot * KmEGTA / (pow(Cai + KmEGTA, 2.0));
+  ca.beta_i = 1.0 / (1.0 + a1 + a2);
+  F[9] = dLTRPNCa;
+  F[10] = dHTRPNCa;
+}
 
+// The CHF scaling (fcn.c:780-786), the concentration and voltage derivatives
+// F[0], F[4..7], F[31] and the currents record (fcn.c:788-1071).  Under
+// voltage clamp state[0] takes the clamp potential (fcn.c:811-814).
+GRW_HD void fcn_assemble(double* state, double V, double Istim, FcnCurrents c, FcnCa ca, double* F,
+                         double* cur, bool keepc, const double J[4], const GrwProtocol& p) {
+  const double Jxfer = J[0], Jtr = J[1], ICa = J[2], Ito2 = J[3];
+  double Ito1 = c.IKv43 + c.IKv14;
   if (p.chf) {  // :780-786
-    IK1 = 0.68 * IK1;
-    Jup = 0.38 * Jup;
-    INaCa = 1.75 * INaCa;
-    IKv43 = 0.143 * IKv43;
-    Ito1 = IKv43 + IKv14;
+    c.IK1 = 0.68 * c.IK1;
+    ca.Jup = 0.38 * ca.Jup;
+    c.INaCa = 1.75 * c.INaCa;
+    c.IKv43 = 0.143 * c.IKv43;
+    Ito1 = c.IKv43 + c.IKv14;
   }
-
+  double a1, a2, a3;
   a1 = Acap / (Vmyo * Faraday * 1000.0);
-  const double dNai = -(INa + INab + 3.0 * (INaCa + INaK) + IKv14_Na) * a1;
-  a3 = IKr + IKs + IK1 + IKp;
-  const double dKi = -(a3 - 2.0 * INaK + (Ito1 - IKv14_Na) + Istim) * a1;
-  a3 = ICab - 2.0 * INaCa + IpCa;
-  const double dCai = beta_i * (Jxfer * VSS / Vmyo - Jup - Jtrpn - a3 * 0.5 * a1);
-  const double dCaNSR = Jup * Vmyo / VNSR - Jtr * VJSR / VNSR;
+  const double dNai = -(c.INa + c.INab + 3.0 * (c.INaCa + c.INaK) + c.IKv14_Na) * a1;
+  a3 = c.IKr + c.IKs + c.IK1 + c.IKp;
+  const double dKi = -(a3 - 2.0 * c.INaK + (Ito1 - c.IKv14_Na) + Istim) * a1;
+  a3 = c.ICab - 2.0 * c.INaCa + c.IpCa;
+  const double dCai = ca.beta_i * (Jxfer * VSS / Vmyo - ca.Jup - ca.Jtrpn - a3 * 0.5 * a1);
+  const double dCaNSR = ca.Jup * Vmyo / VNSR - Jtr * VJSR / VNSR;
   const double dCaTOT = -(a3 + ICa) * 0.5 * a1 * Vmyo * 1.0e6;
-  a1 = INa + ICa + IKr + IKs;
-  a2 = IK1 + IKp + INaCa + INaK + Ito1 + Ito2;
-  a3 = IpCa + ICab + INab;
+  a1 = c.INa + ICa + c.IKr + c.IKs;
+  a2 = c.IK1 + c.IKp + c.INaCa + c.INaK + Ito1 + Ito2;
+  a3 = c.IpCa + c.ICab + c.INab;
   double dV, Itot;
   if (p.vclamp) {
     dV = 0.0;
@@ -340,33 +400,30 @@ GRW_HD void fcn_finish(double t, double* state, double* F, double* cur, bool kee
     Itot = a1 + a2 + a3 + Istim;
     dV = -Itot;
   }
-
-  ItoRates iq;
-  double d43[2][5], d14[2][5];
-  ito_rates(0.543708 * q.ex[27], 0.080185 * q.ex[28], 0.0498424 * q.ex[29], 0.000819482 * q.ex[30], f43,
-            b43, iq);
-  ito_chain(k43x, iq, d43);
-  ito_rates(6.0 * 0.31551 * q.ex[31], 6.0 * 0.001966 * q.ex[32], 6.0 * 0.0004938, 6.0 * 0.0000176, f14, b14,
-            iq);
-  ito_chain(k14x, iq, d14);
-
-  F[0] = dV; F[1] = dmNa; F[2] = dhNa; F[3] = djNa; F[4] = dNai; F[5] = dKi; F[6] = dCai;
-  F[7] = dCaNSR; F[8] = dxKs; F[9] = dLTRPNCa; F[10] = dHTRPNCa;
-  ito_pack(d43, 11, F);
-  ito_pack(d14, 21, F);
-  F[31] = dCaTOT; F[32] = dC1H; F[33] = dC2H; F[34] = dC3H; F[35] = dOH;
-  F[36] = -dC1H - dC2H - dC3H - dOH;
-
+  F[0] = dV; F[4] = dNai; F[5] = dKi; F[6] = dCai; F[7] = dCaNSR; F[31] = dCaTOT;
   if (keepc && cur) {  // :1046-1071
-    cur[0] = INa; cur[1] = IKr; cur[2] = IKs; cur[3] = Ito1; cur[4] = IK1; cur[5] = IKp;
-    cur[6] = INaCa; cur[7] = INaK; cur[8] = IpCa; cur[9] = ICab; cur[10] = INab; cur[11] = ICa;
+    cur[0] = c.INa; cur[1] = c.IKr; cur[2] = c.IKs; cur[3] = Ito1; cur[4] = c.IK1; cur[5] = c.IKp;
+    cur[6] = c.INaCa; cur[7] = c.INaK; cur[8] = c.IpCa; cur[9] = c.ICab; cur[10] = c.INab; cur[11] = ICa;
     cur[12] = ICa * Acap / (-2.0 * Vmyo * Faraday * 1000.0);
-    cur[13] = Jup; cur[14] = Jtrpn; cur[15] = Jtr * VJSR / Vmyo; cur[16] = Jxfer * VSS / Vmyo;
-    cur[17] = IKv43; cur[18] = IKv14; cur[19] = IKv14_K; cur[20] = IKv14_Na; cur[21] = Ito2;
+    cur[13] = ca.Jup; cur[14] = ca.Jtrpn; cur[15] = Jtr * VJSR / Vmyo; cur[16] = Jxfer * VSS / Vmyo;
+    cur[17] = c.IKv43; cur[18] = c.IKv14; cur[19] = c.IKv14_K; cur[20] = c.IKv14_Na; cur[21] = Ito2;
     cur[22] = Istim; cur[23] = Itot;
   }
 }
 
+GRW_HD void fcn_finish(double t, double* state, double* F, double* cur, bool keepc, const double J[4],
+                       const GrwProtocol& p, const FcnTrans& q) {
+  double V, Istim, m_set;
+  fcn_voltage(t, state, p, V, Istim);
+  FcnCurrents c;
+  fcn_currents(state, V, q, c);
+  fcn_ito(state, q, F);
+  fcn_gates(state, V, q, F, &m_set);
+  FcnCa ca;
+  fcn_ca(state, p, q, F, ca);
+  if (m_set >= 0.0) state[1] = m_set;
+  fcn_assemble(state, V, Istim, c, ca, F, cur, keepc, J, p);
+}
 
 // fcn.c:29-1072.  J = {Jxfer, Jtr, ICa, Ito2} from the release units.  Mutates
 // state[1] below -90 mV (:663-669) and state[0] under voltage clamp (:811-814).
