@@ -60,6 +60,7 @@ class GrwStats(C.Structure):
         ("kernel_ms", C.c_double),
         ("critical_steps", C.c_uint64),
         ("tail_ms", C.c_double),
+        ("timed_launches", C.c_uint64),
     ]
 
 

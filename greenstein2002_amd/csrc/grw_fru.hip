@@ -1201,9 +1201,9 @@ __device__ __forceinline__ void stage_bounds(int s, double start, double h, doub
 }
 
 // fcn (fcn.c:29-1072) for the integrator tails, by a whole block of 256
-// threads: thread 0 prepares the arguments of its transcendental calls, the
-// lanes of the first wavefront evaluate them side by side (one exp, one log
-// and one pow call for all), then the four independent groups of fcn_finish
+// threads: thread 0 prepares the arguments of its transcendental calls, three
+// wavefronts evaluate them side by side (the exps, the logs and the pows, one
+// call each), then the four independent groups of fcn_finish
 // run on the four wavefronts (lane 0 of each: currents, Ito1 lattices, gates,
 // Ca handling) and thread 0 assembles the derivatives.  The values are those
 // of the one-thread fcn.  Call from every thread; J and cur are read / written
@@ -1213,9 +1213,9 @@ __device__ __forceinline__ void fcn_block(double t, double* st, double* F, doubl
   const int j = threadIdx.x;
   if (j == 0) fcn_prep(t, st, p, *q);
   __syncthreads();
-  if (j < kFcnExp) q->ex[j] = exp(q->ex_arg[j]);
-  if (j < kFcnLog) q->lg[j] = log(q->lg_arg[j]);
-  if (j < kFcnPow) q->pw[j] = pow(q->pw_base[j], q->pw_exp[j]);
+  if (j < kFcnExp) q->ex[j] = exp(q->ex_arg[j]);  // one call kind per wavefront, side by side
+  else if (j >= 64 && j < 64 + kFcnLog) q->lg[j - 64] = log(q->lg_arg[j - 64]);
+  else if (j >= 128 && j < 128 + kFcnPow) q->pw[j - 128] = pow(q->pw_base[j - 128], q->pw_exp[j - 128]);
   __syncthreads();
   // fcn_finish with its four independent groups on the block's four wavefronts
   __shared__ FcnCa ca_sh;
@@ -1791,6 +1791,7 @@ int launch_sim(grw_ctx* c, const SimArgs& a) {
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->stats.kernel_ms += ms;
+    c->stats.timed_launches++;
   }
   return 0;
 }
@@ -1906,6 +1907,8 @@ void reset_tracking(grw_ctx* c) {
   }
 }
 
+constexpr int kTimeEvery = 8;  // the device integrator times one RK attempt in eight
+
 // Enqueue RK attempts (five k_dev_stage each, after one k_dev_begin) until every active cell
 // of c->hctl has reached its ms_target; the host synchronises once per batch of
 // attempts sized to ~25 simulated ms of the slowest cell, and launches after a
@@ -1932,14 +1935,17 @@ int run_device(grw_ctx* c) {
     }
     if (K == 0) break;
     K = std::max(4, std::min(K, 4000));
+    // timing: HIP events around the five launches of one attempt in kTimeEvery
+    // (events around every launch cost the run itself a few percent)
+    const int nsamp = (K + kTimeEvery - 1) / kTimeEvery;
     if (c->timing) {
-      while ((int)evs.size() < 10 * K) {
+      while ((int)evs.size() < 10 * nsamp) {
         hipEvent_t ev;
         if (hipEventCreate(&ev) != hipSuccess) break;
         evs.push_back(ev);
       }
     }
-    const bool timed = c->timing && (int)evs.size() >= 10 * K;
+    const bool timed = c->timing && (int)evs.size() >= 10 * nsamp;
     std::vector<char> was_active(nc);
     for (int k = 0; k < nc; ++k) was_active[k] = c->hctl[k].active != 0;
     if (first) {  // the run's first attempt; later ones start in the last stage's tail
@@ -1950,13 +1956,15 @@ int run_device(grw_ctx* c) {
     }
     first = false;
     for (int k = 0; k < K; ++k) {
+      const bool tk = timed && k % kTimeEvery == 0;
+      hipEvent_t* ek = evs.data() + 10 * (k / kTimeEvery);
       for (int s5 = 0; s5 < 5; ++s5) {
-        if (timed) (void)hipEventRecord(evs[10 * k + 2 * s5], c->stream);
+        if (tk) (void)hipEventRecord(ek[2 * s5], c->stream);
         if (mixed)
           hipLaunchKernelGGL(k_dev_stage<float>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
         else
           hipLaunchKernelGGL(k_dev_stage<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
-        if (timed) (void)hipEventRecord(evs[10 * k + 2 * s5 + 1], c->stream);
+        if (tk) (void)hipEventRecord(ek[2 * s5 + 1], c->stream);
       }
     }
     hipError_t e = hipGetLastError();
@@ -1988,12 +1996,15 @@ int run_device(grw_ctx* c) {
       if (h.ms_done > ms_prev[k] && done > 0) ms_per_att[k] = (double)(h.ms_done - ms_prev[k]) / (double)done;
       ms_prev[k] = h.ms_done;
     }
-    if (timed) {
-      for (long long k = 0; k < std::min<long long>(eff, K); ++k)
+    if (timed) {  // the sampled attempts that ran (later launches of the batch return at once)
+      for (long long k = 0; k < std::min<long long>(eff, K); k += kTimeEvery)
         for (int s5 = 0; s5 < 5; ++s5) {
           float ms = 0.f;
-          if (hipEventElapsedTime(&ms, evs[10 * k + 2 * s5], evs[10 * k + 2 * s5 + 1]) == hipSuccess)
+          const hipEvent_t* ek = evs.data() + 10 * (k / kTimeEvery);
+          if (hipEventElapsedTime(&ms, ek[2 * s5], ek[2 * s5 + 1]) == hipSuccess) {
             c->stats.kernel_ms += ms;
+            c->stats.timed_launches++;
+          }
         }
     }
     c->stats.launches += (uint64_t)(5 * eff);

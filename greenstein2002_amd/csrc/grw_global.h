@@ -88,7 +88,7 @@ GRW_HD void fcn_prep(double t, const double* state, const GrwProtocol& p, FcnTra
   const double CaNSR = state[7];
   double V = state[0];
   if (p.vclamp) {
-    const double von = floor((t - p.vclamp_shift) / p.vclamp_period) * p.vclamp_period;
+    const double von = floor(rdiv(t - p.vclamp_shift, p.vclamp_period)) * p.vclamp_period;
     const double voff = von + p.vclamp_duration;
     if (((t - p.vclamp_shift) >= von) && (von >= 0.0) && ((t - p.vclamp_shift) < voff))
       V = p.vclamp_set;
@@ -96,30 +96,31 @@ GRW_HD void fcn_prep(double t, const double* state, const GrwProtocol& p, FcnTra
       V = p.vclamp_hold;
   }
   if (Cai <= 0.0) Cai = 1.e-15;
-  q.lg_arg[0] = Nao / Nai;
-  q.lg_arg[1] = Ko / Ki;
-  q.lg_arg[2] = (Ko + 0.01833 * Nao) / (Ki + 0.01833 * Nai);
-  q.lg_arg[3] = Cao / Cai;
+  // the divisions are IEEE (rdiv, grw_model.h), in fewer device operations
+  q.lg_arg[0] = rdiv(Nao, Nai);
+  q.lg_arg[1] = rdiv(Ko, Ki);
+  q.lg_arg[2] = rdiv(Ko + 0.01833 * Nao, Ki + 0.01833 * Nai);
+  q.lg_arg[3] = rdiv(Cao, Cai);
   const double KmNai = 10.0, Kfb = 0.26e-3, Krb = 1.8, Nfb = 0.75, Nrb = 0.75;
-  q.pw_base[0] = KmNai / Nai;
+  q.pw_base[0] = rdiv(KmNai, Nai);
   q.pw_exp[0] = 1.5;
-  q.pw_base[1] = Cai / Kfb;
+  q.pw_base[1] = rdiv(Cai, divk(Kfb));
   q.pw_exp[1] = Nfb;
-  q.pw_base[2] = CaNSR / Krb;
+  q.pw_base[2] = rdiv(CaNSR, divk(Krb));
   q.pw_exp[2] = Nrb;
-  const double VF_over_RT = V / RT_over_F;
+  const double VF_over_RT = rdiv(V, divk(RT_over_F));
   const double eta = 0.35;
   double* e = q.ex_arg;
   e[0] = VF_over_RT;
-  e[1] = (7.488 - V) / 5.98;
+  e[1] = rdiv(7.488 - V, divk(5.98));
   e[2] = -0.1 * VF_over_RT;
   e[3] = eta * VF_over_RT;
   e[4] = -0.1 * (V + 47.13);
-  e[5] = -V / 11.0;
-  e[6] = (80 + V) / -6.8;
+  e[5] = rdiv(-V, divk(11.0));
+  e[6] = rdiv(80 + V, divk(-6.8));
   e[7] = 0.079 * V;
   e[8] = 0.35 * V;
-  e[9] = (V + 10.66) / -11.1;
+  e[9] = rdiv(V + 10.66, divk(-11.1));
   e[10] = 0.2444 * V;
   e[11] = -0.04391 * V;
   e[12] = 0.311 * (V + 79.23);
@@ -134,7 +135,7 @@ GRW_HD void fcn_prep(double t, const double* state, const GrwProtocol& p, FcnTra
   e[21] = 0.00568908859717 * V;
   e[22] = -0.04536642959543 * V;
   e[23] = 0.00000069808924 * V;
-  e[24] = -(V - 24.7) / 13.6;
+  e[24] = rdiv(-(V - 24.7), divk(13.6));
   e[25] = -0.148 * (V - 10.0);
   e[26] = 0.0687 * (V - 10.0);
   e[27] = 0.028983 * V;

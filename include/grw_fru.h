@@ -51,9 +51,12 @@ typedef struct {
   uint64_t fru_steps;    /* micro-steps summed over FRUs (simfru_local.c:127 iterations) */
   uint64_t simfru_calls; /* FRU sub-interval calls (FRUs x distrib calls) */
   uint64_t launches;     /* Monte Carlo kernel launches */
-  double kernel_ms;      /* summed device time of the Monte Carlo kernels */
+  double kernel_ms;      /* summed device time of the timed Monte Carlo launches */
   uint64_t critical_steps; /* sum over launches of the largest FRU-step count in the launch */
   double tail_ms;        /* summed device time of the serial launch tails (integrator) */
+  uint64_t timed_launches; /* launches that kernel_ms covers: every launch of the host-driven
+                              path; the device integrator times the five launches of one RK
+                              attempt in eight, a systematic sample */
 } grw_stats;
 
 const char *grw_last_error(void);
@@ -108,7 +111,8 @@ int grw_calc_fru_avg_local(grw_ctx *ctx, double num11[11]);
 
 int grw_get_stats(grw_ctx *ctx, grw_stats *st);
 int grw_reset_stats(grw_ctx *ctx);
-/* Enable per-launch device timing with HIP events (adds a sync per launch). */
+/* Enable device timing with HIP events: every launch of the host-driven path
+ * (adds a sync per launch), one RK attempt in eight on the device integrator. */
 int grw_set_timing(grw_ctx *ctx, int on);
 
 #ifdef __cplusplus

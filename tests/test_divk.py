@@ -20,7 +20,7 @@ _libm.fma.restype = ctypes.c_double
 _libm.fma.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
 fma = _libm.fma
 
-# the divisors of rdiv(x, divk(...)) in grw_fru.hip and grw_model.h
+# the divisors of rdiv(x, divk(...)) in grw_fru.hip, grw_model.h and grw_global.h
 DIVISORS = {
     "RT_over_F": 8.314 * 310.0 / 96.5,
     "tautr": 3.0,
@@ -31,6 +31,14 @@ DIVISORS = {
     "bL": 1.9356 * 1.0,
     "k_rate": 0.00127215,
     "k_rate2": 3.4188,
+    # fcn_prep (grw_global.h)
+    "Kfb": 0.26e-3,
+    "Krb": 1.8,
+    "5.98": 5.98,
+    "11": 11.0,
+    "-6.8": -6.8,
+    "-11.1": -11.1,
+    "13.6": 13.6,
 }
 
 
