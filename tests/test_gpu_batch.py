@@ -106,3 +106,26 @@ def test_ensemble_matches_cpu_oracle():
     assert close.all(axis=(1, 2)).sum() >= len(seeds) - 2, close.all(axis=(1, 2))
     m_got, m_ref = got[:, :, cols].mean(axis=0), ref[:, :, cols].mean(axis=0)
     np.testing.assert_allclose(m_got, m_ref, rtol=5e-3)
+
+
+def test_full_size_beat_matches_cpu_oracle():
+    """The benchmark configuration itself: 12,500 FRUs, one 1 Hz beat from
+    ic/stable (t = 0..1011 ms), six master seeds in one batch, against the CPU
+    oracle's runs on the same seeds (tests/golden/ensemble_c2.npz, made by
+    gen_ensemble.py c2 and reduced by the same beat_stats).  The FP64 path
+    consumes every seed's random streams exactly as the oracle does, so each
+    beat's statistics agree to the round-off of exp and of the flux-sum order
+    (within 2e-10 relative on all 16 seeds of the fixture); rtol 1e-7 here."""
+    from pathlib import Path
+
+    g = np.load(Path(__file__).parent / "golden" / "ensemble_c2.npz")
+    seeds, ms = g["seeds"][:6], int(g["ms"])
+    ref = g["beats"][:6]
+    specs = [CellSpec(int(g["nfru"]), seed=int(s)) for s in seeds]
+    with Batch(specs) as b:
+        b.run(ms)
+        got = np.array([b.beats(k)[: ref.shape[1], : ref.shape[2]] for k in range(len(seeds))])
+    assert got.shape == ref.shape
+    cols = [0, 1, 2, 4, 5, 7, 8, 9, 10]  # t_stim, V_rest, V_peak, APD90, Cai_peak, Cai_rest, ICa_peak, loads
+    np.testing.assert_allclose(got[:, :, cols], ref[:, :, cols], rtol=1e-7, atol=0)
+    assert np.array_equal(got[:, :, 11], ref[:, :, 11])  # the same number of accepted steps
