@@ -78,6 +78,13 @@ def parse():
                          "Ca entry, Ca release and EC-coupling gain of the step")
     ap.add_argument("--clamp-shift", type=float, default=100.0, help="ms held before the step")
     ap.add_argument("--clamp-duration", type=float, default=200.0, help="ms of the step")
+    ap.add_argument("--sweep", type=int, default=0,
+                    help="only run the parameter sweep: this many cells of --nfru FRUs (1,024 x 12,500 "
+                         "= 12.8 M FRUs for the full sweep) with RyR-opening and LCC mode-switch scales "
+                         "log-uniform in [0.5, 2] and SERCA scale uniform in [0.7, 1.3], mixed precision "
+                         "in one batch, then the first --sweep-fp64 of them again in FP64")
+    ap.add_argument("--sweep-ms", type=int, default=30, help="simulated ms of the sweep (stimulus at 10 ms)")
+    ap.add_argument("--sweep-fp64", type=int, default=64, help="cells of the sweep rerun in FP64")
     ap.add_argument("--summarize", default="",
                     help="only summarise a rocprofv3 output folder (kernel trace and / or counter "
                          "CSVs) per kernel and grid size, as markdown")
@@ -378,8 +385,79 @@ def clamp(args) -> dict:
     return out
 
 
+def sweep(args) -> dict:
+    """The many-cell parameter sweep in one batch (the streaming regime: the
+    MT rows of all cells, 2.5 KB per FRU, far exceed L2 and the Infinity
+    Cache).  Mixed precision for all cells, FP64 for the first few again with
+    the same seeds and scales; beat statistics of the window (records 2, 5, 8:
+    peak V, peak Cai, peak ICa) compared cell by cell."""
+    import numpy as np
+
+    from greenstein2002_amd.batch import Batch, CellSpec
+    from greenstein2002_amd.cell import Protocol
+
+    n = args.sweep
+    rng = np.random.default_rng(1234)
+    k23 = np.exp(rng.uniform(np.log(0.5), np.log(2.0), n))
+    gam = np.exp(rng.uniform(np.log(0.5), np.log(2.0), n))
+    serca = rng.uniform(0.7, 1.3, n)
+    specs = [CellSpec(args.nfru, seed=1234 + k, k23_scale=float(k23[k]), gamma_scale=float(gam[k]),
+                      protocol=Protocol(serca_scale=float(serca[k]))) for k in range(n)]
+
+    def run(cells, prec):
+        b = Batch(cells, precision=prec)
+        b.set_timing(True)
+        b.reset_stats()
+        t0 = time.perf_counter()
+        done = 0
+        while done < args.sweep_ms:
+            m = min(10, args.sweep_ms - done)
+            b.run(m)
+            done += m
+            print(f"[bench] sweep {prec} x {len(cells)}: t = {done} ms, {time.perf_counter() - t0:.1f} s wall",
+                  file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        st = b.stats()
+        beats = np.array([b.beats(k, include_current=True)[0] for k in range(len(cells))])
+        b.close()
+        alg = BYTES_PER_STEP * st["fru_steps"] + BYTES_PER_CALL * st["simfru_calls"]
+        avg_s = st["kernel_ms"] / 1e3 / max(st["timed_launches"], 1)
+        achieved = alg / max(st["launches"], 1) / avg_s / 1e9 if st["kernel_ms"] else 0.0
+        line = dict(precision=prec, cells=len(cells), wall_s=round(wall, 2),
+                    cell_ms_per_s=round(len(cells) * args.sweep_ms / wall, 2),
+                    fru_steps=st["fru_steps"], fru_steps_per_s=round(st["fru_steps"] / wall, 1),
+                    roofline=dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
+                                  frac=round(achieved / HBM_PEAK_GBS, 6), avg_launch_us=round(avg_s * 1e6, 3),
+                                  launches=st["launches"], timed_launches=st["timed_launches"],
+                                  alg_bytes_per_launch=round(alg / max(st["launches"], 1), 1)))
+        return line, beats
+
+    mixed, bm = run(specs, "mixed")
+    m64 = min(args.sweep_fp64, n)
+    fp64, bf = run(specs[:m64], "fp64")
+    cols = {"V_peak": 2, "Cai_peak": 5, "ICa_peak": 8}
+    diff = {}
+    for k, c in cols.items():
+        a, m = bf[:, c], bm[:m64, c]
+        rel = (m - a) / np.abs(a)
+        diff[k] = dict(fp64_mean=float(a.mean()), mixed_mean=float(m.mean()),
+                       mean_rel_err=float((m.mean() - a.mean()) / abs(a.mean())),
+                       max_abs_rel_err_per_cell=float(np.abs(rel).max()))
+    return dict(
+        metric="parameter sweep of many 12,500-FRU cells in one batch (mixed), FP64 rerun of a subset",
+        workload=f"{n} cells x {args.nfru} FRUs ({n * args.nfru:,} FRUs, {n * args.nfru * 2560 / 1e9:.1f} GB of MT "
+                 f"rows), k23 and gamma scales log-uniform [0.5, 2], SERCA uniform [0.7, 1.3] (host PRNG "
+                 f"1234), seeds 1234.., 1 Hz, t = 0..{args.sweep_ms} ms from ic/stable, stimulus at 10 ms",
+        mixed=mixed, fp64_subset=fp64, mixed_vs_fp64_subset=diff,
+        scales=dict(k23=[float(k23.min()), float(k23.max())], gamma=[float(gam.min()), float(gam.max())],
+                    serca=[float(serca.min()), float(serca.max())]))
+
+
 def main():
     args = parse()
+    if args.sweep:
+        print(json.dumps(sweep(args)), flush=True)
+        return
     if args.clamp:
         print(json.dumps(clamp(args)), flush=True)
         return

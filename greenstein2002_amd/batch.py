@@ -93,7 +93,11 @@ class Batch:
         self.attempts = np.zeros(n, np.int64)
         self.rejects = np.zeros(n, np.int64)
         base = load_stable(REWIND)
-        states = np.array([load_stable(sp.nfru, sp.scale)["state"] for sp in self.specs], np.float64)
+        by_shape = {}  # the global state depends on the FRU count and scale only
+        for sp in self.specs:
+            if (sp.nfru, sp.scale) not in by_shape:
+                by_shape[(sp.nfru, sp.scale)] = load_stable(sp.nfru, sp.scale)["state"]
+        states = np.array([by_shape[(sp.nfru, sp.scale)] for sp in self.specs], np.float64)
         self.errw = errweight()
         fru = np.ascontiguousarray(base["fru"], np.float64)
         lt = np.ascontiguousarray(base["ltype"], np.int32)
