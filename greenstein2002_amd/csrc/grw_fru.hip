@@ -151,6 +151,7 @@ struct SimArgs {
   int ncells;
   int pro;          // before the sub-interval: 1 checkpoint, 2 roll back each FRU (first RK stage)
   int avg;          // also reduce the average sums of the FRU state at the end
+  int sorted;       // quads take their FRU from d.perm and record its step count in d.cost
 };
 
 // One cell of a batch: its FRUs are [f0, f0 + nfru) of the shared arrays (f0 a
@@ -174,7 +175,7 @@ struct Dev {
   uint32_t* chan_h;
   int32_t* mtidx_h;
   uint32_t* mtw_h;
-  double* part;              // [grid][16] block partial sums
+  double* part;              // [grid][kSlot] block partial sums
   unsigned int* ticket;      // last-block election counters, one per cell (self-resetting)
   double* out;               // [32]: 0-3 fluxes, 4-8 flux sums, 16-26 avg sums
   unsigned long long* nsteps;
@@ -185,6 +186,8 @@ struct Dev {
   const CellInfo* cells;            // [ncells]
   const int* smap;                  // Monte Carlo grid block -> cell
   const int* bmap;                  // one-thread-per-FRU grid block -> cell
+  int32_t* perm;                    // [ld]: FRU (cell-local) of each quad slot of the cell, when sorted
+  uint32_t* cost;                   // [ld]: micro-steps of each FRU in its last sub-interval, when sorted
 };
 
 // Categorical transition draw, simfru_local.c:217-227 / :234-245: stay with
@@ -768,10 +771,11 @@ __device__ __forceinline__ void fcn_fru4(R CaJSR, R s, const Ghk<R>& g, R Cai, R
 template <class R>
 __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[17]) {
   const int gid = (blockIdx.x - a.b0) * blockDim.x + threadIdx.x;
-  const int f = a.f0 + (gid >> 2);
+  const int slot = gid >> 2;
   const int c = gid & 3;
   unsigned steps = 0;
-  if ((gid >> 2) < a.nfru) {  // quads are active or inactive as a whole
+  if (slot < a.nfru) {  // quads are active or inactive as a whole
+    const int f = a.f0 + (a.sorted ? d.perm[a.f0 + slot] : slot);
     uint32_t* const row = d.mt + (size_t)f * kMtStride;
     double cj, css;
     uint32_t w;
@@ -974,6 +978,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       d.conc[f] = cj;
       d.mtidx[f] = idx;
       d.nreg[f] = nreg;
+      if (a.sorted && a.t1 > a.t0) d.cost[f] = steps;
     }
     acc[0] = css;
     acc[1] = (c == 0) ? cj : 0.0;
@@ -1559,6 +1564,79 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   if (j == 0) atomicAdd(d.tailticks, wall_clock64() - tail0);
 }
 
+// Regrouping of a batch cell's FRUs between RK attempts.  A wavefront runs as
+// long as the slowest of its 16 FRUs, and a FRU that took many micro-steps in
+// one sub-interval (high CaSS, open RyRs) keeps taking many for a while; so
+// before each attempt every active cell's quad slots are refilled with its
+// FRUs ordered by the step count of their last sub-interval, in kSortBins
+// bins, FRU index order within a bin -- a stable counting sort, one block per
+// cell, the same result every run.  Every per-FRU quantity is keyed by the FRU
+// index, so only the order of the flux and average sums changes; a cell that
+// has finished its run keeps its order, so a run continued in pieces sees the
+// same orders as one unbroken run.
+constexpr int kSortBins = 8;
+
+__device__ __forceinline__ int cost_bin(uint32_t s) {
+  return (s >= 2) + (s >= 3) + (s >= 4) + (s >= 6) + (s >= 9) + (s >= 14) + (s >= 22);
+}
+
+__global__ void __launch_bounds__(256) k_sort_cells(const Dev d, const RkCtl* ctls) {
+  const int cell = blockIdx.x;
+  if (!ctls[cell].active) return;
+  const CellInfo ci = d.cells[cell];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int per = (ci.nfru + 255) / 256;
+  const int lo = min(ci.nfru, t * per), hi = min(ci.nfru, lo + per);
+  int cnt[kSortBins];
+#pragma unroll
+  for (int b = 0; b < kSortBins; ++b) cnt[b] = 0;
+  for (int i = lo; i < hi; ++i) {
+    const int bin = cost_bin(d.cost[ci.f0 + i]);
+#pragma unroll
+    for (int b = 0; b < kSortBins; ++b) cnt[b] += (bin == b);
+  }
+  // slot of this thread's first FRU of each bin: earlier bins, then earlier
+  // threads of the same bin (wavefront scan, then the waves in order)
+  __shared__ int wtot[kSortBins][4];
+  __shared__ int binbase[kSortBins];
+  int pos[kSortBins];
+#pragma unroll
+  for (int b = 0; b < kSortBins; ++b) {
+    int x = cnt[b];
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    pos[b] = x - cnt[b];
+    if (lane == 63) wtot[b][wave] = x;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int b = 0; b < kSortBins; ++b) {
+      binbase[b] = acc;
+      for (int w = 0; w < 4; ++w) acc += wtot[b][w];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < kSortBins; ++b) {
+    int off = binbase[b];
+    for (int w = 0; w < wave; ++w) off += wtot[b][w];
+    pos[b] += off;
+  }
+  for (int i = lo; i < hi; ++i) {
+    const int bin = cost_bin(d.cost[ci.f0 + i]);
+    int p = 0;
+#pragma unroll
+    for (int b = 0; b < kSortBins; ++b) {
+      p = (bin == b) ? pos[b] : p;
+      pos[b] += (bin == b);
+    }
+    d.perm[ci.f0 + p] = i;
+  }
+}
+
 // Average-output sums of calc_fru_avg_local.c:135-176 (tree order).
 __global__ void __launch_bounds__(256) k_avg(const SimArgs a, const Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1828,7 +1906,8 @@ int alloc_fru_state(grw_ctx* c) {
       (rc = dalloc(c, &c->d.ticket, (size_t)c->ncells)) || (rc = dalloc(c, &c->d.out, 32)) ||
       (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
       (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
-      (rc = dalloc(c, &c->d.tailticks, 1)))
+      (rc = dalloc(c, &c->d.tailticks, 1)) || (rc = dalloc(c, &c->d.perm, ld)) ||
+      (rc = dalloc(c, &c->d.cost, ld)))
     return rc;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_out, 32 * sizeof(double), hipHostMallocDefault);
@@ -1909,6 +1988,17 @@ void reset_tracking(grw_ctx* c) {
 
 constexpr int kTimeEvery = 8;  // the device integrator times one RK attempt in eight
 
+// The FRU order of every cell back to the identity, step counts to zero (a
+// new initial state).
+int reset_order(grw_ctx* c) {
+  std::vector<int32_t> perm((size_t)c->ld, 0);
+  for (const CellInfo& ci : c->cells)
+    for (int i = 0; i < ci.nfru; ++i) perm[(size_t)ci.f0 + i] = i;
+  HIPCHK(hipMemcpy(c->d.perm, perm.data(), perm.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->d.cost, 0, (size_t)c->ld * sizeof(uint32_t)));
+  return 0;
+}
+
 // Enqueue RK attempts (five k_dev_stage each, after one k_dev_begin) until every active cell
 // of c->hctl has reached its ms_target; the host synchronises once per batch of
 // attempts sized to ~25 simulated ms of the slowest cell, and launches after a
@@ -1918,6 +2008,7 @@ int run_device(grw_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->dctl, c->hctl.data(), sizeof(RkCtl) * nc, hipMemcpyHostToDevice, c->stream));
   SimArgs base = base_args(c);
   base.ncells = nc;
+  base.sorted = c->batch ? 1 : 0;  // batch cells regroup their FRUs between attempts
   const bool mixed = c->cfg.precision == GRW_MIXED;
   std::vector<long long> att_prev(nc, 0);
   std::vector<int> ms_prev(nc);
@@ -1953,6 +2044,7 @@ int run_device(grw_ctx* c) {
         hipLaunchKernelGGL(k_dev_begin<float>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
       else
         hipLaunchKernelGGL(k_dev_begin<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+      if (base.sorted) hipLaunchKernelGGL(k_sort_cells, dim3(nc), dim3(256), 0, c->stream, c->d, c->dctl);
     }
     first = false;
     for (int k = 0; k < K; ++k) {
@@ -1966,6 +2058,7 @@ int run_device(grw_ctx* c) {
           hipLaunchKernelGGL(k_dev_stage<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
         if (tk) (void)hipEventRecord(ek[2 * s5 + 1], c->stream);
       }
+      if (base.sorted) hipLaunchKernelGGL(k_sort_cells, dim3(nc), dim3(256), 0, c->stream, c->d, c->dctl);
     }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
@@ -2112,6 +2205,10 @@ int grw_create(const grw_config* cfg, grw_ctx** out) {
     return rc;
   }
   reset_tracking(c);
+  if (int rc2 = reset_order(c)) {
+    grw_destroy(c);
+    return rc2;
+  }
   *out = c;
   return 0;
 }
@@ -2161,6 +2258,7 @@ int grw_load_state(grw_ctx* c, const double* fru, const int32_t* ltype, const in
   HIPCHK(hipMemcpyAsync(c->d.mt, rows.data(), rows.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   reset_tracking(c);
+  if (int rc = reset_order(c)) return rc;
   return grw_save_state(c);
 }
 
@@ -2225,6 +2323,7 @@ int grw_seed(grw_ctx* c, uint32_t a0) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   reset_tracking(c);
+  if (int rc = reset_order(c)) return rc;
   return grw_save_state(c);
 }
 
@@ -2482,6 +2581,10 @@ int grw_batch_create(const grw_batch_config* cfg, const grw_cell_spec* spec, grw
   }
   for (int k = 0; k < c->ncells; ++k) c->hctl[k].prot = spec[k].prot;
   reset_tracking(c);
+  if (int rc2 = reset_order(c)) {
+    grw_destroy(c);
+    return rc2;
+  }
   *out = c;
   return 0;
 }
@@ -2525,6 +2628,7 @@ int grw_batch_load_ic(grw_ctx* c, const double* states, const double errw[37], i
   (void)hipFree(dic);
   if (e != hipSuccess) return fail(GRW_EDEVICE, std::string("initial state upload failed: ") + hipGetErrorString(e));
   reset_tracking(c);
+  if (int rc = reset_order(c)) return rc;
   for (int k = 0; k < c->ncells; ++k) {
     RkCtl& hc = c->hctl[k];
     const GrwProtocol& p = hc.prot;

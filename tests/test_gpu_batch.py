@@ -1,10 +1,12 @@
 """Batches of independent cells (include/grw_batch.h) and the per-beat
 statistics of the device integrator.
 
-A cell in a batch runs the same kernels on the same block structure as a
-single-cell context (its FRUs start at a multiple of 64 and its reductions span
-only its own blocks), so its trajectory must equal the single-cell device run
-with the same parameters bit for bit, whatever the other cells do.
+A cell in a batch runs the same kernels on the same block structure whatever
+the other cells do (its FRUs start at a multiple of 64, its reductions span
+only its own blocks, and its FRU order between attempts depends on its own
+step counts alone), so its trajectory must equal the same cell run alone as a
+one-cell batch bit for bit.  A single-cell context keeps its FRUs in index
+order, so against it only the summation order of the flux sums differs.
 """
 import numpy as np
 import pytest
@@ -21,6 +23,12 @@ def single_run(sp: CellSpec, ms: int, precision="fp64"):
               k23_scale=sp.k23_scale, gamma_scale=sp.gamma_scale, nfru_scale=sp.nfru_scale) as c:
         r = c.run(ms, log_cap=0)
         return r.states[-1], r.stats["attempts"], r.stats["rejects"], c.beats()
+
+
+def alone_run(sp: CellSpec, ms: int, precision="fp64"):
+    with Batch([sp], precision=precision) as b:
+        b.run(ms)
+        return b.state[0].copy(), int(b.attempts[0]), int(b.rejects[0]), b.beats(0)
 
 
 def test_batch_cells_equal_single_cell_runs():
@@ -41,10 +49,14 @@ def test_batch_cells_equal_single_cell_runs():
         rej = b.rejects.copy()
         beats = [b.beats(k) for k in range(len(specs))]
     for k, sp in enumerate(specs):
-        st, a, r, bt = single_run(sp, ms)
+        st, a, r, bt = alone_run(sp, ms)
         assert np.array_equal(got[k], st), f"cell {k} state differs"
         assert att[k] == a and rej[k] == r, f"cell {k} attempts {att[k]}/{a} rejects {rej[k]}/{r}"
         assert np.array_equal(beats[k], bt, equal_nan=True), f"cell {k} beat records differ"
+        st1, a1, r1, bt1 = single_run(sp, ms)
+        assert att[k] == a1 and rej[k] == r1, f"cell {k} vs single context: attempts {att[k]}/{a1}"
+        np.testing.assert_allclose(got[k], st1, rtol=1e-8, atol=1e-13)
+        np.testing.assert_allclose(beats[k], bt1, rtol=1e-7, atol=1e-12)
 
 
 def test_mixed_batch_runs():
