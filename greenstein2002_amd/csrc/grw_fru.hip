@@ -518,8 +518,11 @@ __device__ __forceinline__ void lcc_arith(int s, R alpha, R beta, R bp, R gam, R
   const R fwd_n = (k < 4) ? (R)(4 - k) * alpha : R(fL);
   const R fwd_c = (k < 4) ? (R)(2 * (4 - k)) * alpha : R(fLprime);
   fwd = ca ? fwd_c : fwd_n;
-  const R om = (k == 0) ? R(omega) : (k == 1) ? R(omega / bL) : (k == 2) ? R(omega / bL2)
-             : (k == 3) ? R(omega / bL3) : R(omega / bL4);
+  R om = R(omega / bL4);  // independent selects (see ryr_total)
+  om = (k == 3) ? R(omega / bL3) : om;
+  om = (k == 2) ? R(omega / bL2) : om;
+  om = (k == 1) ? R(omega / bL) : om;
+  om = (k == 0) ? R(omega) : om;
   mode = ca ? om : (R)(1 << (k & 7)) * gam;
   tot = (ca && k == 4) ? (back + mode) + fwd : (back + fwd) + mode;
   if (s == 6) tot = R(gL);
@@ -831,7 +834,11 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       const R vm35 = V - R(35.0);
       const R e0 = R(0.012) * vm35, e1 = R(-0.05) * vm35;
       const R e2 = rdiv(V + R(12.5), divk(5.0)), e3 = rdiv(V + R(30.0), divk(12.0));
-      const R ex = xexp(c == 0 ? e0 : c == 1 ? e1 : c == 2 ? e2 : e3);  // one exp per lane
+      R earg = e3;  // one exp per lane; independent selects (see ryr_total)
+      earg = (c == 2) ? e2 : earg;
+      earg = (c == 1) ? e1 : earg;
+      earg = (c == 0) ? e0 : earg;
+      const R ex = xexp(earg);
       const R alpha = R(alphacf) * qperm<QB0>(ex);
       const R beta = R(betacf) * qperm<QB1>(ex);
       const R yinf = rdiv(R(yCa_frac), R(1.0) + qperm<QB2>(ex)) + (R(1.0) - R(yCa_frac));
