@@ -965,13 +965,25 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       // ---- the transitions taken
       if (mv_lcc) w = ch_set_lcc(w, lcc_move<R>(ls, u0, dt, lb, lf, lm, lt));
       if (mv_y) w = ch_flip_y(w);
+      // the RyRs that move, lowest first, one exit walk per round: lanes moving
+      // different RyRs share a round instead of running one body per RyR index
+      // (the moves touch disjoint fields of w, so the order is immaterial)
+      unsigned mm = 0;
 #pragma unroll
-      for (int r = 0; r < kNryr; ++r) {
-        if (mv_r[r]) {
-          Exits<R> e;
-          ryr_exits(st[r], k, e);
-          w = ch_set_ryr(w, r, pick<R>(ur[r], dt, e));
+      for (int r = 0; r < kNryr; ++r) mm |= mv_r[r] ? (1u << r) : 0u;
+      while (mm) {
+        const int r = __builtin_ctz(mm);
+        mm &= mm - 1u;
+        int sr = st[0];
+        double ru = ur[0];
+#pragma unroll
+        for (int q = 1; q < kNryr; ++q) {
+          sr = (r == q) ? st[q] : sr;
+          ru = (r == q) ? ur[q] : ru;
         }
+        Exits<R> e;
+        ryr_exits(sr, k, e);
+        w = ch_set_ryr(w, r, pick<R>(ru, dt, e));
       }
       if (mv_ito) w = ch_flip_ito2(w);
       lopen = ch_lcc_open(w);
