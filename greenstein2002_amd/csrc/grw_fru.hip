@@ -194,14 +194,16 @@ struct Dev {
 // probability 1 - dt*k_tot, else walk the cumulative exit probabilities with '>'.
 template <class R>
 __device__ __forceinline__ int pick(double u, double dt, const Exits<R>& e) {
-  // unrolled walk (at most three exits) so the list stays in registers
-  double acc = 1.0 - dt * (double)e.tot;
-  if (!(u > acc)) return 0;
-  acc += dt * (double)e.r[0];
-  if (!(u > acc) || e.n == 1) return e.d[0];
-  acc += dt * (double)e.r[1];
-  if (!(u > acc) || e.n == 2) return e.d[1];
-  return e.d[2];
+  // the walk's three partial sums in the reference's order, then selects from
+  // the last exit back to "stay" (no branches: lanes disagree on the exit)
+  const double acc0 = 1.0 - dt * (double)e.tot;
+  const double acc1 = acc0 + dt * (double)e.r[0];
+  const double acc2 = acc1 + dt * (double)e.r[1];
+  int out = e.d[2];
+  out = (!(u > acc2) || e.n == 2) ? e.d[1] : out;
+  out = (!(u > acc1) || e.n == 1) ? e.d[0] : out;
+  out = !(u > acc0) ? 0 : out;
+  return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -546,6 +548,35 @@ __device__ __forceinline__ int lcc_move(int s, double u, double dt, R back, R fw
     e.r[0] = back; e.d[0] = s - 1; e.r[1] = mode; e.d[1] = s - 6; e.r[2] = fwd; e.d[2] = s + 1; e.n = 3;
   }
   return pick<R>(u, dt, e);
+}
+
+// ryr_exits (grw_model.h) by selects, for a RyR that moves in the four-lane
+// kernel: the exit lists of the eight states with ryr_exits' expressions, the
+// merged states' splitting factors a7, a8 taken from the step's rates (they
+// are the same quotients), chosen by s.
+template <class R>
+__device__ __forceinline__ void ryr_exits_sel(int s, const RyrK<R>& k, R a7, R a8, Exits<R>& e) {
+  const int g = k.regime;
+  const R a72 = (R(1.0) - a7) * R(k32), a73 = a7 * R(k45);  // state 7 (ryr_exits case 7)
+  const R a82 = a8 * R(k52), a83 = a8 * k.k54;               // state 8
+  R tot = (a82 + a83), r0 = a82, r1 = a83, r2 = R(0.0);
+  int d0 = 2, d1 = (g == 3) ? 7 : 4, d2 = 0, n = 2;
+  const bool s1 = s == 1, s2 = s == 2, s3 = s == 3, s4 = s == 4, s5 = s == 5, s6 = s == 6, s7 = s == 7;
+  tot = s7 ? (a72 + a73) : tot; r0 = s7 ? a72 : r0; r1 = s7 ? a73 : r1; d1 = s7 ? 8 : d1;
+  tot = s6 ? R(k65) : tot; r0 = s6 ? R(k65) : r0; d0 = s6 ? 5 : d0; n = s6 ? 1 : n;
+  tot = s5 ? (R(k52) + k.k54 + k.k56) : tot; r0 = s5 ? R(k52) : r0; r1 = s5 ? k.k54 : r1;
+  r2 = s5 ? k.k56 : r2; d1 = s5 ? 4 : d1; d2 = s5 ? 6 : d2; n = s5 ? 3 : n;
+  tot = s4 ? (R(k43) + R(k45)) : tot; r0 = s4 ? R(k43) : r0; r1 = s4 ? R(k45) : r1;
+  d0 = s4 ? 3 : d0; d1 = s4 ? ((g == 1) ? 5 : 8) : d1;
+  tot = s3 ? (R(k32) + k.k34) : tot; r0 = s3 ? R(k32) : r0; r1 = s3 ? k.k34 : r1; d1 = s3 ? 4 : d1;
+  tot = s2 ? (R(k21) + k.k23 + k.k25) : tot; r0 = s2 ? R(k21) : r0; r1 = s2 ? k.k23 : r1;
+  r2 = s2 ? k.k25 : r2; d0 = s2 ? 1 : d0; d1 = s2 ? ((g == 3) ? 7 : 3) : d1;
+  d2 = s2 ? ((g == 1) ? 5 : 8) : d2; n = s2 ? 3 : n;
+  tot = s1 ? k.k12 : tot; r0 = s1 ? k.k12 : r0; n = s1 ? 1 : n;
+  e.tot = tot;
+  e.r[0] = r0; e.r[1] = r1; e.r[2] = r2;
+  e.d[0] = (int8_t)d0; e.d[1] = (int8_t)d1; e.d[2] = (int8_t)d2;
+  e.n = (int8_t)n;
 }
 
 // Total exit rate of a RyR in (resolved) state s: fru_rates_local.c:390-727.
@@ -982,7 +1013,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
           ru = (r == q) ? ur[q] : ru;
         }
         Exits<R> e;
-        ryr_exits(sr, k, e);
+        ryr_exits_sel(sr, k, a7, a8, e);
         w = ch_set_ryr(w, r, pick<R>(ru, dt, e));
       }
       if (mv_ito) w = ch_flip_ito2(w);

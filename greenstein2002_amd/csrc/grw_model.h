@@ -156,11 +156,11 @@ GRW_HD int ch_ryr_open(uint32_t w) {
 
 // ---- exit list of one channel (at most 3 exits)
 template <class R>
-struct Exits {
-  R tot;
-  R r[3];
-  int8_t d[3];
-  int8_t n;
+struct Exits {  // fully defined whatever the number of exits
+  R tot = R(0);
+  R r[3] = {R(0), R(0), R(0)};
+  int8_t d[3] = {0, 0, 0};
+  int8_t n = 0;
 };
 
 // LCC mode-switching chain, fru_rates_local.c:164-341.  States 1-5 C0..C4,
