@@ -21,4 +21,8 @@ def __getattr__(name):  # the whole-cell and batch layers load on first use
         from . import batch
 
         return getattr(batch, name)
+    if name == "run_caf":
+        from . import output
+
+        return output.run_caf
     raise AttributeError(name)

@@ -55,6 +55,32 @@ def read_ic_dir(path: str, nfru: int, names=None) -> dict:
     return out
 
 
+def read_seeds(path: str, nfru: int, word_bytes: int = 8) -> dict:
+    """Binary RNG-state records as initialize_state.c:135-160 reads them (and
+    data_io.c:268-292 writes them): per FRU i the words i, mti, mt[0..623] in
+    ``word_bytes``-byte little-endian words (8: an LP64 ``unsigned long``; 4:
+    the shipped ic/stable/ic_randomseeds_stable.txt).  Records are read in
+    order without rewinding.  Where the reference prints an error and goes on
+    (a short record) or aborts (a record whose first word is not its FRU
+    index), this raises ValueError.  Returns mti [nfru] and mt [nfru, 624]."""
+    if word_bytes not in (4, 8):
+        raise ValueError("word_bytes must be 4 or 8")
+    rec = 626
+    raw = np.fromfile(path, dtype="<u8" if word_bytes == 8 else "<u4")
+    if raw.size < nfru * rec:
+        raise ValueError("Unable to read seeds from file '%s': %d full records, %d wanted"
+                         % (path, raw.size // rec, nfru))
+    r = raw[: nfru * rec].reshape(nfru, rec).astype(np.uint64)
+    bad = np.nonzero(r[:, 0] != np.arange(nfru, dtype=np.uint64))[0]
+    if bad.size:
+        i = int(bad[0])
+        raise ValueError("RNG seed file fails consistency check: %d!=%d" % (int(r[i, 0]), i))
+    mti = r[:, 1].astype(np.int64)
+    if np.any(mti > 624):
+        raise ValueError("RNG seed file holds mti > 624")
+    return dict(mti=mti.astype(np.int32), mt=(r[:, 2:] & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+
+
 def load_stable(nfru: int, scale: float | None = None) -> dict:
     """ic/stable replicated to ``nfru`` FRUs, with state[CaTOT] recomputed for
     that FRU count and scale as initialize_state.c:233-246 does."""

@@ -5,10 +5,11 @@
  * Built by oracle/Makefile together with the UNMODIFIED reference sources
  * (simfru_local.c, fru_rates_local.c, fcn_fru.c, mt19937_local.c,
  * initialize_ran.c, calc_fru_flux_local.c, calc_fru_avg_local.c,
- * parameters_fcn_fru.c, fcn.c, initialize_state.c, data_io.c) compiled in
- * place from the reference tree with the reference's flags (gcc -O3).  The
- * result, oracle/_ref/libgrw_ref.so, is test infrastructure only: it pins the
- * CPU restatement in grw_oracle.c and is never part of the product.
+ * parameters_fcn_fru.c, fcn.c, initialize_state.c, data_io.c, write_fru_props.c)
+ * compiled in place from the reference tree with the reference's flags
+ * (gcc -O3).  The result, oracle/_ref/libgrw_ref.so, is test infrastructure
+ * only: it pins the CPU restatement in grw_oracle.c and the package's file
+ * writers, and is never part of the product.
  *
  * This file defines the globals those translation units expect from caf.c
  * (caf.c:27-38) and the master-side flux routine that fcn.c's lastcall needs
