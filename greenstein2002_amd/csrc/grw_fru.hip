@@ -852,6 +852,14 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     double t = a.t0;
     int bad = 0;
     int nguard = 0;
+    // Each lane's totals of the eight RyR states, one LDS column per lane, so
+    // that a RyR's total is one indexed read instead of a chain of selects
+    // over its state.  A lane reads only what it wrote itself, in program
+    // order, so no barrier is needed.  States 4 and 6 have constant totals.
+    __shared__ R rtot[8][256];
+    const int tl = threadIdx.x;
+    rtot[3][tl] = R(k43 + k45);
+    rtot[5][tl] = R(k65);
     MtWin8 win;  // words of the next step's draws, fetched one step ahead
     // Complete the FRU-state loads here: otherwise the loop header merges them
     // with the prefetch below and the first use of the state in every step
@@ -904,6 +912,12 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       const R T7 = ((R(1.0) - a7) * R(k32)) + (a7 * R(k45));
       const R a8 = rdiv(R(k65), k.k56 + R(k65));
       const R T8 = (a8 * R(k52)) + (a8 * k.k54);
+      rtot[0][tl] = k.k12;
+      rtot[1][tl] = T2;
+      rtot[2][tl] = T3;
+      rtot[4][tl] = T5;
+      rtot[6][tl] = T7;
+      rtot[7][tl] = T8;
       if (nd) {
         const int Ec = (c > 0 ? n0 : 0) + (c > 1 ? n1 : 0) + (c > 2 ? n2 : 0);
         uint32_t vw[kNryr];
@@ -921,7 +935,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
 #pragma unroll
       for (int r = 0; r < kNryr; ++r) {
         w = ch_set_ryr(w, r, st[r]);
-        tr[r] = ryr_total<R>(st[r], k.k12, T2, T3, T5, T7, T8);
+        tr[r] = rtot[st[r] - 1][tl];  // ryr_total of the resolved state
         maxr = rmax(maxr, tr[r]);
       }
       maxr = rmax(maxr, qperm<QX1>(maxr));
