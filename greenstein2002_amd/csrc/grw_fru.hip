@@ -511,21 +511,19 @@ __device__ __forceinline__ double qperm(double v) {
 // by the exact small integers gives the reference's products bit for bit, and
 // the total keeps its summation order ((back + fwd) + mode, but
 // (back + mode) + fwd for CCa4).
+// The mode-Ca switch rates omega / bL^k come from the caller (a table lookup);
+// the mode-normal ones 2^k gamma are exact scalings (ldexp).  For k = 0 the
+// product k * beta is +0, the reference's absent back exit.
 template <class R>
-__device__ __forceinline__ void lcc_arith(int s, R alpha, R beta, R bp, R gam, R& back, R& fwd,
+__device__ __forceinline__ void lcc_arith(int s, R alpha, R beta, R bp, R gam, R om, R& back, R& fwd,
                                           R& mode, R& tot) {
   const bool ca = (s >= 7) & (s <= 11);
   const int k = ca ? s - 7 : s - 1;
-  back = (k == 0) ? R(0.0) : (R)k * (ca ? bp : beta);
+  back = (R)k * (ca ? bp : beta);
   const R fwd_n = (k < 4) ? (R)(4 - k) * alpha : R(fL);
   const R fwd_c = (k < 4) ? (R)(2 * (4 - k)) * alpha : R(fLprime);
   fwd = ca ? fwd_c : fwd_n;
-  R om = R(omega / bL4);  // independent selects (see ryr_total)
-  om = (k == 3) ? R(omega / bL3) : om;
-  om = (k == 2) ? R(omega / bL2) : om;
-  om = (k == 1) ? R(omega / bL) : om;
-  om = (k == 0) ? R(omega) : om;
-  mode = ca ? om : (R)(1 << (k & 7)) * gam;
+  mode = ca ? om : ldexp(gam, k & 7);
   tot = (ca && k == 4) ? (back + mode) + fwd : (back + fwd) + mode;
   if (s == 6) tot = R(gL);
   if (s == 12) tot = R(gLprime);
@@ -554,47 +552,49 @@ __device__ __forceinline__ int lcc_move(int s, double u, double dt, R back, R fw
 // kernel: the exit lists of the eight states with ryr_exits' expressions, the
 // merged states' splitting factors a7, a8 taken from the step's rates (they
 // are the same quotients), chosen by s.
+// tot is the state's total, the same sum ryr_exits forms (looked up by the caller).
 template <class R>
-__device__ __forceinline__ void ryr_exits_sel(int s, const RyrK<R>& k, R a7, R a8, Exits<R>& e) {
+__device__ __forceinline__ void ryr_exits_sel(int s, const RyrK<R>& k, R a7, R a8, R tot, Exits<R>& e) {
   const int g = k.regime;
   const R a72 = (R(1.0) - a7) * R(k32), a73 = a7 * R(k45);  // state 7 (ryr_exits case 7)
   const R a82 = a8 * R(k52), a83 = a8 * k.k54;               // state 8
-  R tot = (a82 + a83), r0 = a82, r1 = a83, r2 = R(0.0);
+  R r0 = a82, r1 = a83, r2 = R(0.0);
   int d0 = 2, d1 = (g == 3) ? 7 : 4, d2 = 0, n = 2;
   const bool s1 = s == 1, s2 = s == 2, s3 = s == 3, s4 = s == 4, s5 = s == 5, s6 = s == 6, s7 = s == 7;
-  tot = s7 ? (a72 + a73) : tot; r0 = s7 ? a72 : r0; r1 = s7 ? a73 : r1; d1 = s7 ? 8 : d1;
-  tot = s6 ? R(k65) : tot; r0 = s6 ? R(k65) : r0; d0 = s6 ? 5 : d0; n = s6 ? 1 : n;
-  tot = s5 ? (R(k52) + k.k54 + k.k56) : tot; r0 = s5 ? R(k52) : r0; r1 = s5 ? k.k54 : r1;
+  r0 = s7 ? a72 : r0; r1 = s7 ? a73 : r1; d1 = s7 ? 8 : d1;
+  r0 = s6 ? R(k65) : r0; d0 = s6 ? 5 : d0; n = s6 ? 1 : n;
+  r0 = s5 ? R(k52) : r0; r1 = s5 ? k.k54 : r1;
   r2 = s5 ? k.k56 : r2; d1 = s5 ? 4 : d1; d2 = s5 ? 6 : d2; n = s5 ? 3 : n;
-  tot = s4 ? (R(k43) + R(k45)) : tot; r0 = s4 ? R(k43) : r0; r1 = s4 ? R(k45) : r1;
+  r0 = s4 ? R(k43) : r0; r1 = s4 ? R(k45) : r1;
   d0 = s4 ? 3 : d0; d1 = s4 ? ((g == 1) ? 5 : 8) : d1;
-  tot = s3 ? (R(k32) + k.k34) : tot; r0 = s3 ? R(k32) : r0; r1 = s3 ? k.k34 : r1; d1 = s3 ? 4 : d1;
-  tot = s2 ? (R(k21) + k.k23 + k.k25) : tot; r0 = s2 ? R(k21) : r0; r1 = s2 ? k.k23 : r1;
+  r0 = s3 ? R(k32) : r0; r1 = s3 ? k.k34 : r1; d1 = s3 ? 4 : d1;
+  r0 = s2 ? R(k21) : r0; r1 = s2 ? k.k23 : r1;
   r2 = s2 ? k.k25 : r2; d0 = s2 ? 1 : d0; d1 = s2 ? ((g == 3) ? 7 : 3) : d1;
   d2 = s2 ? ((g == 1) ? 5 : 8) : d2; n = s2 ? 3 : n;
-  tot = s1 ? k.k12 : tot; r0 = s1 ? k.k12 : r0; n = s1 ? 1 : n;
+  r0 = s1 ? k.k12 : r0; n = s1 ? 1 : n;
   e.tot = tot;
   e.r[0] = r0; e.r[1] = r1; e.r[2] = r2;
   e.d[0] = (int8_t)d0; e.d[1] = (int8_t)d1; e.d[2] = (int8_t)d2;
   e.n = (int8_t)n;
 }
 
-// Total exit rate of a RyR in (resolved) state s: fru_rates_local.c:390-727.
-// Independent selects: a chained conditional on s becomes a branch tree, and
-// since the lanes of a wavefront hold different states every arm of it runs,
-// each behind its own exec-mask handling.
-template <class R>
-__device__ __forceinline__ R ryr_total(int s, R T1, R T2, R T3, R T5, R T7, R T8) {
-  R t = T8;
-  t = (s == 1) ? T1 : t;
-  t = (s == 2) ? T2 : t;
-  t = (s == 3) ? T3 : t;
-  t = (s == 4) ? R(k43 + k45) : t;
-  t = (s == 5) ? T5 : t;
-  t = (s == 6) ? R(k65) : t;
-  t = (s == 7) ? T7 : t;
-  return t;
+// ryr_collapse and ryr_needs_draw of the raw states 1..8 in one regime, four
+// bits per state (bits 0-2: the resolved state - 1; bit 3: the resolved state
+// needs a uniform), so that a lane resolves each RyR with bit-field extracts
+// instead of compares and selects on its state.
+constexpr uint32_t ryr_resolve_bits(int regime) {
+  uint32_t v = 0;
+  for (int s = 1; s <= 8; ++s) {
+    const int t = ryr_collapse(s, regime);
+    v |= (uint32_t)((t - 1) | (ryr_needs_draw(t, regime) ? 8 : 0)) << (4 * (s - 1));
+  }
+  return v;
 }
+static_assert(((ryr_resolve_bits(3) >> 8) & 15u) == 6u, "regime 3: state 3 collapses to 7");
+static_assert(((ryr_resolve_bits(1) >> 24) & 15u) == 14u, "regime 1: state 7 stays and needs a uniform");
+static_assert(((ryr_resolve_bits(2) >> 16) & 15u) == 7u, "regime 2: state 5 collapses to 8, no uniform");
+static_assert(((ryr_resolve_bits(2) >> 24) & 15u) == 14u, "regime 2: state 7 needs a uniform");
+static_assert(((ryr_resolve_bits(3) >> 4) & 15u) == 1u, "regime 3: state 2 stays");
 
 // Row position of absolute stream position g in [0, 1872): positions below 624
 // are words of a loaded full generation, read as is; the others are lazy.
@@ -860,6 +860,14 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     const int tl = threadIdx.x;
     rtot[3][tl] = R(k43 + k45);
     rtot[5][tl] = R(k65);
+    // omega / bL^k of the mode-Ca LCC states CCa0..CCa4, likewise one column
+    // per lane (fru_rates_local.c:70-76 constant expressions)
+    __shared__ R lom[5][256];
+    lom[0][tl] = R(omega);
+    lom[1][tl] = R(omega / bL);
+    lom[2][tl] = R(omega / bL2);
+    lom[3][tl] = R(omega / bL3);
+    lom[4][tl] = R(omega / bL4);
     MtWin8 win;  // words of the next step's draws, fetched one step ahead
     // Complete the FRU-state loads here: otherwise the loop header merges them
     // with the prefetch below and the first use of the state in every step
@@ -873,7 +881,11 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       const R vm35 = V - R(35.0);
       const R e0 = R(0.012) * vm35, e1 = R(-0.05) * vm35;
       const R e2 = rdiv(V + R(12.5), divk(5.0)), e3 = rdiv(V + R(30.0), divk(12.0));
-      R earg = e3;  // one exp per lane; independent selects (see ryr_total)
+      // One exp per lane.  Independent selects: a chained conditional on a
+      // lane-varying value becomes a branch tree, and since the lanes of a
+      // wavefront disagree every arm of it runs, each behind its own exec-mask
+      // handling.
+      R earg = e3;
       earg = (c == 2) ? e2 : earg;
       earg = (c == 1) ? e1 : earg;
       earg = (c == 0) ? e0 : earg;
@@ -887,17 +899,24 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       const int ls = ch_lcc(w);
       bad |= (ls < 1 || ls > 12);
       R lb, lf, lm, lt;
-      lcc_arith<R>(ls, alpha, beta, bp, gcf * cc, lb, lf, lm, lt);
+      const int lk = ls - 7;  // CCa index; other states read entry 0 and ignore it
+      lcc_arith<R>(ls, alpha, beta, bp, gcf * cc, lom[(lk >= 0 && lk <= 4) ? lk : 0][tl], lb, lf, lm, lt);
       const R vr = rdiv(ch_y(w) == 2 ? (R(1.0) - yinf) : yinf, tau);
       const R ir = (ch_ito2(w) == 1) ? R(kfIto2) * cc : R(kbIto2);
       R maxr = rmax(rmax(rmax(R(0.0), lt), vr), ir);
       const RyrK<R> k = ryr_k(cc, k23c);
       int st[kNryr];
       int nd = 0;
+      unsigned need = 0;
+      uint32_t rbits = ryr_resolve_bits(3);  // independent selects, as for earg
+      rbits = (k.regime == 2) ? ryr_resolve_bits(2) : rbits;
+      rbits = (k.regime == 1) ? ryr_resolve_bits(1) : rbits;
 #pragma unroll
       for (int r = 0; r < kNryr; ++r) {
-        st[r] = ryr_collapse(ch_ryr(w, r), k.regime);
-        nd += ryr_needs_draw(st[r], k.regime) ? 1 : 0;
+        const uint32_t b = (rbits >> (4u * ((w >> (5 + 3 * r)) & 7u))) & 15u;  // raw state ch_ryr(w, r)
+        st[r] = (int)(b & 7u) + 1;
+        nd += (int)(b >> 3);
+        need |= (b >> 3) << r;
       }
       // merged-state uniforms come first, cleft by cleft (fru_rates_local.c:452,476,581)
       const int n0 = qperm<QB0>(nd), n1 = qperm<QB1>(nd), n2 = qperm<QB2>(nd), n3 = qperm<QB3>(nd);
@@ -924,7 +943,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
         mt_run<kNryr>(row, idx + Ec, nd, vw);
 #pragma unroll
         for (int r = 0; r < kNryr; ++r) {
-          if (ryr_needs_draw(st[r], k.regime)) {
+          if ((need >> r) & 1u) {
             st[r] = ryr_split(st[r], unif(vw[0]), k);
 #pragma unroll
             for (int j = 0; j < kNryr - 1; ++j) vw[j] = vw[j + 1];
@@ -935,7 +954,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
 #pragma unroll
       for (int r = 0; r < kNryr; ++r) {
         w = ch_set_ryr(w, r, st[r]);
-        tr[r] = rtot[st[r] - 1][tl];  // ryr_total of the resolved state
+        tr[r] = rtot[st[r] - 1][tl];  // total of the resolved state (fru_rates_local.c:390-727)
         maxr = rmax(maxr, tr[r]);
       }
       maxr = rmax(maxr, qperm<QX1>(maxr));
@@ -1027,7 +1046,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
           ru = (r == q) ? ur[q] : ru;
         }
         Exits<R> e;
-        ryr_exits_sel(sr, k, a7, a8, e);
+        ryr_exits_sel(sr, k, a7, a8, rtot[sr - 1][tl], e);
         w = ch_set_ryr(w, r, pick<R>(ru, dt, e));
       }
       if (mv_ito) w = ch_flip_ito2(w);

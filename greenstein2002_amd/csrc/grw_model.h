@@ -240,11 +240,11 @@ GRW_HD void ryr_exits(int s, const RyrK<R>& k, Exits<R>& e) {
 
 // Does state s need one uniform to be split before its rates exist?
 // fru_rates_local.c:450-496 (regime 1, states 7 and 8) and :579-601 (regime 2, state 7).
-GRW_HD bool ryr_needs_draw(int s, int regime) {
+GRW_HD constexpr bool ryr_needs_draw(int s, int regime) {
   return (regime == 1 && (s == 7 || s == 8)) || (regime == 2 && s == 7);
 }
 // Deterministic collapse into the merged states, :551-577 and :647-701.
-GRW_HD int ryr_collapse(int s, int regime) {
+GRW_HD constexpr int ryr_collapse(int s, int regime) {
   if (regime == 3 && (s == 3 || s == 4)) return 7;
   if (regime != 1 && (s == 5 || s == 6)) return 8;
   return s;
