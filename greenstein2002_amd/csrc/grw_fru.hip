@@ -800,6 +800,15 @@ __device__ __forceinline__ void fcn_fru4(R CaJSR, R s, const Ghk<R>& g, R Cai, R
   ds = bSS * (JDHPR + JRyR - Jxfer - Jss + Jss_prev);
 }
 
+// A value moved into vector registers: the Monte Carlo loop is short of scalar
+// registers (its 64-bit constants are re-materialised or spilled every step),
+// so its uniform loop invariants live in VGPRs instead.
+template <class T>
+__device__ __forceinline__ T in_vgpr(T x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // One sub-interval of every FRU with four lanes per FRU; returns this lane's
 // FRU-step count (nonzero only in lane 0 of a quad) and its flux terms in acc.
 template <class R>
@@ -840,16 +849,18 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
         }
       }
     }
-    const double span0 = a.depf[0] - a.dep0[0];
-    const double span1 = a.depf[1] - a.dep0[1];
-    const double span2 = a.depf[2] - a.dep0[2];
+    const double t0 = in_vgpr(a.t0), t1 = in_vgpr(a.t1);
+    const double o0 = in_vgpr(a.dep0[0]), o1 = in_vgpr(a.dep0[1]), o2 = in_vgpr(a.dep0[2]);
+    const double span0 = in_vgpr(a.depf[0] - a.dep0[0]);
+    const double span1 = in_vgpr(a.depf[1] - a.dep0[1]);
+    const double span2 = in_vgpr(a.depf[2] - a.dep0[2]);
     const double tint = a.t1 - a.t0;
-    const DivK kint{tint, 1.0 / tint};  // the sub-interval length, fixed for the launch
-    double dV = a.dep0[0], dCai = a.dep0[1], dNSR = a.dep0[2];
+    const DivK kint{in_vgpr(tint), in_vgpr(1.0 / tint)};  // the sub-interval length, fixed for the launch
+    double dV = o0, dCai = o1, dNSR = o2;
     bool lopen = ch_lcc_open(w);
     int nopen = ch_ryr_open(w);
-    const R k23c = (R)a.k23c, gcf = (R)a.gcf;
-    double t = a.t0;
+    const R k23c = in_vgpr((R)a.k23c), gcf = in_vgpr((R)a.gcf);
+    double t = t0;
     int bad = 0;
     int nguard = 0;
     // Each lane's totals of the eight RyR states, one LDS column per lane, so
@@ -873,9 +884,9 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     // with the prefetch below and the first use of the state in every step
     // waits for all outstanding loads, prefetch included.
     __builtin_amdgcn_s_waitcnt(0);
-    if (t < a.t1) mt_fetch8(row, idx + 8 * c, win);
+    if (t < t1) mt_fetch8(row, idx + 8 * c, win);
     Ghk<R> gv = ghk_terms<R>((R)dV);
-    while (t < a.t1) {
+    while (t < t1) {
       // ---- rates at the step start (fru_rates_local.c:126-777)
       const R V = (R)dV;
       const R vm35 = V - R(35.0);
@@ -968,7 +979,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       cm = rmax(cm, qperm<QX2>(cm));
       const double cmax = rmax(0.e-3, cm);
       if (cmax > 1.5e-3) dt = rmin(dt, 5.001e-3);
-      dt = rmin(dt, a.t1 - t);
+      dt = rmin(dt, t1 - t);
       // an unknown channel state, non-finite rates or a runaway (quad-uniform)
       if (bad || !(t + dt > t) || ++nguard > kStepCap) {
         bad = 1;
@@ -1018,10 +1029,10 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       const double k1j = dt * (double)djs, k1s = dt * (double)dss;
       const double y1j = cj + k1j, y1s = css + k1s;
       t = t + dt;
-      const double frac = rdiv(t - a.t0, kint);  // (t - t0) / tint, correctly rounded
-      dV = a.dep0[0] + frac * span0;
-      dCai = a.dep0[1] + frac * span1;
-      dNSR = a.dep0[2] + frac * span2;
+      const double frac = rdiv(t - t0, kint);  // (t - t0) / tint, correctly rounded
+      dV = o0 + frac * span0;
+      dCai = o1 + frac * span1;
+      dNSR = o2 + frac * span2;
       gv = ghk_terms<R>((R)dV);  // also the next step's first evaluation
       fcn_fru4<R>((R)y1j, (R)y1s, gv, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
       cj = cj + (k1j + dt * (double)djs) / 2.0;
