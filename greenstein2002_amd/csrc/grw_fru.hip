@@ -596,6 +596,13 @@ static_assert(((ryr_resolve_bits(2) >> 16) & 15u) == 7u, "regime 2: state 5 coll
 static_assert(((ryr_resolve_bits(2) >> 24) & 15u) == 14u, "regime 2: state 7 needs a uniform");
 static_assert(((ryr_resolve_bits(3) >> 4) & 15u) == 1u, "regime 3: state 2 stays");
 
+// The four-lane kernel keeps a second copy of row words 0..7 in the row's
+// padding, positions 624..631 (refreshed by the quad when a launch starts,
+// and every store to either copy goes to both), so that a window of words
+// running past the end of the generation reads on without wrapping.
+constexpr int kMtMirror = 8;
+static_assert(kMtN + kMtMirror <= kMtStride, "the copy lives in the row padding");
+
 // Row position of absolute stream position g in [0, 1872): positions below 624
 // are words of a loaded full generation, read as is; the others are lazy.
 __device__ __forceinline__ int mt_row_pos(int g) {
@@ -635,7 +642,9 @@ __device__ __forceinline__ void mt_run(uint32_t* row, int g0, int n, uint32_t ou
     if (k < n) {
       if (g >= kMtN) {
         const uint32_t wv = mt_twist(cur[k], nxt[k], far[k]);
-        row[mt_row_pos(g)] = wv;
+        const int pos = mt_row_pos(g);
+        row[pos] = wv;
+        if (pos < kMtMirror) row[pos + kMtN] = wv;  // the row-end copy (mt_finish8)
         out[k] = MtLane::temper(wv);
       } else {
         out[k] = MtLane::temper(cur[k]);
@@ -644,66 +653,50 @@ __device__ __forceinline__ void mt_run(uint32_t* row, int g0, int n, uint32_t ou
   }
 }
 
-// The eight words of one cleft's draws, fetched as 16-byte chunks.  Row
-// positions q..q+8 (the draws and the word after them) lie in the three chunks
-// from q & ~3, and the words they are xored with, q+397..q+404 (mod 624, both
-// for q < 227 and q >= 227), in the three chunks from (q+397 mod 624) & ~3;
-// 624 = 4 * 156, so chunks never straddle the end of a generation and chunk
-// indices simply wrap mod 156.
+// The eight words of one cleft's draws: row positions q..q+8 (the draws and the
+// word after them, at most 631) and the words they are xored with,
+// q+397..q+404 mod 624 (at most 630), both for q < 227 and q >= 227; dword
+// loads at fixed offsets from two bases.
 struct MtWin8 {
-  uint4 a0, a1, a2, b0, b1, b2;
+  uint32_t cur[9], far[8];
 };
-
-__device__ __forceinline__ int mt_chunk_next(int j) { return j == kMtN / 4 - 1 ? 0 : j + 1; }
 
 __device__ __forceinline__ void mt_fetch8(const uint32_t* row, int g0, MtWin8& w) {
   const int q = mt_row_pos(g0);
   int qf = q + 397;
   qf -= (qf >= kMtN) ? kMtN : 0;
-  const uint4* r4 = reinterpret_cast<const uint4*>(row);
-  const int ja = q >> 2, jb = qf >> 2;
-  const int ja1 = mt_chunk_next(ja), jb1 = mt_chunk_next(jb);
-  w.a0 = r4[ja];
-  w.a1 = r4[ja1];
-  w.a2 = r4[mt_chunk_next(ja1)];
-  w.b0 = r4[jb];
-  w.b1 = r4[jb1];
-  w.b2 = r4[mt_chunk_next(jb1)];
-}
-
-// Words r .. r+M-1 of the twelve words of three chunks (r in 0..3), by masks
-// rather than selects, which the compiler would turn into an indexed load from
-// a scratch copy of the array.
-template <int M>
-__device__ __forceinline__ void mt_select(const uint4& x0, const uint4& x1, const uint4& x2, int r,
-                                          uint32_t out[M]) {
-  const uint32_t S[12] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w, x2.x, x2.y, x2.z, x2.w};
-  const uint32_t m1 = 0u - (uint32_t)(r & 1), m2 = 0u - (uint32_t)((r >> 1) & 1);
-  uint32_t t[M + 2];
+  const uint32_t* const pc = row + q;
+  const uint32_t* const pf = row + qf;
 #pragma unroll
-  for (int j = 0; j < M + 2; ++j) t[j] = (S[j] & ~m1) | (S[j + 1] & m1);
+  for (int k = 0; k < 9; ++k) w.cur[k] = pc[k];
 #pragma unroll
-  for (int k = 0; k < M; ++k) out[k] = (t[k] & ~m2) | (t[k + 2] & m2);
+  for (int k = 0; k < 8; ++k) w.far[k] = pf[k];
 }
 
 // Regeneration and stores for stream positions g0 .. g0+7 from a fetched window.
 // Branch-free: every word is twisted and a mask keeps the loaded word for a
 // position of the loaded generation (g < 624), which is stored back unchanged
-// (cur[k] is the word at row position g); row positions q + k wrap to the row
-// start past 623, the store address selecting between two bases.
+// (cur[k] is the word at row position g).  Row positions q + k past 623 are the
+// copies of words 0..7, so the stores go to q + k and, for a window at either
+// end of the row (rare), also to the other copy.
 __device__ __forceinline__ void mt_finish8(uint32_t* row, int g0, const MtWin8& w, uint32_t out[8]) {
   const int q = mt_row_pos(g0);
-  uint32_t cur[9], far[8];
-  mt_select<9>(w.a0, w.a1, w.a2, q & 3, cur);
-  mt_select<8>(w.b0, w.b1, w.b2, (q + 1) & 3, far);  // 397 = 1 mod 4
-  uint32_t* const lo = row + q;
-  uint32_t* const hi = row + (q - kMtN);
+  uint32_t* const p = row + q;
+  uint32_t wv[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const uint32_t m = 0u - (uint32_t)(g0 + k >= kMtN);
-    const uint32_t wv = (mt_twist(cur[k], cur[k + 1], far[k]) & m) | (cur[k] & ~m);
-    (q + k < kMtN ? lo : hi)[k] = wv;
-    out[k] = MtLane::temper(wv);
+    wv[k] = (mt_twist(w.cur[k], w.cur[k + 1], w.far[k]) & m) | (w.cur[k] & ~m);
+    p[k] = wv[k];
+    out[k] = MtLane::temper(wv[k]);
+  }
+  if (q < kMtMirror || q > kMtN - kMtMirror) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int pos = q + k;
+      if (pos < kMtMirror) row[pos + kMtN] = wv[k];
+      if (pos >= kMtN) row[pos - kMtN] = wv[k];
+    }
   }
 }
 
@@ -849,6 +842,10 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
         }
       }
     }
+    // the row-end copy of words 0..7 (see kMtMirror): lanes 0 and 1 of the quad
+    // copy four words each; the fence makes them visible to the quad
+    if (c < 2) reinterpret_cast<uint4*>(row + kMtN)[c] = reinterpret_cast<const uint4*>(row)[c];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     const double t0 = in_vgpr(a.t0), t1 = in_vgpr(a.t1);
     const double o0 = in_vgpr(a.dep0[0]), o1 = in_vgpr(a.dep0[1]), o2 = in_vgpr(a.dep0[2]);
     const double span0 = in_vgpr(a.depf[0] - a.dep0[0]);
@@ -1004,8 +1001,7 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
       // used are positions idx .. idx+31 and those they regenerate from:
       // positions idx+1 .. idx+32 (not yet regenerated) and words at least 195
       // positions older than idx, none of which this step's stores (positions
-      // idx-32 .. idx-1) or any store before their use touches.  The 16-byte
-      // chunks also bring up to three unused words on either side.
+      // idx-32 .. idx-1, in either copy) or any store before their use touches.
       mt_fetch8(row, idx + 8 * c, win);
       // ---- transition decisions of this cleft (simfru_local.c:215-276); they
       // use only the step-start state, so they are taken ahead of the Heun step
