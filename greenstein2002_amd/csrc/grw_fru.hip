@@ -1365,10 +1365,12 @@ __device__ __forceinline__ void attempt_start(RkCtl* ctl, const CellInfo& ci, co
       }
       ctl->avg_pending = 0;
     }
-    if (eval) flux_currents(fl, st[0], st[6], st[7], ci.nfru, ci.scale, Jsh);
+  } else if (j == 64 && eval) {
+    // beside the averages on thread 0; fcn_block reads J only after its first barrier
+    flux_currents(fl, st[0], st[6], st[7], ci.nfru, ci.scale, Jsh);
   }
-  __syncthreads();
   if (eval) fcn_block(tstart, st, F1, cursh, keepc, Jsh, ctl->prot, fq);
+  else __syncthreads();
   if (j == 0 && active) {
     if (resume) {
       ctl->resume = 0;
@@ -1469,6 +1471,17 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   a.avg = (s0 == 5);  // the attempt's end state: averages for an output sample
   double acc[17] = {};
   count_steps(d, simfru4_body<R>(a, d, acc));
+  // the control block's state vectors for the tail, loaded by every block
+  // before the reduction so that the last block does not wait for them after it
+  const int j = threadIdx.x;
+  double st_j = 0.0, y_j = 0.0, k_j[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (j < kNglobal && s0 >= 1 && s0 <= 5) {
+    st_j = ctl->state[j];
+    y_j = ctl->y[j];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (q < s0) k_j[q] = ctl->k[q][j];
+  }
   __shared__ double tot[17];
   if (!reduce_range<17>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)) return;
   const unsigned long long tail0 = wall_clock64();
@@ -1478,8 +1491,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   __shared__ double Jsh[4];
   __shared__ FcnTrans fq;
   __shared__ int decision;
-  const int j = threadIdx.x;
-  const int s = ctl->stage;
+  const int s = s0;
   if (s < 1 || s > 5) {  // launches out of sequence: stop the run
     if (j == 0) {
       ctl->err = 1;
@@ -1489,13 +1501,16 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   }
   const double h = ctl->h, start = ctl->start;
   if (j < kNglobal) {
-    st[j] = ctl->state[j];
-    yv[j] = ctl->y[j];
-    for (int q = 0; q < s; ++q) kk[q][j] = ctl->k[q][j];
+    st[j] = st_j;
+    yv[j] = y_j;
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (q < s) kk[q][j] = k_j[q];
   }
   __syncthreads();
-  if (j == 0) flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], ci.nfru, ci.scale, Jsh);  // mpi_master.c:401-468
-  __syncthreads();
+  // the flux conversion (mpi_master.c:401-468) on wavefront 1, beside fcn_prep
+  // on thread 0: fcn_block reads J only after its first barrier
+  if (j == 64) flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], ci.nfru, ci.scale, Jsh);
   fcn_block(a.t1, yv, F, nullptr, false, Jsh, ctl->prot, &fq);
   if (j == 0)
     for (int q = 0; q < 4; ++q) ctl->J[q] = Jsh[q];

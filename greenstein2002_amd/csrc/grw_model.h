@@ -141,17 +141,15 @@ GRW_HD uint32_t ch_pack(int lcc, int y, const int* ryr, int ito2) {
   return w;
 }
 // open indicators: simfru_local.c:101-117
+// by bit tables: bit s of 0x1040 marks the open LCC states 6 and 12, bit
+// s - 1 of 0x4C the open RyR states 3, 4 and 7
 GRW_HD bool ch_lcc_open(uint32_t w) {
-  const int s = ch_lcc(w);
-  return (s == 6 || s == 12) && ch_y(w) == 2;
+  return ((0x1040u >> (w & 15u)) & (w >> 4) & 1u) != 0u;
 }
 GRW_HD int ch_ryr_open(uint32_t w) {
-  int n = 0;
-  for (int i = 0; i < kNryr; ++i) {
-    const int s = ch_ryr(w, i);
-    n += (s == 3 || s == 4 || s == 7);
-  }
-  return n;
+  uint32_t m = 0;
+  for (int i = 0; i < kNryr; ++i) m |= ((0x4Cu >> ((w >> (5 + 3 * i)) & 7u)) & 1u) << i;
+  return __builtin_popcount(m);
 }
 
 // ---- exit list of one channel (at most 3 exits)
