@@ -26,13 +26,22 @@ host.  Two further measurements share the line:
   (each cell's first stimulus at 10 ms included); beats/s counts simulated
   cycle lengths completed per wall second, and a sample of the same cells run
   one after another through the single-cell driver gives the sequential rate.
+
+``roofline.traffic`` is the kernel's measured HBM traffic per launch: bench.py
+runs itself twice more under ``rocprofv3 --pmc`` (FETCH_SIZE, then WRITE_SIZE)
+over the first 35 simulated ms of the same cell, and twice for a 256 MiB
+device-to-device copy whose counter values give the bytes per counter unit
+(``--no-traffic`` skips this; without rocprofv3 it is null).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -63,6 +72,8 @@ def parse():
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs that measure roofline.traffic")
     ap.add_argument("--batch-cells", type=int, default=256, help="cells of the batched run (0 skips it)")
     ap.add_argument("--batch-ms", type=int, default=30, help="simulated ms of the batched run")
     ap.add_argument("--batch-seq-sample", type=int, default=8,
@@ -115,6 +126,86 @@ def calibrate_copy(mib: int) -> None:
     hip.hipFree(b)
     print(json.dumps({"calibration": "copy", "bytes_read_per_launch": nbytes,
                       "bytes_written_per_launch": nbytes, "launches": 5}), flush=True)
+
+
+def _counter_means(folder: str, counter: str) -> dict:
+    """Mean value per dispatch of one counter, per (kernel, workgroups), from a
+    rocprofv3 --output-format csv folder."""
+    import csv
+    import glob
+    from collections import defaultdict
+
+    acc = defaultdict(list)
+    for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                grid = int(r["Grid_Size"]) // max(int(r["Workgroup_Size"]), 1)
+                acc[(r["Kernel_Name"], grid)].append(float(r["Counter_Value"]))
+    return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+
+def measure_traffic(args) -> dict | None:
+    """HBM bytes per launch of k_dev_stage: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) over t = 0..35 ms of the same cell (stimulus at 10 ms and the
+    start of release), each in its own rocprofv3 run,
+    scaled by the bytes per KiB-unit the same counter shows for a 256 MiB
+    device-to-device copy (FETCH_SIZE reads half the copied bytes on MI355X).
+    Returns None when rocprofv3 is missing or a run fails."""
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+    win_ms = 35  # from t = 0 (no warmup, so that the launch count covers every dispatch)
+    cell_cmd = [sys.executable, os.path.abspath(__file__), "--steps", str(win_ms), "--warmup", "0",
+                "--nfru", str(args.nfru), "--seed", str(args.seed), "--no-mixed", "--batch-cells", "0",
+                "--cpu-ms", "0", "--no-timing", "--no-traffic"]
+    copy_cmd = [sys.executable, os.path.abspath(__file__), "--calibrate-copy-mib", "256"]
+    out = {}
+    with tempfile.TemporaryDirectory(prefix="grw_pmc_", dir=env["TMPDIR"]) as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            for tag, cmd in (("cell", cell_cmd), ("copy", copy_cmd)):
+                d = os.path.join(tmp, f"{tag}_{counter}")
+                try:
+                    r = subprocess.run([prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", tag, "--"] + cmd,
+                                       capture_output=True, text=True, timeout=180, env=env, start_new_session=True)
+                except subprocess.TimeoutExpired:
+                    return None
+                if tag == "cell":
+                    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+                    if not lines:
+                        return None
+                    out[f"cell_{counter}"] = json.loads(lines[-1])
+                means = _counter_means(d, counter)
+                if not means:
+                    return None
+                out[f"{tag}_{counter}_means"] = means
+    nbytes = 256 * (1 << 20)
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        copy = [v for (k, g), v in out[f"copy_{counter}_means"].items() if "copyBuffer" in k]
+        stage = [(v, g) for (k, g), v in out[f"cell_{counter}_means"].items() if "k_dev_stage" in k]
+        if not copy or not stage:
+            return None
+        (kib_copy, _n) = max(copy)
+        (kib, ndisp), grid = max(stage, key=lambda x: x[0][1])
+        res[counter] = dict(kib_per_dispatch=round(kib, 1), dispatches=ndisp, workgroups=grid,
+                            bytes_per_kib=round(nbytes / (kib_copy * 1024.0), 4))
+    cfg = out["cell_FETCH_SIZE"]["config"]
+    launches = cfg.get("launches") or 0
+    if not launches:
+        return None
+    f, w = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    read = f["kib_per_dispatch"] * 1024.0 * f["bytes_per_kib"] * f["dispatches"] / launches
+    written = w["kib_per_dispatch"] * 1024.0 * w["bytes_per_kib"] * w["dispatches"] / launches
+    alg = cfg.get("alg_bytes_per_launch")
+    return dict(bytes_per_launch=round(read + written, 1), read_per_launch=round(read, 1),
+                written_per_launch=round(written, 1),
+                window="t = 0..%d ms of the same cell (%d launches)" % (win_ms, launches),
+                alg_bytes_per_launch_window=alg,
+                ratio_to_algorithmic=round((read + written) / alg, 3) if alg else None,
+                counters=res)
 
 
 def summarize(folder: str) -> None:
@@ -527,7 +618,10 @@ def main():
                    "nfru": args.nfru, "precision": args.precision, "lanes_per_fru": args.lanes,
                    "driver": args.driver,
                    "rk_attempts": attempts, "rk_rejects": rejects,
-                   "fru_steps": st["fru_steps"], "peak_V_mV": round(float(V.max()), 3),
+                   "fru_steps": st["fru_steps"], "launches": st["launches"],
+                   "alg_bytes_per_launch": round((BYTES_PER_STEP * st["fru_steps"] + BYTES_PER_CALL *
+                                                  st["simfru_calls"]) / max(st["launches"], 1), 1),
+                   "peak_V_mV": round(float(V.max()), 3),
                    "peak_Cai_uM": round(float(states[:, 6].max()) * 1e3, 4)},
         "roofline": roof,
     }
@@ -546,6 +640,11 @@ def main():
                              if mbeat and fbeat else None)
     if args.batch_cells > 0:
         line["batched"] = batched(args)
+    if roof is not None and not args.no_traffic and args.driver == "device" and args.precision == "fp64":
+        tr = measure_traffic(args)
+        if tr is not None:
+            roof["traffic"] = tr["bytes_per_launch"]
+            roof["traffic_detail"] = tr
     if args.cpu_ms > 0:
         line["cpu_baseline"] = cpu_baseline(args.nfru, args.cpu_ms, args.seed)
         line["cpu_baseline"]["value"] = round(line["cpu_baseline"]["value"], 5)
