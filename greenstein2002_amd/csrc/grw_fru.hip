@@ -1483,7 +1483,10 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       if (q < s0) k_j[q] = ctl->k[q][j];
   }
   __shared__ double tot[17];
-  if (!reduce_range<17>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)) return;
+  // the eleven average sums (tot[6..16]) exist only in the attempt's last stage
+  if (a.avg ? !reduce_range<17>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)
+            : !reduce_range<6>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot))
+    return;
   const unsigned long long tail0 = wall_clock64();
   if (threadIdx.x == 0 && base.ncells == 1) close_launch_stats(d);
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
