@@ -214,15 +214,58 @@ __device__ __forceinline__ int pick(double u, double dt, const Exits<R>& e) {
 // d.part[b * kSlot, (b + 1) * kSlot); slot_off + NV must not exceed kSlot.
 constexpr int kSlot = 32;
 
+// The wavefront's xor butterfly v += v[lane ^ k] for k = 32, 16, 8, 4, 2, 1 in
+// that order, with each step's partner value moved by lane permutes instead of
+// LDS.  k = 32, 16: the permlane swaps leave the two registers of a lane
+// holding v_i and v_(i^k) in some order, and their sum is the same either way;
+// k = 8, 4: DPP row rotations by 8 and 4, whose source lane is i ^ 8, or i ^ 4
+// or i ^ 12 (after the first three steps lanes i ^ 4 and i ^ 12 hold the same
+// value); k = 2, 1: quad permutes.  Every step adds the same two numbers as
+// the plain butterfly, so the result is the same in every lane, bit for bit.
+__device__ __forceinline__ double swap_sum(double v, bool rows16) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)b, hi = (unsigned)((unsigned long long)b >> 32);
+  unsigned alo, clo, ahi, chi;
+  if (rows16) {
+    const auto pl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto ph = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    alo = pl[0]; clo = pl[1]; ahi = ph[0]; chi = ph[1];
+  } else {
+    const auto pl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto ph = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    alo = pl[0]; clo = pl[1]; ahi = ph[0]; chi = ph[1];
+  }
+  const double a = __longlong_as_double((long long)(((unsigned long long)ahi << 32) | alo));
+  const double c = __longlong_as_double((long long)(((unsigned long long)chi << 32) | clo));
+  return a + c;
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)b, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return v + __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double wave_sum_xor(double v) {
+  v = swap_sum(v, false);  // k = 32
+  v = swap_sum(v, true);   // k = 16
+  v = dpp_add<0x128>(v);   // row_ror:8, k = 8
+  v = dpp_add<0x124>(v);   // row_ror:4, k = 4
+  v = dpp_add<0x4E>(v);    // quad_perm [2,3,0,1], k = 2
+  v = dpp_add<0xB1>(v);    // quad_perm [1,0,3,2], k = 1
+  return v;
+}
+
 template <int NV>
 __device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigned int* ticket, int b0, int nb,
                              int slot_off, double* tot_sh) {
   static_assert(NV <= kSlot, "block partials overflow their slot row");
   __shared__ double wsum[4][NV];
   __shared__ bool last;
-  for (int off = 32; off >= 1; off >>= 1)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off);
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum_xor(v[k]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   if (lane == 0)
 #pragma unroll
@@ -250,9 +293,8 @@ __device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigne
     for (int k = 0; k < NV; ++k)
       s[k] += __hip_atomic_load(&d.part[(size_t)b * kSlot + slot_off + k], __ATOMIC_RELAXED,
                                 __HIP_MEMORY_SCOPE_AGENT);
-  for (int off = 32; off >= 1; off >>= 1)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) s[k] += __shfl_xor(s[k], off);
+  for (int k = 0; k < NV; ++k) s[k] = wave_sum_xor(s[k]);
   __syncthreads();
   if (lane == 0)
 #pragma unroll
