@@ -248,6 +248,24 @@ __device__ __forceinline__ double dpp_add(double v) {
   return v + __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// Sum and maximum of a 32-bit value over the wavefront, by the same lane
+// permutes (integers: the order is immaterial).
+template <bool MAX>
+__device__ __forceinline__ unsigned wave_u32(unsigned v) {
+  const auto p32 = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  v = MAX ? max((unsigned)p32[0], (unsigned)p32[1]) : (unsigned)p32[0] + (unsigned)p32[1];
+  const auto p16 = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+  v = MAX ? max((unsigned)p16[0], (unsigned)p16[1]) : (unsigned)p16[0] + (unsigned)p16[1];
+  unsigned w = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x128, 0xF, 0xF, false);
+  v = MAX ? max(v, w) : v + w;
+  w = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x124, 0xF, 0xF, false);
+  v = MAX ? max(v, w) : v + w;
+  w = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false);
+  v = MAX ? max(v, w) : v + w;
+  w = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false);
+  return MAX ? max(v, w) : v + w;
+}
+
 __device__ __forceinline__ double wave_sum_xor(double v) {
   v = swap_sum(v, false);  // k = 32
   v = swap_sum(v, true);   // k = 16
@@ -272,18 +290,25 @@ __device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigne
     for (int k = 0; k < NV; ++k) wsum[wave][k] = v[k];
   __syncthreads();
   if (threadIdx.x == 0) {
+    // The partials go out as agent-scope atomic stores, which write through to
+    // the point where every XCD sees them, and the wait for their completion
+    // orders them before the ticket.  A device fence here would write back and
+    // invalidate this XCD's whole L2, evicting the MT rows of the blocks still
+    // running on it; the last block reads the partials with agent-scope atomic
+    // loads and needs no acquire fence either (nothing else it reads was
+    // written in this launch).
     for (int k = 0; k < NV; ++k) {
       double s = wsum[0][k];
       for (int w = 1; w < nw; ++w) s += wsum[w][k];
-      d.part[(size_t)blockIdx.x * kSlot + slot_off + k] = s;
+      __hip_atomic_store(&d.part[(size_t)blockIdx.x * kSlot + slot_off + k], s, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
+    __builtin_amdgcn_s_waitcnt(0);
+    const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     last = (t == (unsigned)nb - 1u);
   }
   __syncthreads();
   if (!last) return false;
-  __threadfence();
   // every thread of the last block sums a strided subset of the slots (fixed order)
   double s[NV];
 #pragma unroll
@@ -1141,15 +1166,13 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
   return steps;
 }
 
+// (a lane's count is at most kStepCap + 1, so the wavefront's sum fits 32 bits)
+static_assert((unsigned long long)(kStepCap + 1) * 64ull < (1ull << 32), "step counts overflow");
 __device__ __forceinline__ void count_steps(const Dev& d, unsigned steps) {
-  unsigned long long s = steps;
-  unsigned m = steps;
-  for (int off = 32; off >= 1; off >>= 1) {
-    s += __shfl_xor(s, off);
-    m = max(m, (unsigned)__shfl_xor((int)m, off));
-  }
+  const unsigned s = wave_u32<false>(steps);
+  const unsigned m = wave_u32<true>(steps);
   if ((threadIdx.x & 63) == 0 && s) {
-    atomicAdd(d.nsteps, s);
+    atomicAdd(d.nsteps, (unsigned long long)s);
     atomicMax(d.maxsteps, m);
   }
 }
