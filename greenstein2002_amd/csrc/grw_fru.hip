@@ -1371,11 +1371,18 @@ __device__ __forceinline__ void stage_bounds(int s, double start, double h, doub
 __device__ __forceinline__ void fcn_block(double t, double* st, double* F, double* cur, bool keepc, const double* J,
                           const GrwProtocol& p, FcnTrans* q) {
   const int j = threadIdx.x;
-  if (j == 0) fcn_prep(t, st, p, *q);
-  __syncthreads();
-  if (j < kFcnExp) q->ex[j] = exp(q->ex_arg[j]);  // one call kind per wavefront, side by side
-  else if (j >= 64 && j < 64 + kFcnLog) q->lg[j - 64] = log(q->lg_arg[j - 64]);
-  else if (j >= 128 && j < 128 + kFcnPow) q->pw[j - 128] = pow(q->pw_base[j - 128], q->pw_exp[j - 128]);
+  // fcn_prep's arguments and the calls, one per lane, one call kind per
+  // wavefront side by side (wavefront 3 is free for the caller's work)
+  if (j < kFcnExp) {
+    const double V = fcn_prep_V(t, st, p);
+    q->ex[j] = exp(fcn_ex_arg(j, V, rdiv(V, divk(RT_over_F))));
+  } else if (j >= 64 && j < 64 + kFcnLog) {
+    q->lg[j - 64] = log(fcn_lg_arg(j - 64, st));
+  } else if (j >= 128 && j < 128 + kFcnPow) {
+    double b, e;
+    fcn_pw_arg(j - 128, st, b, e);
+    q->pw[j - 128] = pow(b, e);
+  }
   __syncthreads();
   // fcn_finish with its four independent groups on the block's four wavefronts
   __shared__ FcnCa ca_sh;
@@ -1430,8 +1437,9 @@ __device__ __forceinline__ void attempt_start(RkCtl* ctl, const CellInfo& ci, co
       }
       ctl->avg_pending = 0;
     }
-  } else if (j == 64 && eval) {
-    // beside the averages on thread 0; fcn_block reads J only after its first barrier
+  } else if (j == 192 && eval) {
+    // beside the averages on thread 0 and fcn_block's transcendentals on
+    // wavefronts 0-2; fcn_block reads J only after its first barrier
     flux_currents(fl, st[0], st[6], st[7], ci.nfru, ci.scale, Jsh);
   }
   if (eval) fcn_block(tstart, st, F1, cursh, keepc, Jsh, ctl->prot, fq);
@@ -1553,7 +1561,6 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
             : !reduce_range<6>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot))
     return;
   const unsigned long long tail0 = wall_clock64();
-  if (threadIdx.x == 0 && base.ncells == 1) close_launch_stats(d);
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
   __shared__ double st[kNglobal], yv[kNglobal], F[kNglobal], kk[6][kNglobal], ev[kNglobal];
   __shared__ double Jsh[4];
@@ -1576,9 +1583,13 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       if (q < s) kk[q][j] = k_j[q];
   }
   __syncthreads();
-  // the flux conversion (mpi_master.c:401-468) on wavefront 1, beside fcn_prep
-  // on thread 0: fcn_block reads J only after its first barrier
-  if (j == 64) flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], ci.nfru, ci.scale, Jsh);
+  // the flux conversion (mpi_master.c:401-468) and the launch's step statistics
+  // (a returning atomic) on wavefront 3, beside fcn_block's transcendentals on
+  // wavefronts 0-2: fcn_block reads J only after its first barrier
+  if (j == 192) {
+    if (base.ncells == 1) close_launch_stats(d);
+    flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], ci.nfru, ci.scale, Jsh);
+  }
   fcn_block(a.t1, yv, F, nullptr, false, Jsh, ctl->prot, &fq);
   if (j == 0)
     for (int q = 0; q < 4; ++q) ctl->J[q] = Jsh[q];

@@ -81,11 +81,9 @@ struct FcnTrans {
   double lg[kFcnLog], pw[kFcnPow], ex[kFcnExp];
 };
 
-// The arguments, with the clamp and the floors of fcn.c:448-483, :552, :562.
-GRW_HD void fcn_prep(double t, const double* state, const GrwProtocol& p, FcnTrans& q) {
-  const double Nai = state[4], Ki = state[5];
-  double Cai = state[6];
-  const double CaNSR = state[7];
+// The membrane potential the transcendental arguments use: the state's, or
+// the clamp's under voltage clamp (fcn.c:448-483).
+GRW_HD double fcn_prep_V(double t, const double* state, const GrwProtocol& p) {
   double V = state[0];
   if (p.vclamp) {
     const double von = floor(rdiv(t - p.vclamp_shift, p.vclamp_period)) * p.vclamp_period;
@@ -95,55 +93,96 @@ GRW_HD void fcn_prep(double t, const double* state, const GrwProtocol& p, FcnTra
     else
       V = p.vclamp_hold;
   }
-  if (Cai <= 0.0) Cai = 1.e-15;
-  // the divisions are IEEE (rdiv, grw_model.h), in fewer device operations
-  q.lg_arg[0] = rdiv(Nao, Nai);
-  q.lg_arg[1] = rdiv(Ko, Ki);
-  q.lg_arg[2] = rdiv(Ko + 0.01833 * Nao, Ki + 0.01833 * Nai);
-  q.lg_arg[3] = rdiv(Cao, Cai);
+  return V;
+}
+
+// The arguments of fcn.c's state-dependent exp calls (fcn.c:448-1044), each of the form
+// a (V + b), a (V + b) / c with a = +-1, or a VF/RT, with the reference's
+// operations: row j holds the form and constants of argument j.  (V + 0 is V
+// for the exp, x / c is IEEE division by the constant, and 7.488 - V equals
+// -(V - 7.488) exactly.)
+struct ExArg {
+  int form;  // 0: a (V + b); 1: a (V + b) / c; 2: a VF/RT
+  double a, b;
+  DivK c;
+};
+constexpr DivK kNoDiv{1.0, 1.0};
+constexpr ExArg kExArg[kFcnExp] = {
+    {2, 1.0, 0.0, kNoDiv},                      // 0  VF/RT
+    {1, -1.0, -7.488, divk(5.98)},              // 1  (7.488 - V) / 5.98
+    {2, -0.1, 0.0, kNoDiv},                     // 2  -0.1 VF/RT
+    {2, 0.35, 0.0, kNoDiv},                     // 3  eta VF/RT
+    {0, -0.1, 47.13, kNoDiv},                   // 4
+    {1, -1.0, 0.0, divk(11.0)},                 // 5  -V / 11
+    {1, 1.0, 80.0, divk(-6.8)},                 // 6  (80 + V) / -6.8
+    {0, 0.079, 0.0, kNoDiv},                    // 7
+    {0, 0.35, 0.0, kNoDiv},                     // 8
+    {1, 1.0, 10.66, divk(-11.1)},               // 9
+    {0, 0.2444, 0.0, kNoDiv},                   // 10
+    {0, -0.04391, 0.0, kNoDiv},                 // 11
+    {0, 0.311, 79.23, kNoDiv},                  // 12
+    {0, -0.01052, 0.0, kNoDiv},                 // 13
+    {0, -0.1378, 40.14, kNoDiv},                // 14
+    {0, -2.535E-7, 0.0, kNoDiv},                // 15
+    {0, -0.1, 32.0, kNoDiv},                    // 16
+    {0, 0.03304608038835, 0.0, kNoDiv},         // 17
+    {0, -0.04306054163980, 0.0, kNoDiv},        // 18
+    {0, 0.02617412715118, 0.0, kNoDiv},         // 19
+    {0, -0.02691385498399, 0.0, kNoDiv},        // 20
+    {0, 0.00568908859717, 0.0, kNoDiv},         // 21
+    {0, -0.04536642959543, 0.0, kNoDiv},        // 22
+    {0, 0.00000069808924, 0.0, kNoDiv},         // 23
+    {1, -1.0, -24.7, divk(13.6)},               // 24 -(V - 24.7) / 13.6
+    {0, -0.148, -10.0, kNoDiv},                 // 25
+    {0, 0.0687, -10.0, kNoDiv},                 // 26
+    {0, 0.028983, 0.0, kNoDiv},                 // 27
+    {0, -0.0468437, 0.0, kNoDiv},               // 28
+    {0, -0.000373016, 0.0, kNoDiv},             // 29
+    {0, 0.00000005374, 0.0, kNoDiv},            // 30
+    {0, 0.00695, 0.0, kNoDiv},                  // 31
+    {0, -0.08527, 0.0, kNoDiv},                 // 32
+};
+
+GRW_HD double fcn_ex_arg(int j, double V, double VF_over_RT) {
+  const ExArg& r = kExArg[j];
+  const double x = r.a * (V + r.b);
+  if (r.form == 1) return rdiv(x, r.c);
+  return (r.form == 2) ? r.a * VF_over_RT : x;
+}
+
+// The log arguments (fcn.c:549-559; Cai floored at 1e-15, :556-557) and the
+// pow bases and exponents (fcn.c:631, :759-760).  The divisions are IEEE
+// (rdiv, grw_model.h).
+GRW_HD double fcn_lg_arg(int k, const double* state) {
+  const double Nai = state[4], Ki = state[5];
+  const double Cai = (state[6] <= 0.0) ? 1.e-15 : state[6];
+  switch (k) {
+    case 0: return rdiv(Nao, Nai);
+    case 1: return rdiv(Ko, Ki);
+    case 2: return rdiv(Ko + 0.01833 * Nao, Ki + 0.01833 * Nai);
+    default: return rdiv(Cao, Cai);
+  }
+}
+GRW_HD void fcn_pw_arg(int k, const double* state, double& base, double& expo) {
+  const double Nai = state[4];
+  const double Cai = (state[6] <= 0.0) ? 1.e-15 : state[6];
+  const double CaNSR = state[7];
   const double KmNai = 10.0, Kfb = 0.26e-3, Krb = 1.8, Nfb = 0.75, Nrb = 0.75;
-  q.pw_base[0] = rdiv(KmNai, Nai);
-  q.pw_exp[0] = 1.5;
-  q.pw_base[1] = rdiv(Cai, divk(Kfb));
-  q.pw_exp[1] = Nfb;
-  q.pw_base[2] = rdiv(CaNSR, divk(Krb));
-  q.pw_exp[2] = Nrb;
+  switch (k) {
+    case 0: base = rdiv(KmNai, Nai); expo = 1.5; return;
+    case 1: base = rdiv(Cai, divk(Kfb)); expo = Nfb; return;
+    default: base = rdiv(CaNSR, divk(Krb)); expo = Nrb; return;
+  }
+}
+
+// All arguments on one thread (the host fcn; the device integrator computes
+// them one per lane beside the calls, fcn_block in grw_fru.hip).
+GRW_HD void fcn_prep(double t, const double* state, const GrwProtocol& p, FcnTrans& q) {
+  const double V = fcn_prep_V(t, state, p);
+  for (int k = 0; k < kFcnLog; ++k) q.lg_arg[k] = fcn_lg_arg(k, state);
+  for (int k = 0; k < kFcnPow; ++k) fcn_pw_arg(k, state, q.pw_base[k], q.pw_exp[k]);
   const double VF_over_RT = rdiv(V, divk(RT_over_F));
-  const double eta = 0.35;
-  double* e = q.ex_arg;
-  e[0] = VF_over_RT;
-  e[1] = rdiv(7.488 - V, divk(5.98));
-  e[2] = -0.1 * VF_over_RT;
-  e[3] = eta * VF_over_RT;
-  e[4] = -0.1 * (V + 47.13);
-  e[5] = rdiv(-V, divk(11.0));
-  e[6] = rdiv(80 + V, divk(-6.8));
-  e[7] = 0.079 * V;
-  e[8] = 0.35 * V;
-  e[9] = rdiv(V + 10.66, divk(-11.1));
-  e[10] = 0.2444 * V;
-  e[11] = -0.04391 * V;
-  e[12] = 0.311 * (V + 79.23);
-  e[13] = -0.01052 * V;
-  e[14] = -0.1378 * (V + 40.14);
-  e[15] = -2.535E-7 * V;
-  e[16] = -0.1 * (V + 32);
-  e[17] = 0.03304608038835 * V;
-  e[18] = -0.04306054163980 * V;
-  e[19] = 0.02617412715118 * V;
-  e[20] = -0.02691385498399 * V;
-  e[21] = 0.00568908859717 * V;
-  e[22] = -0.04536642959543 * V;
-  e[23] = 0.00000069808924 * V;
-  e[24] = rdiv(-(V - 24.7), divk(13.6));
-  e[25] = -0.148 * (V - 10.0);
-  e[26] = 0.0687 * (V - 10.0);
-  e[27] = 0.028983 * V;
-  e[28] = -0.0468437 * V;
-  e[29] = -0.000373016 * V;
-  e[30] = 0.00000005374 * V;
-  e[31] = 0.00695 * V;
-  e[32] = -0.08527 * V;
+  for (int j = 0; j < kFcnExp; ++j) q.ex_arg[j] = fcn_ex_arg(j, V, VF_over_RT);
 }
 
 GRW_HD void fcn_trans(FcnTrans& q) {
