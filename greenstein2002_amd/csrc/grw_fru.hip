@@ -77,6 +77,7 @@ struct MtLane {
     const uint32_t w = far ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
     ++nreg;
     row[p] = w;
+    if (p < kMtMirror) row[p + kMtN] = w;  // the copy of words 0..7 (kMtMirror)
     idx = (p == kMtN - 1) ? kMtN : idx + 1;
     return temper(w);
   }
@@ -116,6 +117,7 @@ struct MtLane {
         int q = p + k;
         q -= (q >= kMtN) ? kMtN : 0;
         row[q] = far[k];
+        if (q < kMtMirror) row[q + kMtN] = far[k];  // the copy of words 0..7
       }
       nreg += 32;
       idx += 32;
@@ -663,13 +665,6 @@ static_assert(((ryr_resolve_bits(2) >> 16) & 15u) == 7u, "regime 2: state 5 coll
 static_assert(((ryr_resolve_bits(2) >> 24) & 15u) == 14u, "regime 2: state 7 needs a uniform");
 static_assert(((ryr_resolve_bits(3) >> 4) & 15u) == 1u, "regime 3: state 2 stays");
 
-// The four-lane kernel keeps a second copy of row words 0..7 in the row's
-// padding, positions 624..631 (refreshed by the quad when a launch starts,
-// and every store to either copy goes to both), so that a window of words
-// running past the end of the generation reads on without wrapping.
-constexpr int kMtMirror = 8;
-static_assert(kMtN + kMtMirror <= kMtStride, "the copy lives in the row padding");
-
 // Row position of absolute stream position g in [0, 1872): positions below 624
 // are words of a loaded full generation, read as is; the others are lazy.
 __device__ __forceinline__ int mt_row_pos(int g) {
@@ -711,7 +706,7 @@ __device__ __forceinline__ void mt_run(uint32_t* row, int g0, int n, uint32_t ou
         const uint32_t wv = mt_twist(cur[k], nxt[k], far[k]);
         const int pos = mt_row_pos(g);
         row[pos] = wv;
-        if (pos < kMtMirror) row[pos + kMtN] = wv;  // the row-end copy (mt_finish8)
+        if (pos < kMtMirror) row[pos + kMtN] = wv;  // the copy of words 0..7 (kMtMirror)
         out[k] = MtLane::temper(wv);
       } else {
         out[k] = MtLane::temper(cur[k]);
@@ -801,11 +796,17 @@ __device__ __forceinline__ void mt_rollback(uint32_t* row, int idx_h, int nreg, 
 #pragma unroll
     for (int k = 1; k < 33; ++k) {
       const int o = lo - 1 + k;
-      if (o <= hi) row[(q0 + o) % kMtN] = (y[k] & 0x80000000u) | (y[k - 1] & 0x7fffffffu);
+      if (o <= hi) {
+        const int q = (q0 + o) % kMtN;
+        const uint32_t v = (y[k] & 0x80000000u) | (y[k - 1] & 0x7fffffffu);
+        row[q] = v;
+        if (q < kMtMirror) row[q + kMtN] = v;  // the copy of words 0..7
+      }
     }
     hi = lo - 1;
   }
   row[q0] = w_first;
+  if (q0 < kMtMirror) row[q0 + kMtN] = w_first;
 }
 
 // Row word that the first regeneration after a checkpoint at idx overwrites.
@@ -909,10 +910,6 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
         }
       }
     }
-    // the row-end copy of words 0..7 (see kMtMirror): lanes 0 and 1 of the quad
-    // copy four words each; the fence makes them visible to the quad
-    if (c < 2) reinterpret_cast<uint4*>(row + kMtN)[c] = reinterpret_cast<const uint4*>(row)[c];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     const double t0 = in_vgpr(a.t0), t1 = in_vgpr(a.t1);
     const double o0 = in_vgpr(a.dep0[0]), o1 = in_vgpr(a.dep0[1]), o2 = in_vgpr(a.dep0[2]);
     const double span0 = in_vgpr(a.depf[0] - a.dep0[0]);
@@ -1489,7 +1486,7 @@ __device__ __forceinline__ void attempt_start(RkCtl* ctl, const CellInfo& ci, co
 // unbroken run; then attempt_start in the cell's last block.
 template <class R>
 __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev d, RkCtl* ctls) {
-  const int cell = d.smap[blockIdx.x];
+  const int cell = base.ncells == 1 ? 0 : d.smap[blockIdx.x];  // (one cell: no dependent load)
   const CellInfo ci = d.cells[cell];
   RkCtl* ctl = ctls + cell;
   const int active = ctl->active, avgp = ctl->avg_pending;
@@ -1523,7 +1520,7 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
 // output loop (caf.c:150-251).
 template <class R>
 __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctls) {
-  const int cell = d.smap[blockIdx.x];
+  const int cell = base.ncells == 1 ? 0 : d.smap[blockIdx.x];  // (one cell: no dependent load)
   const CellInfo ci = d.cells[cell];
   RkCtl* ctl = ctls + cell;
   if (!ctl->active) return;
@@ -1955,9 +1952,11 @@ __global__ void k_seed_cells(const SimArgs a, const Dev d) {
   const int f = ci.f0 + il;
   uint32_t* row = d.mt + (size_t)f * kMtStride;
   row[0] = x;
+  row[kMtN] = x;
   for (uint32_t k = 1; k < kMtN; ++k) {
     x = 1812433253u * (x ^ (x >> 30)) + k;
     row[k] = x;
+    if (k < (uint32_t)kMtMirror) row[k + kMtN] = x;  // the copy of words 0..7
   }
   d.mtidx[f] = kMtN;  // the first draw regenerates word 0
   d.nreg[f] = 0;
@@ -2434,6 +2433,7 @@ int grw_load_state(grw_ctx* c, const double* fru, const int32_t* ltype, const in
     if (mti[i] < 0 || mti[i] > kMtN) return fail(GRW_EINVAL, "mti out of range at FRU " + std::to_string(i));
     idx[i] = mti[i];  // mti == 624 is lazy position 0 of the next generation
     std::memcpy(&rows[(size_t)i * kMtStride], mt + (size_t)i * kMtN, kMtN * sizeof(uint32_t));
+    std::memcpy(&rows[(size_t)i * kMtStride + kMtN], mt + (size_t)i * kMtN, kMtMirror * sizeof(uint32_t));
   }
   HIPCHK(hipMemcpyAsync(c->d.conc, conc.data(), conc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d.chan, chan.data(), chan.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));

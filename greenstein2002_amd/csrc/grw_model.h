@@ -21,6 +21,11 @@ constexpr int kNcleft = 4;
 constexpr int kNryr = 5;
 constexpr int kMtN = 624;
 constexpr int kMtStride = 640;  // words per MT row in HBM (2560 B, 128 B aligned)
+// Row positions 624..631 (in the row's padding) always hold a second copy of
+// words 0..7: every writer of a row stores both copies, so that a window of
+// words running past the end of the generation reads on without wrapping.
+constexpr int kMtMirror = 8;
+static_assert(kMtN + kMtMirror <= kMtStride, "the copy lives in the row padding");
 
 // ---- physical constants: parameters_fcn_fru.c:21-90 (same constant expressions)
 constexpr double Ko = 4.0, Nao = 138.0, Cao = 2.0, Clo = 150.0, Cli = 20.0;
