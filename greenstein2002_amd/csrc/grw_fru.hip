@@ -190,6 +190,8 @@ struct Dev {
   const int* bmap;                  // one-thread-per-FRU grid block -> cell
   int32_t* perm;                    // [ld]: FRU (cell-local) of each quad slot of the cell, when sorted
   uint32_t* cost;                   // [ld]: micro-steps of each FRU in its last sub-interval, when sorted
+  int32_t* bkey;                    // [grid]: cost class of each Monte Carlo block's costliest FRU, when sorted
+  int32_t* border;                  // [grid]: Monte Carlo block run by each dispatched block, when sorted
 };
 
 // Categorical transition draw, simfru_local.c:217-227 / :234-245: stay with
@@ -279,8 +281,8 @@ __device__ __forceinline__ double wave_sum_xor(double v) {
 }
 
 template <int NV>
-__device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigned int* ticket, int b0, int nb,
-                             int slot_off, double* tot_sh) {
+__device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigned int* ticket, int blk, int b0,
+                             int nb, int slot_off, double* tot_sh) {
   static_assert(NV <= kSlot, "block partials overflow their slot row");
   __shared__ double wsum[4][NV];
   __shared__ bool last;
@@ -302,7 +304,7 @@ __device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigne
     for (int k = 0; k < NV; ++k) {
       double s = wsum[0][k];
       for (int w = 1; w < nw; ++w) s += wsum[w][k];
-      __hip_atomic_store(&d.part[(size_t)blockIdx.x * kSlot + slot_off + k], s, __ATOMIC_RELAXED,
+      __hip_atomic_store(&d.part[(size_t)blk * kSlot + slot_off + k], s, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     }
     __builtin_amdgcn_s_waitcnt(0);
@@ -341,7 +343,7 @@ __device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigne
 
 template <int NV>
 __device__ __forceinline__ bool reduce_grid_block(double v[NV], const Dev& d, int slot_off, double* tot_sh) {
-  return reduce_range<NV>(v, d, d.ticket, 0, gridDim.x, slot_off, tot_sh);
+  return reduce_range<NV>(v, d, d.ticket, blockIdx.x, 0, gridDim.x, slot_off, tot_sh);
 }
 
 // Thread 0 of the last block gets the totals in tot[] (returns true there only).
@@ -873,8 +875,8 @@ __device__ __forceinline__ T in_vgpr(T x) {
 // One sub-interval of every FRU with four lanes per FRU; returns this lane's
 // FRU-step count (nonzero only in lane 0 of a quad) and its flux terms in acc.
 template <class R>
-__device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[17]) {
-  const int gid = (blockIdx.x - a.b0) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[17], int blk) {
+  const int gid = (blk - a.b0) * blockDim.x + threadIdx.x;
   const int slot = gid >> 2;
   const int c = gid & 3;
   unsigned steps = 0;
@@ -1184,7 +1186,7 @@ __device__ __forceinline__ void close_launch_stats(const Dev& d) {
 template <class R, bool REDUCE>
 __global__ void __launch_bounds__(256) k_simfru4(const SimArgs a, const Dev d) {
   double acc[17] = {};
-  count_steps(d, simfru4_body<R>(a, d, acc));
+  count_steps(d, simfru4_body<R>(a, d, acc, blockIdx.x));
   if (REDUCE) {
     double tot[5];
     if (reduce_grid<5>(acc, d, 0, tot)) {
@@ -1478,6 +1480,14 @@ __device__ __forceinline__ void attempt_start(RkCtl* ctl, const CellInfo& ci, co
   }
 }
 
+// The Monte Carlo block a dispatched block runs.  A batch dispatches its blocks
+// costliest first (k_order_blocks), so that the long ones do not start last
+// and leave the launch waiting on them; a block's FRUs, its partial-sum slot
+// and so every result are those of the block it runs, whatever the order.
+__device__ __forceinline__ int mc_block(const SimArgs& base, const Dev& d) {
+  return base.sorted ? d.border[blockIdx.x] : (int)blockIdx.x;
+}
+
 // Attempt start of every cell at the beginning of a run (later attempts start
 // in the tail of the previous attempt's last stage): a sub-interval of length
 // zero through the stage kernel's per-FRU code, which rolls every FRU back or
@@ -1486,7 +1496,8 @@ __device__ __forceinline__ void attempt_start(RkCtl* ctl, const CellInfo& ci, co
 // unbroken run; then attempt_start in the cell's last block.
 template <class R>
 __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev d, RkCtl* ctls) {
-  const int cell = base.ncells == 1 ? 0 : d.smap[blockIdx.x];  // (one cell: no dependent load)
+  const int blk = mc_block(base, d);
+  const int cell = base.ncells == 1 ? 0 : d.smap[blk];  // (one cell: no dependent load)
   const CellInfo ci = d.cells[cell];
   RkCtl* ctl = ctls + cell;
   const int active = ctl->active, avgp = ctl->avg_pending;
@@ -1504,9 +1515,9 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
   a.pro = !active ? 0 : ctl->resume ? 2 : ctl->success ? 1 : 0;  // mpi_slave.c:150-226
   a.avg = avgp;
   double acc[17] = {};
-  (void)simfru4_body<R>(a, d, acc);
+  (void)simfru4_body<R>(a, d, acc, blk);
   __shared__ double tot[17];
-  if (!reduce_range<17>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)) return;
+  if (!reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot)) return;
   __shared__ double st[kNglobal], F1[kNglobal], Jsh[4], cursh[kNcur];
   __shared__ FcnTrans fq;
   if (threadIdx.x == 0) ctl->pro = 0;  // this launch did the per-FRU work
@@ -1520,7 +1531,8 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
 // output loop (caf.c:150-251).
 template <class R>
 __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctls) {
-  const int cell = base.ncells == 1 ? 0 : d.smap[blockIdx.x];  // (one cell: no dependent load)
+  const int blk = mc_block(base, d);
+  const int cell = base.ncells == 1 ? 0 : d.smap[blk];  // (one cell: no dependent load)
   const CellInfo ci = d.cells[cell];
   RkCtl* ctl = ctls + cell;
   if (!ctl->active) return;
@@ -1540,7 +1552,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   a.pro = (s0 == 1) ? ctl->pro : 0;
   a.avg = (s0 == 5);  // the attempt's end state: averages for an output sample
   double acc[17] = {};
-  count_steps(d, simfru4_body<R>(a, d, acc));
+  count_steps(d, simfru4_body<R>(a, d, acc, blk));
   // the control block's state vectors for the tail, loaded by every block
   // before the reduction so that the last block does not wait for them after it
   const int j = threadIdx.x;
@@ -1554,8 +1566,8 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   }
   __shared__ double tot[17];
   // the eleven average sums (tot[6..16]) exist only in the attempt's last stage
-  if (a.avg ? !reduce_range<17>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot)
-            : !reduce_range<6>(acc, d, d.ticket + cell, ci.sb0, ci.snb, 0, tot))
+  if (a.avg ? !reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot)
+            : !reduce_range<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot))
     return;
   const unsigned long long tail0 = wall_clock64();
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
@@ -1755,17 +1767,27 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
 // index, so only the order of the flux and average sums changes; a cell that
 // has finished its run keeps its order, so a run continued in pieces sees the
 // same orders as one unbroken run.
-constexpr int kSortBins = 8;
+constexpr int kSortBins = 16;
 
 __device__ __forceinline__ int cost_bin(uint32_t s) {
-  return (s >= 2) + (s >= 3) + (s >= 4) + (s >= 6) + (s >= 9) + (s >= 14) + (s >= 22);
+  constexpr uint32_t edge[kSortBins - 1] = {2, 3, 4, 6, 9, 14, 22, 34, 52, 80, 120, 180, 270, 400, 600};
+  int b = 0;
+#pragma unroll
+  for (int k = 0; k < kSortBins - 1; ++k) b += (s >= edge[k]);
+  return b;
 }
 
+// Each block's key for the dispatch order: the cost class of its last quad
+// slot, the costliest of the block since the slots ascend by class; -1 for the
+// blocks of a cell that has finished.
 __global__ void __launch_bounds__(256) k_sort_cells(const Dev d, const RkCtl* ctls) {
   const int cell = blockIdx.x;
-  if (!ctls[cell].active) return;
   const CellInfo ci = d.cells[cell];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (!ctls[cell].active) {
+    for (int b = t; b < ci.snb; b += blockDim.x) d.bkey[ci.sb0 + b] = -1;
+    return;
+  }
   const int per = (ci.nfru + 255) / 256;
   const int lo = min(ci.nfru, t * per), hi = min(ci.nfru, lo + per);
   int cnt[kSortBins];
@@ -1815,6 +1837,68 @@ __global__ void __launch_bounds__(256) k_sort_cells(const Dev d, const RkCtl* ct
       pos[b] += (bin == b);
     }
     d.perm[ci.f0 + p] = i;
+    if ((p & 63) == 63 || p == ci.nfru - 1) d.bkey[ci.sb0 + (p >> 6)] = bin;
+  }
+}
+
+// The dispatch order of a batch's Monte Carlo blocks: by key, highest first,
+// block index order within a key (a stable counting sort in one block of 1024
+// threads, each over a contiguous range of blocks).  Longest-first list
+// scheduling: the blocks that take the most micro-steps start in the first
+// wave of the launch and the short ones fill the compute units behind them.
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderRanks = kSortBins + 1;  // the keys kSortBins-1 .. 0, then -1
+
+__global__ void __launch_bounds__(kOrderThreads) k_order_blocks(const Dev d, int nblk) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  constexpr int nw = kOrderThreads / 64;
+  const int per = (nblk + kOrderThreads - 1) / kOrderThreads;
+  const int lo = min(nblk, t * per), hi = min(nblk, lo + per);
+  int cnt[kOrderRanks];
+#pragma unroll
+  for (int r = 0; r < kOrderRanks; ++r) cnt[r] = 0;
+  for (int i = lo; i < hi; ++i) {
+    const int r = kSortBins - 1 - d.bkey[i];
+#pragma unroll
+    for (int q = 0; q < kOrderRanks; ++q) cnt[q] += (r == q);
+  }
+  __shared__ int wtot[kOrderRanks][nw];
+  __shared__ int rbase[kOrderRanks];
+  int pos[kOrderRanks];
+#pragma unroll
+  for (int r = 0; r < kOrderRanks; ++r) {
+    int x = cnt[r];
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    pos[r] = x - cnt[r];
+    if (lane == 63) wtot[r][wave] = x;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int r = 0; r < kOrderRanks; ++r) {
+      rbase[r] = acc;
+      for (int w = 0; w < nw; ++w) acc += wtot[r][w];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kOrderRanks; ++r) {
+    int off = rbase[r];
+    for (int w = 0; w < wave; ++w) off += wtot[r][w];
+    pos[r] += off;
+  }
+  for (int i = lo; i < hi; ++i) {
+    const int r = kSortBins - 1 - d.bkey[i];
+    int p = 0;
+#pragma unroll
+    for (int q = 0; q < kOrderRanks; ++q) {
+      p = (r == q) ? pos[q] : p;
+      pos[q] += (r == q);
+    }
+    d.border[p] = i;
   }
 }
 
@@ -2090,7 +2174,8 @@ int alloc_fru_state(grw_ctx* c) {
       (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
       (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
       (rc = dalloc(c, &c->d.tailticks, 1)) || (rc = dalloc(c, &c->d.perm, ld)) ||
-      (rc = dalloc(c, &c->d.cost, ld)))
+      (rc = dalloc(c, &c->d.cost, ld)) || (rc = dalloc(c, &c->d.bkey, (size_t)c->grid_s)) ||
+      (rc = dalloc(c, &c->d.border, (size_t)c->grid_s)))
     return rc;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_out, 32 * sizeof(double), hipHostMallocDefault);
@@ -2171,6 +2256,13 @@ void reset_tracking(grw_ctx* c) {
 
 constexpr int kTimeEvery = 8;  // the device integrator times one RK attempt in eight
 
+// Between RK attempts of a batch: regroup every active cell's FRUs by cost,
+// then order the Monte Carlo blocks for dispatch.
+void sort_and_order(grw_ctx* c) {
+  hipLaunchKernelGGL(k_sort_cells, dim3(c->ncells), dim3(256), 0, c->stream, c->d, c->dctl);
+  hipLaunchKernelGGL(k_order_blocks, dim3(1), dim3(kOrderThreads), 0, c->stream, c->d, c->grid_s);
+}
+
 // The FRU order of every cell back to the identity, step counts to zero (a
 // new initial state).
 int reset_order(grw_ctx* c) {
@@ -2179,6 +2271,9 @@ int reset_order(grw_ctx* c) {
     for (int i = 0; i < ci.nfru; ++i) perm[(size_t)ci.f0 + i] = i;
   HIPCHK(hipMemcpy(c->d.perm, perm.data(), perm.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(c->d.cost, 0, (size_t)c->ld * sizeof(uint32_t)));
+  std::vector<int32_t> order((size_t)c->grid_s);
+  for (int b = 0; b < c->grid_s; ++b) order[(size_t)b] = b;
+  HIPCHK(hipMemcpy(c->d.border, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -2227,7 +2322,7 @@ int run_device(grw_ctx* c) {
         hipLaunchKernelGGL(k_dev_begin<float>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
       else
         hipLaunchKernelGGL(k_dev_begin<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
-      if (base.sorted) hipLaunchKernelGGL(k_sort_cells, dim3(nc), dim3(256), 0, c->stream, c->d, c->dctl);
+      if (base.sorted) sort_and_order(c);
     }
     first = false;
     for (int k = 0; k < K; ++k) {
@@ -2241,7 +2336,7 @@ int run_device(grw_ctx* c) {
           hipLaunchKernelGGL(k_dev_stage<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
         if (tk) (void)hipEventRecord(ek[2 * s5 + 1], c->stream);
       }
-      if (base.sorted) hipLaunchKernelGGL(k_sort_cells, dim3(nc), dim3(256), 0, c->stream, c->d, c->dctl);
+      if (base.sorted) sort_and_order(c);
     }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
