@@ -1360,23 +1360,37 @@ __device__ __forceinline__ void stage_bounds(int s, double start, double h, doub
 }
 
 // fcn (fcn.c:29-1072) for the integrator tails, by a whole block of 256
-// threads: thread 0 prepares the arguments of its transcendental calls, three
-// wavefronts evaluate them side by side (the exps, the logs and the pows, one
-// call each), then the four independent groups of fcn_finish
-// run on the four wavefronts (lane 0 of each: currents, Ito1 lattices, gates,
-// Ca handling) and thread 0 assembles the derivatives.  The values are those
-// of the one-thread fcn.  Call from every thread; J and cur are read / written
-// by thread 0 only, st is read by all and written by thread 0.
+// threads.  First the transcendental calls, one per lane and one call kind per
+// wavefront side by side (a wavefront runs the arms its lanes take one after
+// the other): the exps on wavefront 0, which also evaluates fcn_voltage for
+// the groups; the logs on wavefront 1, then there the IK1 factor's exp, whose
+// argument needs EK; the pows on wavefront 2; wavefront 3 is free for the
+// caller's work.  Then the independent groups of fcn_finish: the membrane
+// currents (thread 0), the two Ito1 lattices side by side (threads 64 and 65),
+// the gates (thread 128) and the Ca handling (thread 192); thread 0 assembles
+// the derivatives.  The values are those of the one-thread fcn.  Call from
+// every thread; J and cur are read / written by thread 0 only, st is read by
+// all and written by thread 0.
 __device__ __forceinline__ void fcn_block(double t, double* st, double* F, double* cur, bool keepc, const double* J,
                           const GrwProtocol& p, FcnTrans* q) {
   const int j = threadIdx.x;
-  // fcn_prep's arguments and the calls, one per lane, one call kind per
-  // wavefront side by side (wavefront 3 is free for the caller's work)
-  if (j < kFcnExp) {
-    const double V = fcn_prep_V(t, st, p);
-    q->ex[j] = exp(fcn_ex_arg(j, V, rdiv(V, divk(RT_over_F))));
-  } else if (j >= 64 && j < 64 + kFcnLog) {
-    q->lg[j - 64] = log(fcn_lg_arg(j - 64, st));
+  if (j < 64) {
+    double Vv, Is;
+    fcn_voltage(t, st, p, Vv, Is);
+    if (j == 0) {
+      q->V = Vv;
+      q->Istim = Is;
+    }
+    if (j < kFcnExp) {
+      const double V = fcn_prep_V(t, st, p);
+      q->ex[j] = exp(fcn_ex_arg(j, V, rdiv(V, divk(RT_over_F))));
+    }
+  } else if (j < 64 + kFcnLog) {
+    double Vv, Is;
+    fcn_voltage(t, st, p, Vv, Is);
+    const double lg = log(fcn_lg_arg(j - 64, st));
+    q->lg[j - 64] = lg;
+    if (j == 65) q->k1x = exp(fcn_k1_arg(Vv, lg));
   } else if (j >= 128 && j < 128 + kFcnPow) {
     double b, e;
     fcn_pw_arg(j - 128, st, b, e);
@@ -1386,16 +1400,16 @@ __device__ __forceinline__ void fcn_block(double t, double* st, double* F, doubl
   // fcn_finish with its four independent groups on the block's four wavefronts
   __shared__ FcnCa ca_sh;
   __shared__ double mset_sh;
-  double V, Istim;
+  double V = 0.0, Istim = 0.0;
   FcnCurrents c;
   if (j == 0) {
-    fcn_voltage(t, st, p, V, Istim);
-    fcn_currents(st, V, *q, c);
-  } else if (j == 64) {
-    fcn_ito(st, *q, F);
+    V = q->V;
+    Istim = q->Istim;
+    fcn_currents(st, V, *q, q->k1x, c);
+  } else if (j == 64 || j == 65) {
+    fcn_ito_lattice(st, *q, j - 64, F);
   } else if (j == 128) {
-    fcn_voltage(t, st, p, V, Istim);
-    fcn_gates(st, V, *q, F, &mset_sh);
+    fcn_gates(st, q->V, *q, F, &mset_sh);
   } else if (j == 192) {
     fcn_ca(st, p, *q, F, ca_sh);
   }

@@ -73,13 +73,26 @@ GRW_HD void ito_pack(const double dx[2][5], int i0, double* F) {
 // or time: their arguments (fcn_prep) and values (fcn_trans).  The host
 // evaluates them one by one; the device integrator evaluates each kind as one
 // call across the lanes of a wavefront (same function, same argument, same
-// value).  Calls with constant arguments and the one that needs EK stay in
-// fcn_finish.
+// value).  Calls with constant arguments stay in fcn_finish; the exp of the
+// IK1 factor, whose argument needs EK, follows the logs (fcn_k1_arg).
 constexpr int kFcnLog = 4, kFcnPow = 3, kFcnExp = 33;
 struct FcnTrans {
   double lg_arg[kFcnLog], pw_base[kFcnPow], pw_exp[kFcnPow], ex_arg[kFcnExp];
   double lg[kFcnLog], pw[kFcnPow], ex[kFcnExp];
+  double k1x;         // exp(fcn_k1_arg(V, lg[1])) (the device's; the host computes its own)
+  double V, Istim;    // fcn_voltage at the call's time (the device's)
 };
+
+// pow(x, 2.0) of fcn.c:774-775.  Compilers fold it into the product on the
+// host (exact: the product is the correctly rounded square); the device's math
+// library does not fold it, and its pow takes ~250 instructions.
+GRW_HD double pow2(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return x * x;
+#else
+  return pow(x, 2.0);
+#endif
+}
 
 // The membrane potential the transcendental arguments use: the state's, or
 // the clamp's under voltage clamp (fcn.c:448-483).
@@ -223,8 +236,16 @@ struct FcnCurrents {
   double INa, IKr, IKs, IKv43, IKv14, IKv14_K, IKv14_Na, IK1, INab, IKp, INaK, INaCa, ICab, IpCa;
 };
 
-// The sarcolemmal currents before the CHF scaling (fcn.c:485-746).
-GRW_HD void fcn_currents(const double* state, double V, const FcnTrans& q, FcnCurrents& c) {
+// The argument of the IK1 factor's exp (fcn.c:612): 1.76 (V - EK) F/RT
+// with EK = RT/F log(Ko / Ki).
+GRW_HD double fcn_k1_arg(double V, double lg1) {
+  const double EK = RT_over_F * lg1;
+  return 1.76 / RT_over_F * (V - EK);
+}
+
+// The sarcolemmal currents before the CHF scaling (fcn.c:485-746); k1x is
+// exp(fcn_k1_arg(V, q.lg[1])).
+GRW_HD void fcn_currents(const double* state, double V, const FcnTrans& q, double k1x, FcnCurrents& c) {
   const double GKr = 0.029, GKs = 0.035, GK1 = 3.0, GKp = 1.2 * 0.002216, GNa = 12.8;
   const double kNaCa = 0.9 * 0.30, KmNa = 87.5, KmCa = 1.38, KmK1 = 13.0, ksat = 0.2;
   const double INaKmax = 1.3 * 0.693, KmKo = 1.5, IpCamax = 0.6 * 0.05, KmpCa = 0.0005;
@@ -262,7 +283,7 @@ GRW_HD void fcn_currents(const double* state, double V, const FcnTrans& q, FcnCu
   c.IKv14_Na = (fabs(V) < 1.e-6) ? 0.02 * PKv14 * OK14 * 1000.0 * Faraday * (Nai - Nao)
                                  : 0.02 * PKv14 * OK14 * VFsq_over_RT * (a1 / a2);
   c.IKv14 = c.IKv14_K + c.IKv14_Na;
-  const double K1_inf = 1.0 / (0.94 + exp(1.76 / RT_over_F * (V - EK)));
+  const double K1_inf = 1.0 / (0.94 + k1x);
   c.IK1 = GK1 * Ko / (Ko + KmK1) * K1_inf * (V - EK);
   c.INab = GNab * (V - ENa);
   const double KpV = 1.0 / (1.0 + q.ex[1]);
@@ -286,25 +307,39 @@ GRW_HD void fcn_currents(const double* state, double V, const FcnTrans& q, FcnCu
   c.IpCa = IpCamax * Cai / (KmpCa + Cai);
 }
 
-// The two Ito1 lattices (fcn.c:860-998): F[11..30].
-GRW_HD void fcn_ito(const double* state, const FcnTrans& q, double* F) {
+// One of the two Ito1 lattices (fcn.c:860-998): Kv4.3 (kv14 = 0, F[11..20])
+// or Kv1.4 (kv14 = 1, F[21..30]).  Straight-line code with the lattice's
+// constants selected, so that two lanes evaluate the two lattices side by
+// side; the Kv1.4 inactivation rates have no exp, and a rate times 1.0 is
+// the rate.
+GRW_HD void fcn_ito_lattice(const double* state, const FcnTrans& q, int kv14, double* F) {
   const double f43[4] = {1.8936, 14.224647456, 158.574378389, 142.936645351};
   const double b43[4] = {6.77348, 15.6212705152, 28.7532603313, 524.576206679};
   const double f14[4] = {0.20005, 0.320280, 13.50909223, 1151.7651385};
   const double b14[4] = {2.2300, 12.000299, 5.3701338025, 5.2396395511};
-  double k43x[2][5], k14x[2][5];
-  ito_unpack(state, 11, k43x);
-  ito_unpack(state, 21, k14x);
+  const bool k = kv14 != 0;
+  double f[4], b[4];
+  for (int i = 0; i < 4; ++i) {
+    f[i] = k ? f14[i] : f43[i];
+    b[i] = k ? b14[i] : b43[i];
+  }
+  const int i0 = k ? 21 : 11;
+  const double aa = (k ? 6.0 * 0.31551 : 0.543708) * q.ex[k ? 31 : 27];
+  const double ba = (k ? 6.0 * 0.001966 : 0.080185) * q.ex[k ? 32 : 28];
+  const double ai = (k ? 6.0 * 0.0004938 : 0.0498424) * (k ? 1.0 : q.ex[29]);
+  const double bi = (k ? 6.0 * 0.0000176 : 0.000819482) * (k ? 1.0 : q.ex[30]);
+  double x[2][5], dx[2][5];
+  ito_unpack(state, i0, x);
   ItoRates iq;
-  double d43[2][5], d14[2][5];
-  ito_rates(0.543708 * q.ex[27], 0.080185 * q.ex[28], 0.0498424 * q.ex[29], 0.000819482 * q.ex[30], f43,
-            b43, iq);
-  ito_chain(k43x, iq, d43);
-  ito_rates(6.0 * 0.31551 * q.ex[31], 6.0 * 0.001966 * q.ex[32], 6.0 * 0.0004938, 6.0 * 0.0000176, f14, b14,
-            iq);
-  ito_chain(k14x, iq, d14);
-  ito_pack(d43, 11, F);
-  ito_pack(d14, 21, F);
+  ito_rates(aa, ba, ai, bi, f, b, iq);
+  ito_chain(x, iq, dx);
+  ito_pack(dx, i0, F);
+}
+
+// The two Ito1 lattices: F[11..30].
+GRW_HD void fcn_ito(const double* state, const FcnTrans& q, double* F) {
+  fcn_ito_lattice(state, q, 0, F);
+  fcn_ito_lattice(state, q, 1, F);
 }
 
 // The INa gates, the HERG chain and xKs (fcn.c:640-778): F[1..3], F[8],
@@ -398,8 +433,8 @@ GRW_HD void fcn_ca(const double* state, const GrwProtocol& p, const FcnTrans& q,
   const double dLTRPNCa = kltrpn_plus * Cai * (1.0 - LTRPNCa) - kltrpn_minus * LTRPNCa;
   const double dHTRPNCa = khtrpn_plus * Cai * (1.0 - HTRPNCa) - khtrpn_minus * HTRPNCa;
   ca.Jtrpn = LTRPNtot * dLTRPNCa + HTRPNtot * dHTRPNCa;
-  const double a1 = CMDNtot * KmCMDN / (pow(Cai + KmCMDN, 2.0));
-  const double a2 = EGTAtot * KmEGTA / (pow(Cai + KmEGTA, 2.0));
+  const double a1 = CMDNtot * KmCMDN / (pow2(Cai + KmCMDN));
+  const double a2 = EGTAtot * KmEGTA / (pow2(Cai + KmEGTA));
   ca.beta_i = 1.0 / (1.0 + a1 + a2);
   F[9] = dLTRPNCa;
   F[10] = dHTRPNCa;
@@ -456,7 +491,7 @@ GRW_HD void fcn_finish(double t, double* state, double* F, double* cur, bool kee
   double V, Istim, m_set;
   fcn_voltage(t, state, p, V, Istim);
   FcnCurrents c;
-  fcn_currents(state, V, q, c);
+  fcn_currents(state, V, q, exp(fcn_k1_arg(V, q.lg[1])), c);
   fcn_ito(state, q, F);
   fcn_gates(state, V, q, F, &m_set);
   FcnCa ca;
