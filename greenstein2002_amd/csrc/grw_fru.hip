@@ -226,6 +226,8 @@ constexpr int kSlot = 32;
 // or i ^ 12 (after the first three steps lanes i ^ 4 and i ^ 12 hold the same
 // value); k = 2, 1: quad permutes.  Every step adds the same two numbers as
 // the plain butterfly, so the result is the same in every lane, bit for bit.
+// (MAX: the larger of the two instead of the sum, for values that are not NaN.)
+template <bool MAX = false>
 __device__ __forceinline__ double swap_sum(double v, bool rows16) {
   const long long b = __double_as_longlong(v);
   const unsigned lo = (unsigned)b, hi = (unsigned)((unsigned long long)b >> 32);
@@ -241,15 +243,16 @@ __device__ __forceinline__ double swap_sum(double v, bool rows16) {
   }
   const double a = __longlong_as_double((long long)(((unsigned long long)ahi << 32) | alo));
   const double c = __longlong_as_double((long long)(((unsigned long long)chi << 32) | clo));
-  return a + c;
+  return MAX ? rmax(a, c) : a + c;
 }
 
-template <int CTRL>
+template <int CTRL, bool MAX = false>
 __device__ __forceinline__ double dpp_add(double v) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)b, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
   const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
-  return v + __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+  const double w = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+  return MAX ? rmax(v, w) : v + w;
 }
 
 // Sum and maximum of a 32-bit value over the wavefront, by the same lane
@@ -270,13 +273,14 @@ __device__ __forceinline__ unsigned wave_u32(unsigned v) {
   return MAX ? max(v, w) : v + w;
 }
 
+template <bool MAX = false>
 __device__ __forceinline__ double wave_sum_xor(double v) {
-  v = swap_sum(v, false);  // k = 32
-  v = swap_sum(v, true);   // k = 16
-  v = dpp_add<0x128>(v);   // row_ror:8, k = 8
-  v = dpp_add<0x124>(v);   // row_ror:4, k = 4
-  v = dpp_add<0x4E>(v);    // quad_perm [2,3,0,1], k = 2
-  v = dpp_add<0xB1>(v);    // quad_perm [1,0,3,2], k = 1
+  v = swap_sum<MAX>(v, false);  // k = 32
+  v = swap_sum<MAX>(v, true);   // k = 16
+  v = dpp_add<0x128, MAX>(v);   // row_ror:8, k = 8
+  v = dpp_add<0x124, MAX>(v);   // row_ror:4, k = 4
+  v = dpp_add<0x4E, MAX>(v);    // quad_perm [2,3,0,1], k = 2
+  v = dpp_add<0xB1, MAX>(v);    // quad_perm [1,0,3,2], k = 1
   return v;
 }
 
@@ -1365,15 +1369,20 @@ __device__ __forceinline__ void stage_bounds(int s, double start, double h, doub
 // the other): the exps on wavefront 0, which also evaluates fcn_voltage for
 // the groups; the logs on wavefront 1, then there the IK1 factor's exp, whose
 // argument needs EK; the pows on wavefront 2; wavefront 3 is free for the
-// caller's work.  Then the independent groups of fcn_finish: the membrane
-// currents (thread 0), the two Ito1 lattices side by side (threads 64 and 65),
-// the gates (thread 128) and the Ca handling (thread 192); thread 0 assembles
-// the derivatives.  The values are those of the one-thread fcn.  Call from
-// every thread; J and cur are read / written by thread 0 only, st is read by
-// all and written by thread 0.
+// caller's work.  Then the independent groups of fcn_finish, balanced over
+// the four wavefronts: the channel currents and xKs (thread 0), the two Ito1
+// lattices side by side (threads 64 and 65), the INa gates and the HERG chain
+// (thread 128), the Ca handling and the pumps (thread 192); thread 0
+// assembles the derivatives.  The values are those of the one-thread fcn.
+// Call from every thread; J and cur are read / written by thread 0 only, st is
+// read by all and written by thread 0.
 __device__ __forceinline__ void fcn_block(double t, double* st, double* F, double* cur, bool keepc, const double* J,
-                          const GrwProtocol& p, FcnTrans* q) {
+                          const GrwProtocol& prot, FcnTrans* q) {
   const int j = threadIdx.x;
+  // the protocol fields in registers from the start: read through the
+  // reference after the caller's stores to its control block, they would be
+  // loaded again behind every barrier
+  const GrwProtocol p = prot;
   if (j < 64) {
     double Vv, Is;
     fcn_voltage(t, st, p, Vv, Is);
@@ -1399,22 +1408,33 @@ __device__ __forceinline__ void fcn_block(double t, double* st, double* F, doubl
   __syncthreads();
   // fcn_finish with its four independent groups on the block's four wavefronts
   __shared__ FcnCa ca_sh;
-  __shared__ double mset_sh;
+  __shared__ double mset_sh, pump_sh[4];
   double V = 0.0, Istim = 0.0;
   FcnCurrents c;
   if (j == 0) {
     V = q->V;
     Istim = q->Istim;
     fcn_currents(st, V, *q, q->k1x, c);
+    fcn_xks(st, V, *q, F);
   } else if (j == 64 || j == 65) {
     fcn_ito_lattice(st, *q, j - 64, F);
   } else if (j == 128) {
     fcn_gates(st, q->V, *q, F, &mset_sh);
   } else if (j == 192) {
     fcn_ca(st, p, *q, F, ca_sh);
+    FcnCurrents cp;
+    fcn_pumps(st, q->V, *q, cp);
+    pump_sh[0] = cp.INaK;
+    pump_sh[1] = cp.INaCa;
+    pump_sh[2] = cp.ICab;
+    pump_sh[3] = cp.IpCa;
   }
   __syncthreads();
   if (j == 0) {
+    c.INaK = pump_sh[0];
+    c.INaCa = pump_sh[1];
+    c.ICab = pump_sh[2];
+    c.IpCa = pump_sh[3];
     if (mset_sh >= 0.0) st[1] = mset_sh;
     fcn_assemble(st, V, Istim, c, ca_sh, F, cur, keepc, J, p);
   }
@@ -1570,10 +1590,11 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   // the control block's state vectors for the tail, loaded by every block
   // before the reduction so that the last block does not wait for them after it
   const int j = threadIdx.x;
-  double st_j = 0.0, y_j = 0.0, k_j[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double st_j = 0.0, y_j = 0.0, k_j[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, errw_j = 0.0;
   if (j < kNglobal && s0 >= 1 && s0 <= 5) {
     st_j = ctl->state[j];
     y_j = ctl->y[j];
+    if (s0 == 5) errw_j = ctl->errw[j];
 #pragma unroll
     for (int q = 0; q < 5; ++q)
       if (q < s0) k_j[q] = ctl->k[q][j];
@@ -1585,7 +1606,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     return;
   const unsigned long long tail0 = wall_clock64();
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
-  __shared__ double st[kNglobal], yv[kNglobal], F[kNglobal], kk[6][kNglobal], ev[kNglobal];
+  __shared__ double st[kNglobal], yv[kNglobal], F[kNglobal], kk[6][kNglobal];
   __shared__ double Jsh[4];
   __shared__ FcnTrans fq;
   __shared__ int decision;
@@ -1650,22 +1671,32 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   }
   // ---- truncation error and step control (rk_pd54m.c:225-333)
   const double huge_number = 1e99;
+  __shared__ double tts_sh[2];
+  double evj = 0.0;
   if (j < kNglobal) {
     const double y4 = st[j] + 31.0 / 540.0 * kk[0][j] + 190.0 / 297.0 * kk[2][j] - 145.0 / 108.0 * kk[3][j] +
                       351.0 / 220.0 * kk[4][j] + 1.0 / 20.0 * kk[5][j];
     const double ym = st[j] + 19.0 / 216.0 * kk[0][j] + 1000.0 / 2079.0 * kk[2][j] -
                       125.0 / 216.0 * kk[3][j] + 81.0 / 88.0 * kk[4][j] + 5.0 / 56.0 * kk[5][j];
-    ev[j] = fabs((ym - y4) * 0.2 * ctl->errw[j]);
+    evj = fabs((ym - y4) * 0.2 * errw_j);
     yv[j] = y4;
+  }
+  // The error norm (rk_pd54m.c:225-237): the largest component, or
+  // huge_number once any component is not below it (NaN included), which is
+  // the sequential loop's result whatever the order; lanes past the state hold
+  // 0, which changes no maximum of non-negative values.
+  double err = 0.0;
+  if (j < 64) err = wave_sum_xor<true>((evj < huge_number) ? evj : huge_number);
+  // the stimulus alignment's remainders for either outcome (rk_pd54m.c:279,
+  // :317), on wavefront 1 beside the norm
+  if (j == 64) {
+    const GrwProtocol& p = ctl->prot;
+    tts_sh[0] = p.shift - fmod(start + h, p.period);
+    tts_sh[1] = p.shift - fmod(start, p.period);
   }
   __syncthreads();
   if (j == 0) {
     const GrwProtocol& p = ctl->prot;
-    double err = 0.0;
-    for (int q = 0; q < kNglobal; ++q) {
-      if (ev[q] < huge_number) err = rmax(ev[q], err);
-      else err = huge_number;
-    }
     const bool accept = err < p.tolrk;
     double hn = h;
     if (accept || h <= p.step_min) {
@@ -1679,7 +1710,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       } else if (accept) {
         hn = rmin(0.85 * h * pow(p.tolrk / err, 0.2), 4.0 * h);
         ctl->oh = hn;
-        double tts = p.shift - fmod(ns, p.period);
+        double tts = tts_sh[0];
         if (tts < 0.0) tts = tts + p.period;
         if (tts < hn) hn = rmax(p.step_min, tts - 0.5 * p.step_min);
       } else {
@@ -1694,7 +1725,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       decision = 0;
       ctl->rejects++;
       hn = 0.85 * h * pow(p.tolrk / err, 0.2);
-      double tts = p.shift - fmod(start, p.period);
+      double tts = tts_sh[1];
       if (tts < 0.0) tts = tts + p.period;
       if (tts < hn) hn = rmax(p.step_min, tts - 0.5 * p.step_min);
       ctl->success = 0;

@@ -23,16 +23,20 @@ struct ItoRates {
   double fw[2][5], bw[2][5], vx[2][5];
 };
 
-GRW_HD void ito_rates(double aa, double ba, double ai, double bi, const double f[4],
-                      const double b[4], ItoRates& q) {
+// The lattice constants f, b come with their reciprocals (DivK): every
+// division here is by one of them.
+GRW_HD void ito_rates(double aa, double ba, double ai, double bi, const DivK f[4], const DivK b[4],
+                      ItoRates& q) {
   q.fw[0][0] = 4.0 * aa; q.fw[0][1] = 3.0 * aa; q.fw[0][2] = 2.0 * aa; q.fw[0][3] = aa; q.fw[0][4] = 0;
-  q.fw[1][0] = 4.0 * b[0] * aa; q.fw[1][1] = 3.0 * b[1] * aa / b[0];
-  q.fw[1][2] = 2.0 * b[2] * aa / b[1]; q.fw[1][3] = b[3] * aa / b[2]; q.fw[1][4] = 0;
+  q.fw[1][0] = 4.0 * b[0].b * aa; q.fw[1][1] = rdiv(3.0 * b[1].b * aa, b[0]);
+  q.fw[1][2] = rdiv(2.0 * b[2].b * aa, b[1]); q.fw[1][3] = rdiv(b[3].b * aa, b[2]); q.fw[1][4] = 0;
   q.bw[0][0] = 0; q.bw[0][1] = ba; q.bw[0][2] = 2.0 * ba; q.bw[0][3] = 3.0 * ba; q.bw[0][4] = 4.0 * ba;
-  q.bw[1][0] = 0; q.bw[1][1] = ba / f[0]; q.bw[1][2] = 2.0 * f[0] * ba / f[1];
-  q.bw[1][3] = 3.0 * f[1] * ba / f[2]; q.bw[1][4] = 4.0 * f[2] * ba / f[3];
-  q.vx[0][0] = bi; q.vx[0][1] = f[0] * bi; q.vx[0][2] = f[1] * bi; q.vx[0][3] = f[2] * bi; q.vx[0][4] = f[3] * bi;
-  q.vx[1][0] = ai; q.vx[1][1] = ai / b[0]; q.vx[1][2] = ai / b[1]; q.vx[1][3] = ai / b[2]; q.vx[1][4] = ai / b[3];
+  q.bw[1][0] = 0; q.bw[1][1] = rdiv(ba, f[0]); q.bw[1][2] = rdiv(2.0 * f[0].b * ba, f[1]);
+  q.bw[1][3] = rdiv(3.0 * f[1].b * ba, f[2]); q.bw[1][4] = rdiv(4.0 * f[2].b * ba, f[3]);
+  q.vx[0][0] = bi; q.vx[0][1] = f[0].b * bi; q.vx[0][2] = f[1].b * bi; q.vx[0][3] = f[2].b * bi;
+  q.vx[0][4] = f[3].b * bi;
+  q.vx[1][0] = ai; q.vx[1][1] = rdiv(ai, b[0]); q.vx[1][2] = rdiv(ai, b[1]); q.vx[1][3] = rdiv(ai, b[2]);
+  q.vx[1][4] = rdiv(ai, b[3]);
 }
 
 // Derivatives of the lattice; summation order per state as in fcn.c:883-998.
@@ -243,26 +247,23 @@ GRW_HD double fcn_k1_arg(double V, double lg1) {
   return 1.76 / RT_over_F * (V - EK);
 }
 
-// The sarcolemmal currents before the CHF scaling (fcn.c:485-746); k1x is
-// exp(fcn_k1_arg(V, q.lg[1])).
+// The sarcolemmal channel currents before the CHF scaling (fcn.c:485-623);
+// k1x is exp(fcn_k1_arg(V, q.lg[1])).  The pumps and exchanger follow in
+// fcn_pumps.
 GRW_HD void fcn_currents(const double* state, double V, const FcnTrans& q, double k1x, FcnCurrents& c) {
   const double GKr = 0.029, GKs = 0.035, GK1 = 3.0, GKp = 1.2 * 0.002216, GNa = 12.8;
-  const double kNaCa = 0.9 * 0.30, KmNa = 87.5, KmCa = 1.38, KmK1 = 13.0, ksat = 0.2;
-  const double INaKmax = 1.3 * 0.693, KmKo = 1.5, IpCamax = 0.6 * 0.05, KmpCa = 0.0005;
-  const double GCab = 3.3 * 7.684e-5, GNab = 0.001;
+  const double KmK1 = 13.0;
+  const double GNab = 0.001;
   const double KvScale = 1.13 * 1.03 * 1.55, Kv43Frac = 0.77;
   const double GKv43 = Kv43Frac * KvScale * 0.1;
   const double PKv14 = (1.0 - Kv43Frac) * KvScale * 4.792933e-7;
   double mNa = state[1];
   const double hNa = state[2], jNa = state[3], Nai = state[4], Ki = state[5];
-  double Cai = state[6];
   const double xKs = state[8], OH = state[35];
 
   const double ENa = RT_over_F * q.lg[0];
   const double EK = RT_over_F * q.lg[1];
   const double EKs = RT_over_F * q.lg[2];
-  if (Cai <= 0.0) Cai = 1.e-15;
-  const double ECa = 0.5 * RT_over_F * q.lg[3];
 
   if (mNa <= 0.0) mNa = 1.e-15;
   c.INa = GNa * (mNa * mNa * mNa) * hNa * jNa * (V - ENa);
@@ -274,7 +275,7 @@ GRW_HD void fcn_currents(const double* state, double V, const FcnTrans& q, doubl
   const double VFsq_over_RT = (1000.0 * Faraday) * VF_over_RT;
   const double exp_VFRT = q.ex[0];
   const double OK14 = state[21 + 4];
-  double a1, a2, a3, a4, a5;
+  double a1, a2;
   a1 = Ki * exp_VFRT - Ko;
   a2 = exp_VFRT - 1.0;
   c.IKv14_K = (fabs(V) < 1.e-6) ? PKv14 * OK14 * 1000.0 * Faraday * (Ki - Ko)
@@ -288,7 +289,19 @@ GRW_HD void fcn_currents(const double* state, double V, const FcnTrans& q, doubl
   c.INab = GNab * (V - ENa);
   const double KpV = 1.0 / (1.0 + q.ex[1]);
   c.IKp = GKp * KpV * (V - EK);
+}
 
+// INaK, INaCa, ICab and IpCa before the CHF scaling (fcn.c:625-649).
+GRW_HD void fcn_pumps(const double* state, double V, const FcnTrans& q, FcnCurrents& c) {
+  const double kNaCa = 0.9 * 0.30, KmNa = 87.5, KmCa = 1.38, ksat = 0.2;
+  const double INaKmax = 1.3 * 0.693, KmKo = 1.5, IpCamax = 0.6 * 0.05, KmpCa = 0.0005;
+  const double GCab = 3.3 * 7.684e-5;
+  const double Nai = state[4];
+  double Cai = state[6];
+  if (Cai <= 0.0) Cai = 1.e-15;
+  const double ECa = 0.5 * RT_over_F * q.lg[3];
+  const double exp_VFRT = q.ex[0];
+  double a1, a2, a3, a4, a5;
   const double sigma = (exp(Nao / 67.3) - 1.0) / 7.0;
   a1 = 1.0 + 0.1245 * q.ex[2];
   a2 = 0.0365 * sigma / exp_VFRT;
@@ -313,12 +326,12 @@ GRW_HD void fcn_currents(const double* state, double V, const FcnTrans& q, doubl
 // side; the Kv1.4 inactivation rates have no exp, and a rate times 1.0 is
 // the rate.
 GRW_HD void fcn_ito_lattice(const double* state, const FcnTrans& q, int kv14, double* F) {
-  const double f43[4] = {1.8936, 14.224647456, 158.574378389, 142.936645351};
-  const double b43[4] = {6.77348, 15.6212705152, 28.7532603313, 524.576206679};
-  const double f14[4] = {0.20005, 0.320280, 13.50909223, 1151.7651385};
-  const double b14[4] = {2.2300, 12.000299, 5.3701338025, 5.2396395511};
+  constexpr DivK f43[4] = {divk(1.8936), divk(14.224647456), divk(158.574378389), divk(142.936645351)};
+  constexpr DivK b43[4] = {divk(6.77348), divk(15.6212705152), divk(28.7532603313), divk(524.576206679)};
+  constexpr DivK f14[4] = {divk(0.20005), divk(0.320280), divk(13.50909223), divk(1151.7651385)};
+  constexpr DivK b14[4] = {divk(2.2300), divk(12.000299), divk(5.3701338025), divk(5.2396395511)};
   const bool k = kv14 != 0;
-  double f[4], b[4];
+  DivK f[4], b[4];
   for (int i = 0; i < 4; ++i) {
     f[i] = k ? f14[i] : f43[i];
     b[i] = k ? b14[i] : b43[i];
@@ -342,14 +355,14 @@ GRW_HD void fcn_ito(const double* state, const FcnTrans& q, double* F) {
   fcn_ito_lattice(state, q, 1, F);
 }
 
-// The INa gates, the HERG chain and xKs (fcn.c:640-778): F[1..3], F[8],
-// F[32..36].  Below -90 mV the m gate is set to its steady state
-// (fcn.c:663-669): *m_set receives that value, else -1.
+// The INa gates and the HERG chain (fcn.c:651-743): F[1..3], F[32..36].
+// Below -90 mV the m gate is set to its steady state (fcn.c:663-669): *m_set
+// receives that value, else -1.  xKs follows in fcn_xks.
 GRW_HD void fcn_gates(const double* state, double V, const FcnTrans& q, double* F, double* m_set) {
   const double TH = 5.320000001;
   const double C2H_to_C3H = TH * 0.02608362043337, C3H_to_C2H = TH * 0.14832978132145;
   double mNa = state[1];
-  const double hNa = state[2], jNa = state[3], xKs = state[8];
+  const double hNa = state[2], jNa = state[3];
   const double C1H = state[32], C2H = state[33], C3H = state[34], OH = state[35];
   const double IH = 1.0 - OH - C1H - C2H - C3H;
   if (mNa <= 0.0) mNa = 1.e-15;
@@ -397,6 +410,15 @@ GRW_HD void fcn_gates(const double* state, double V, const FcnTrans& q, double* 
                       ((C3H_to_IH + C3H_to_OH + C3H_to_C2H) * C3H);
   const double dOH = (C3H_to_OH * C3H + IH_to_OH * IH) - ((OH_to_C3H + OH_to_IH) * OH);
 
+  F[1] = dmNa; F[2] = dhNa; F[3] = djNa;
+  F[32] = dC1H; F[33] = dC2H; F[34] = dC3H; F[35] = dOH;
+  F[36] = -dC1H - dC2H - dC3H - dOH;
+}
+
+// The IKs gate xKs (fcn.c:745-755): F[8].
+GRW_HD void fcn_xks(const double* state, double V, const FcnTrans& q, double* F) {
+  const double xKs = state[8];
+  double a1, a2;
   const double xKs_inf = 1.0 / (1.0 + q.ex[24]);
   if (fabs(V - 10) < 1.e-6) {
     a1 = (7.19e-5) / 0.148;
@@ -406,11 +428,7 @@ GRW_HD void fcn_gates(const double* state, double V, const FcnTrans& q, double* 
     a2 = 1.31e-4 * (V - 10.0) / (q.ex[26] - 1.0);
   }
   const double tau_xKs = 1.0 / (a1 + a2);
-  const double dxKs = (xKs_inf - xKs) / tau_xKs;
-
-  F[1] = dmNa; F[2] = dhNa; F[3] = djNa; F[8] = dxKs;
-  F[32] = dC1H; F[33] = dC2H; F[34] = dC3H; F[35] = dOH;
-  F[36] = -dC1H - dC2H - dC3H - dOH;
+  F[8] = (xKs_inf - xKs) / tau_xKs;
 }
 
 struct FcnCa {
@@ -492,8 +510,10 @@ GRW_HD void fcn_finish(double t, double* state, double* F, double* cur, bool kee
   fcn_voltage(t, state, p, V, Istim);
   FcnCurrents c;
   fcn_currents(state, V, q, exp(fcn_k1_arg(V, q.lg[1])), c);
+  fcn_pumps(state, V, q, c);
   fcn_ito(state, q, F);
   fcn_gates(state, V, q, F, &m_set);
+  fcn_xks(state, V, q, F);
   FcnCa ca;
   fcn_ca(state, p, q, F, ca);
   if (m_set >= 0.0) state[1] = m_set;

@@ -39,6 +39,11 @@ DIVISORS = {
     "-6.8": -6.8,
     "-11.1": -11.1,
     "13.6": 13.6,
+    # the Ito1 lattice constants (grw_global.h fcn_ito_lattice)
+    "f43_1": 1.8936, "f43_2": 14.224647456, "f43_3": 158.574378389, "f43_4": 142.936645351,
+    "b43_1": 6.77348, "b43_2": 15.6212705152, "b43_3": 28.7532603313, "b43_4": 524.576206679,
+    "f14_1": 0.20005, "f14_2": 0.320280, "f14_3": 13.50909223, "f14_4": 1151.7651385,
+    "b14_1": 2.2300, "b14_2": 12.000299, "b14_3": 5.3701338025, "b14_4": 5.2396395511,
 }
 
 
