@@ -284,9 +284,11 @@ __device__ __forceinline__ double wave_sum_xor(double v) {
   return v;
 }
 
+// blk < 0: a block without Monte Carlo work that only counts as an arrival
+// (extra: the number of those among the ticket's arrivals).
 template <int NV>
 __device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigned int* ticket, int blk, int b0,
-                             int nb, int slot_off, double* tot_sh) {
+                             int nb, int slot_off, double* tot_sh, int extra = 0) {
   static_assert(NV <= kSlot, "block partials overflow their slot row");
   __shared__ double wsum[4][NV];
   __shared__ bool last;
@@ -305,15 +307,16 @@ __device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigne
     // running on it; the last block reads the partials with agent-scope atomic
     // loads and needs no acquire fence either (nothing else it reads was
     // written in this launch).
-    for (int k = 0; k < NV; ++k) {
-      double s = wsum[0][k];
-      for (int w = 1; w < nw; ++w) s += wsum[w][k];
-      __hip_atomic_store(&d.part[(size_t)blk * kSlot + slot_off + k], s, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (blk >= 0)
+      for (int k = 0; k < NV; ++k) {
+        double s = wsum[0][k];
+        for (int w = 1; w < nw; ++w) s += wsum[w][k];
+        __hip_atomic_store(&d.part[(size_t)blk * kSlot + slot_off + k], s, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
     __builtin_amdgcn_s_waitcnt(0);
     const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == (unsigned)nb - 1u);
+    last = (t == (unsigned)(nb + extra) - 1u);
   }
   __syncthreads();
   if (!last) return false;
@@ -1251,6 +1254,11 @@ __device__ __forceinline__ void fru_rollback(const SimArgs& a, const Dev& d, int
 //   flux (mM/ms).  Under voltage clamp the beats are the clamp periods
 //   (vclamp_shift, vclamp_period).
 constexpr int kBeatStats = 15;
+// The part of a stage's fcn that does not depend on the release units'
+// fluxes, as the stage's fcn block leaves it for the tail: F[0..36] (the
+// assembled components excepted), the 14 currents of FcnCurrents, the three
+// of FcnCa, V and Istim.
+constexpr int kPreCur = 37, kPreCa = 51, kPreV = 54, kPreIstim = 55, kPre = 56;
 
 struct RkCtl {
   GrwProtocol prot;
@@ -1277,6 +1285,7 @@ struct RkCtl {
   double bt[kBeatStats];
   double prev_t, prev_V, prev_jin, prev_jrel;
   double pend_t, pend_V, pend_Cai, pend_NSR, pend_JSR;  // the last output sample
+  double pre[kPre];  // this stage's fcn without the assembly, from the launch's fcn block
 };
 
 __device__ __forceinline__ void beat_flush(RkCtl* c) {
@@ -1376,8 +1385,13 @@ __device__ __forceinline__ void stage_bounds(int s, double start, double h, doub
 // assembles the derivatives.  The values are those of the one-thread fcn.
 // Call from every thread; J and cur are read / written by thread 0 only, st is
 // read by all and written by thread 0.
-__device__ __forceinline__ void fcn_block(double t, double* st, double* F, double* cur, bool keepc, const double* J,
-                          const GrwProtocol& prot, FcnTrans* q) {
+// fcn_block without the assembly, which alone needs the release units' fluxes:
+// F except the assembled components, and in thread 0 the currents c, the Ca
+// terms ca, the membrane potential V and the stimulus Istim; *mset as
+// fcn_gates leaves it.  Ends behind a barrier.
+__device__ __forceinline__ void fcn_block_groups(double t, const double* st, double* F, const GrwProtocol& prot,
+                                                 FcnTrans* q, FcnCurrents& c, FcnCa& ca, double& V,
+                                                 double& Istim, double* mset) {
   const int j = threadIdx.x;
   // the protocol fields in registers from the start: read through the
   // reference after the caller's stores to its control block, they would be
@@ -1408,9 +1422,9 @@ __device__ __forceinline__ void fcn_block(double t, double* st, double* F, doubl
   __syncthreads();
   // fcn_finish with its four independent groups on the block's four wavefronts
   __shared__ FcnCa ca_sh;
-  __shared__ double mset_sh, pump_sh[4];
-  double V = 0.0, Istim = 0.0;
-  FcnCurrents c;
+  __shared__ double pump_sh[4];
+  V = 0.0;
+  Istim = 0.0;
   if (j == 0) {
     V = q->V;
     Istim = q->Istim;
@@ -1419,7 +1433,7 @@ __device__ __forceinline__ void fcn_block(double t, double* st, double* F, doubl
   } else if (j == 64 || j == 65) {
     fcn_ito_lattice(st, *q, j - 64, F);
   } else if (j == 128) {
-    fcn_gates(st, q->V, *q, F, &mset_sh);
+    fcn_gates(st, q->V, *q, F, mset);
   } else if (j == 192) {
     fcn_ca(st, p, *q, F, ca_sh);
     FcnCurrents cp;
@@ -1435,8 +1449,20 @@ __device__ __forceinline__ void fcn_block(double t, double* st, double* F, doubl
     c.INaCa = pump_sh[1];
     c.ICab = pump_sh[2];
     c.IpCa = pump_sh[3];
+    ca = ca_sh;
+  }
+}
+
+__device__ __forceinline__ void fcn_block(double t, double* st, double* F, double* cur, bool keepc, const double* J,
+                          const GrwProtocol& prot, FcnTrans* q) {
+  __shared__ double mset_sh;
+  FcnCurrents c;
+  FcnCa ca;
+  double V, Istim;
+  fcn_block_groups(t, st, F, prot, q, c, ca, V, Istim, &mset_sh);
+  if (threadIdx.x == 0) {
     if (mset_sh >= 0.0) st[1] = mset_sh;
-    fcn_assemble(st, V, Istim, c, ca_sh, F, cur, keepc, J, p);
+    fcn_assemble(st, V, Istim, c, ca, F, cur, keepc, J, prot);
   }
   __syncthreads();
 }
@@ -1518,8 +1544,9 @@ __device__ __forceinline__ void attempt_start(RkCtl* ctl, const CellInfo& ci, co
 // costliest first (k_order_blocks), so that the long ones do not start last
 // and leave the launch waiting on them; a block's FRUs, its partial-sum slot
 // and so every result are those of the block it runs, whatever the order.
-__device__ __forceinline__ int mc_block(const SimArgs& base, const Dev& d) {
-  return base.sorted ? d.border[blockIdx.x] : (int)blockIdx.x;
+__device__ __forceinline__ int mc_block(const SimArgs& base, const Dev& d, int first) {
+  const int i = (int)blockIdx.x - first;
+  return base.sorted ? d.border[i] : i;
 }
 
 // Attempt start of every cell at the beginning of a run (later attempts start
@@ -1530,7 +1557,7 @@ __device__ __forceinline__ int mc_block(const SimArgs& base, const Dev& d) {
 // unbroken run; then attempt_start in the cell's last block.
 template <class R>
 __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev d, RkCtl* ctls) {
-  const int blk = mc_block(base, d);
+  const int blk = mc_block(base, d, 0);
   const int cell = base.ncells == 1 ? 0 : d.smap[blk];  // (one cell: no dependent load)
   const CellInfo ci = d.cells[cell];
   RkCtl* ctl = ctls + cell;
@@ -1565,8 +1592,11 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
 // output loop (caf.c:150-251).
 template <class R>
 __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctls) {
-  const int blk = mc_block(base, d);
-  const int cell = base.ncells == 1 ? 0 : d.smap[blk];  // (one cell: no dependent load)
+  // blocks [0, ncells): each cell's fcn block, dispatched first; then the
+  // Monte Carlo blocks
+  const bool fblock = (int)blockIdx.x < base.ncells;
+  const int blk = fblock ? -1 : mc_block(base, d, base.ncells);
+  const int cell = fblock ? (int)blockIdx.x : base.ncells == 1 ? 0 : d.smap[blk];  // (one cell: no dependent load)
   const CellInfo ci = d.cells[cell];
   RkCtl* ctl = ctls + cell;
   if (!ctl->active) return;
@@ -1586,10 +1616,44 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   a.pro = (s0 == 1) ? ctl->pro : 0;
   a.avg = (s0 == 5);  // the attempt's end state: averages for an output sample
   double acc[17] = {};
-  count_steps(d, simfru4_body<R>(a, d, acc, blk));
+  const int j = threadIdx.x;
+  __shared__ double st[kNglobal], yv[kNglobal], F[kNglobal], kk[6][kNglobal];
+  __shared__ FcnTrans fq;
+  if (fblock) {
+    // This stage's fcn at (t1, y) but for the assembly, the one part that
+    // needs the fluxes of this launch's sub-interval (the stage's y is known
+    // before it): computed beside the Monte Carlo blocks and left in the
+    // control block for the tail, with agent-scope stores completed before
+    // this block's arrival at the ticket.  (fcn_gates' reset of the m gate
+    // below -90 mV applies to the stage's y, which nothing reads afterwards.)
+    if (s0 >= 1 && s0 <= 5) {
+      if (j < kNglobal) yv[j] = ctl->y[j];
+      __syncthreads();
+      __shared__ double mset_sh;
+      FcnCurrents c;
+      FcnCa ca;
+      double V, Istim;
+      fcn_block_groups(a.t1, yv, F, ctl->prot, &fq, c, ca, V, Istim, &mset_sh);
+      double* pre = ctl->pre;
+      if (j < kNglobal) __hip_atomic_store(pre + j, F[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (j == 0) {
+        const double cv[14] = {c.INa, c.IKr, c.IKs, c.IKv43, c.IKv14, c.IKv14_K, c.IKv14_Na,
+                               c.IK1, c.INab, c.IKp, c.INaK, c.INaCa, c.ICab, c.IpCa};
+        for (int q = 0; q < 14; ++q)
+          __hip_atomic_store(pre + kPreCur + q, cv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(pre + kPreCa, ca.Jup, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(pre + kPreCa + 1, ca.Jtrpn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(pre + kPreCa + 2, ca.beta_i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(pre + kPreV, V, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(pre + kPreIstim, Istim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+    }
+  } else {
+    count_steps(d, simfru4_body<R>(a, d, acc, blk));
+  }
   // the control block's state vectors for the tail, loaded by every block
   // before the reduction so that the last block does not wait for them after it
-  const int j = threadIdx.x;
   double st_j = 0.0, y_j = 0.0, k_j[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, errw_j = 0.0;
   if (j < kNglobal && s0 >= 1 && s0 <= 5) {
     st_j = ctl->state[j];
@@ -1600,15 +1664,14 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       if (q < s0) k_j[q] = ctl->k[q][j];
   }
   __shared__ double tot[17];
-  // the eleven average sums (tot[6..16]) exist only in the attempt's last stage
-  if (a.avg ? !reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot)
-            : !reduce_range<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot))
+  // the eleven average sums (tot[6..16]) exist only in the attempt's last stage;
+  // the fcn block is one more arrival
+  if (a.avg ? !reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1)
+            : !reduce_range<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1))
     return;
   const unsigned long long tail0 = wall_clock64();
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
-  __shared__ double st[kNglobal], yv[kNglobal], F[kNglobal], kk[6][kNglobal];
-  __shared__ double Jsh[4];
-  __shared__ FcnTrans fq;
+  __shared__ double Jsh[4], pre[kPre];
   __shared__ int decision;
   const int s = s0;
   if (s < 1 || s > 5) {  // launches out of sequence: stop the run
@@ -1626,17 +1689,29 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     for (int q = 0; q < 5; ++q)
       if (q < s) kk[q][j] = k_j[q];
   }
-  __syncthreads();
-  // the flux conversion (mpi_master.c:401-468) and the launch's step statistics
-  // (a returning atomic) on wavefront 3, beside fcn_block's transcendentals on
-  // wavefronts 0-2: fcn_block reads J only after its first barrier
-  if (j == 192) {
-    if (base.ncells == 1) close_launch_stats(d);
-    flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], ci.nfru, ci.scale, Jsh);
+  // the fcn block's part of this stage's fcn (agent-scope loads: written in
+  // this launch), the flux conversion (mpi_master.c:401-468) on wavefront 3
+  // and the launch's step statistics (a returning atomic) on wavefront 2
+  if (j < kPre) {
+    const double v = __hip_atomic_load(ctl->pre + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (j < kNglobal) F[j] = v;
+    else pre[j] = v;
   }
-  fcn_block(a.t1, yv, F, nullptr, false, Jsh, ctl->prot, &fq);
-  if (j == 0)
+  if (j == 128 && base.ncells == 1) close_launch_stats(d);
+  if (j == 192) flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], ci.nfru, ci.scale, Jsh);
+  __syncthreads();
+  if (j == 0) {  // fcn's assembly with this launch's fluxes
+    FcnCurrents c;
+    c.INa = pre[kPreCur]; c.IKr = pre[kPreCur + 1]; c.IKs = pre[kPreCur + 2]; c.IKv43 = pre[kPreCur + 3];
+    c.IKv14 = pre[kPreCur + 4]; c.IKv14_K = pre[kPreCur + 5]; c.IKv14_Na = pre[kPreCur + 6];
+    c.IK1 = pre[kPreCur + 7]; c.INab = pre[kPreCur + 8]; c.IKp = pre[kPreCur + 9];
+    c.INaK = pre[kPreCur + 10]; c.INaCa = pre[kPreCur + 11]; c.ICab = pre[kPreCur + 12];
+    c.IpCa = pre[kPreCur + 13];
+    const FcnCa ca{pre[kPreCa], pre[kPreCa + 1], pre[kPreCa + 2]};
+    fcn_assemble(yv, pre[kPreV], pre[kPreIstim], c, ca, F, nullptr, false, Jsh, ctl->prot);
     for (int q = 0; q < 4; ++q) ctl->J[q] = Jsh[q];
+  }
+  __syncthreads();
   if (j < kNglobal) {
     kk[s][j] = h * F[j];
     ctl->k[s][j] = kk[s][j];
@@ -2376,9 +2451,9 @@ int run_device(grw_ctx* c) {
       for (int s5 = 0; s5 < 5; ++s5) {
         if (tk) (void)hipEventRecord(ek[2 * s5], c->stream);
         if (mixed)
-          hipLaunchKernelGGL(k_dev_stage<float>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+          hipLaunchKernelGGL(k_dev_stage<float>, dim3(nc + c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
         else
-          hipLaunchKernelGGL(k_dev_stage<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+          hipLaunchKernelGGL(k_dev_stage<double>, dim3(nc + c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
         if (tk) (void)hipEventRecord(ek[2 * s5 + 1], c->stream);
       }
       if (base.sorted) sort_and_order(c);
