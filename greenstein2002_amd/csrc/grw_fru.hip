@@ -1257,8 +1257,9 @@ constexpr int kBeatStats = 15;
 // The part of a stage's fcn that does not depend on the release units'
 // fluxes, as the stage's fcn block leaves it for the tail: F[0..36] (the
 // assembled components excepted), the 14 currents of FcnCurrents, the three
-// of FcnCa, V and Istim.
-constexpr int kPreCur = 37, kPreCa = 51, kPreV = 54, kPreIstim = 55, kPre = 56;
+// of FcnCa, V and Istim; and the flux conversion's exp at the stage's end
+// potential.
+constexpr int kPreCur = 37, kPreCa = 51, kPreV = 54, kPreIstim = 55, kPreE1 = 56, kPre = 57;
 
 struct RkCtl {
   GrwProtocol prot;
@@ -1288,7 +1289,52 @@ struct RkCtl {
   double pre[kPre];  // this stage's fcn without the assembly, from the launch's fcn block
 };
 
-__device__ __forceinline__ void beat_flush(RkCtl* c) {
+// The beat-tracking fields of a control block in registers, for beat_update:
+// loaded together instead of one dependent load after another (beat_load),
+// the changed ones written back afterwards (beat_store).
+struct BeatLocal {
+  double* out_beats;
+  double shift, period;  // the beats' origin and length (pacing or clamp)
+  int beat_cap, beat_n, beat_cur;
+  double bt[kBeatStats];
+  double prev_t, prev_V, prev_jin, prev_jrel;
+  double pend_t, pend_V, pend_Cai, pend_NSR, pend_JSR;
+};
+
+__device__ __forceinline__ BeatLocal beat_load(const RkCtl* c) {
+  BeatLocal b;
+  b.out_beats = c->out_beats;
+  const bool vc = c->prot.vclamp != 0;
+  b.shift = vc ? c->prot.vclamp_shift : c->prot.shift;
+  b.period = vc ? c->prot.vclamp_period : c->prot.period;
+  b.beat_cap = c->beat_cap;
+  b.beat_n = c->beat_n;
+  b.beat_cur = c->beat_cur;
+  for (int q = 0; q < kBeatStats; ++q) b.bt[q] = c->bt[q];
+  b.prev_t = c->prev_t;
+  b.prev_V = c->prev_V;
+  b.prev_jin = c->prev_jin;
+  b.prev_jrel = c->prev_jrel;
+  b.pend_t = c->pend_t;
+  b.pend_V = c->pend_V;
+  b.pend_Cai = c->pend_Cai;
+  b.pend_NSR = c->pend_NSR;
+  b.pend_JSR = c->pend_JSR;
+  return b;
+}
+
+__device__ __forceinline__ void beat_store(RkCtl* c, const BeatLocal& b) {
+  c->beat_n = b.beat_n;
+  c->beat_cur = b.beat_cur;
+  for (int q = 0; q < kBeatStats; ++q) c->bt[q] = b.bt[q];
+  c->prev_t = b.prev_t;
+  c->prev_V = b.prev_V;
+  c->prev_jin = b.prev_jin;
+  c->prev_jrel = b.prev_jrel;
+}
+
+template <class B>
+__device__ __forceinline__ void beat_flush(B* c) {
   if (c->beat_cur >= 0 && c->beat_n < c->beat_cap) {
     for (int q = 0; q < kBeatStats; ++q) c->out_beats[(size_t)c->beat_n * kBeatStats + q] = c->bt[q];
     c->beat_n++;
@@ -1298,7 +1344,8 @@ __device__ __forceinline__ void beat_flush(RkCtl* c) {
 
 // The output sample at the beat's stimulus time supplies its rest values; it
 // may be taken before or after the beat's first accepted step.
-__device__ __forceinline__ void beat_take_sample(RkCtl* c) {
+template <class B>
+__device__ __forceinline__ void beat_take_sample(B* c) {
   if (c->beat_cur >= 0 && c->pend_t == c->bt[0]) {
     c->bt[1] = c->pend_V;
     c->bt[7] = c->pend_Cai;
@@ -1307,11 +1354,11 @@ __device__ __forceinline__ void beat_take_sample(RkCtl* c) {
   }
 }
 
-__device__ __forceinline__ void beat_update(RkCtl* c, double t, double V, double Cai, double ICa, double jin, double jrel) {
+__device__ __forceinline__ void beat_update(BeatLocal* c, double t, double V, double Cai, double ICa, double jin,
+                                            double jrel) {
   if (!c->out_beats) return;
-  const GrwProtocol& p = c->prot;
-  const double shift = p.vclamp ? p.vclamp_shift : p.shift;
-  const double period = p.vclamp ? p.vclamp_period : p.period;
+  const double shift = c->shift;
+  const double period = c->period;
   const int b = (t >= shift) ? (int)floor((t - shift) / period) : -1;
   const bool seg = c->prev_t == c->prev_t;  // a previous accepted step exists
   const double din = seg ? 0.5 * (c->prev_jin + jin) * (t - c->prev_t) : 0.0;
@@ -1363,12 +1410,15 @@ __device__ __forceinline__ void beat_update(RkCtl* c, double t, double V, double
 
 // Stage s (1..5) sub-interval bounds, rk_pd54m.c:124,145-146,166-167,187-188,208-209.
 __device__ __forceinline__ void stage_bounds(int s, double start, double h, double& ta, double& tb) {
+  // (divisions by the constants through their reciprocals: rdiv(x, divk(c)))
+  const double h5 = rdiv(h, divk(5.0)), h310 = rdiv(h * 3.0, divk(10.0));
+  const double h35 = rdiv(h * 3.0, divk(5.0)), h23 = rdiv(h * 2.0, divk(3.0));
   switch (s) {
-    case 1: ta = start; tb = start + h / 5.0; break;
-    case 2: ta = start + h / 5.0; tb = start + h * 3.0 / 10.0; break;
-    case 3: ta = start + h * 3.0 / 10.0; tb = start + h * 3.0 / 5.0; break;
-    case 4: ta = start + h * 3.0 / 5.0; tb = start + h * 2.0 / 3.0; break;
-    default: ta = start + h * 2.0 / 3.0; tb = start + h; break;
+    case 1: ta = start; tb = start + h5; break;
+    case 2: ta = start + h5; tb = start + h310; break;
+    case 3: ta = start + h310; tb = start + h35; break;
+    case 4: ta = start + h35; tb = start + h23; break;
+    default: ta = start + h23; tb = start + h; break;
   }
 }
 
@@ -1526,7 +1576,7 @@ __device__ __forceinline__ void attempt_start(RkCtl* ctl, const CellInfo& ci, co
     }
     const double k0 = h * F1[j];
     ctl->k[0][j] = k0;
-    ctl->y[j] = st[j] + k0 / 5.0;
+    ctl->y[j] = st[j] + rdiv(k0, divk(5.0));
   }
   __syncthreads();
   if (j == 0) {
@@ -1647,6 +1697,8 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
         __hip_atomic_store(pre + kPreV, V, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(pre + kPreIstim, Istim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+      if (j == 64)
+        __hip_atomic_store(pre + kPreE1, flux_exp(a.depf[0]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __builtin_amdgcn_s_waitcnt(0);
     }
   } else {
@@ -1692,13 +1744,16 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   // the fcn block's part of this stage's fcn (agent-scope loads: written in
   // this launch), the flux conversion (mpi_master.c:401-468) on wavefront 3
   // and the launch's step statistics (a returning atomic) on wavefront 2
-  if (j < kPre) {
+  if (j < kPreE1) {
     const double v = __hip_atomic_load(ctl->pre + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (j < kNglobal) F[j] = v;
     else pre[j] = v;
   }
   if (j == 128 && base.ncells == 1) close_launch_stats(d);
-  if (j == 192) flux_currents(tot, a.depf[0], a.depf[1], a.depf[2], ci.nfru, ci.scale, Jsh);
+  if (j == 192) {
+    const double e1 = __hip_atomic_load(ctl->pre + kPreE1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    flux_currents_e(tot, a.depf[0], e1, a.depf[1], a.depf[2], ci.nfru, ci.scale, Jsh);
+  }
   __syncthreads();
   if (j == 0) {  // fcn's assembly with this launch's fluxes
     FcnCurrents c;
@@ -1745,6 +1800,17 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     return;
   }
   // ---- truncation error and step control (rk_pd54m.c:225-333)
+  // (thread 0 loads the beat tracking and the step log's position now, for
+  // the accepted step's bookkeeping below)
+  BeatLocal bl;
+  int log_n = 0, log_cap = 0;
+  double* out_log = nullptr;
+  if (j == 0) {
+    bl = beat_load(ctl);
+    log_n = ctl->log_n;
+    log_cap = ctl->log_cap;
+    out_log = ctl->out_log;
+  }
   const double huge_number = 1e99;
   __shared__ double tts_sh[2];
   double evj = 0.0;
@@ -1821,16 +1887,18 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   }
   __syncthreads();
   if (j == 0) {
-    if (ctl->log_n < ctl->log_cap) {
-      double* L = ctl->out_log + 4 * (size_t)ctl->log_n;
-      L[0] = ctl->start; L[1] = yv[0]; L[2] = yv[6]; L[3] = ctl->J[2];
-      ctl->log_n++;
+    const double tnew = start + h;  // the accepted step's end, ctl->start now
+    if (log_n < log_cap) {
+      double* L = out_log + 4 * (size_t)log_n;
+      L[0] = tnew; L[1] = yv[0]; L[2] = yv[6]; L[3] = Jsh[2];
+      ctl->log_n = log_n + 1;
     }
     // Ca entering through the LCCs and released by the RyRs, both per litre of
     // myoplasm (fcn.c:1058 and the scaling of send_calc_fru_avg, mpi_master.c:178)
-    const double jin = ctl->J[2] * Acap / (-2.0 * Vmyo * Faraday * 1000.0);
-    const double jrel = tot[5] * ci.scale * VSS / Vmyo;
-    beat_update(ctl, ctl->start, yv[0], yv[6], ctl->J[2], jin, jrel);
+    const double jin = rdiv(Jsh[2] * Acap, divk(-2.0 * Vmyo * Faraday * 1000.0));
+    const double jrel = rdiv(tot[5] * ci.scale * VSS, divk(Vmyo));
+    beat_update(&bl, tnew, yv[0], yv[6], Jsh[2], jin, jrel);
+    beat_store(ctl, bl);
   }
   if (decision == 2) {
   // ---- the caf.c loop: record this call's outputs, set up the next call

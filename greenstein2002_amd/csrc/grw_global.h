@@ -478,8 +478,8 @@ GRW_HD void fcn_assemble(double* state, double V, double Istim, FcnCurrents c, F
   a3 = c.IKr + c.IKs + c.IK1 + c.IKp;
   const double dKi = -(a3 - 2.0 * c.INaK + (Ito1 - c.IKv14_Na) + Istim) * a1;
   a3 = c.ICab - 2.0 * c.INaCa + c.IpCa;
-  const double dCai = ca.beta_i * (Jxfer * VSS / Vmyo - ca.Jup - ca.Jtrpn - a3 * 0.5 * a1);
-  const double dCaNSR = ca.Jup * Vmyo / VNSR - Jtr * VJSR / VNSR;
+  const double dCai = ca.beta_i * (rdiv(Jxfer * VSS, divk(Vmyo)) - ca.Jup - ca.Jtrpn - a3 * 0.5 * a1);
+  const double dCaNSR = rdiv(ca.Jup * Vmyo, divk(VNSR)) - rdiv(Jtr * VJSR, divk(VNSR));
   const double dCaTOT = -(a3 + ICa) * 0.5 * a1 * Vmyo * 1.0e6;
   a1 = c.INa + ICa + c.IKr + c.IKs;
   a2 = c.IK1 + c.IKp + c.INaCa + c.INaK + Ito1 + Ito2;
@@ -497,8 +497,9 @@ GRW_HD void fcn_assemble(double* state, double V, double Istim, FcnCurrents c, F
   if (keepc && cur) {  // :1046-1071
     cur[0] = c.INa; cur[1] = c.IKr; cur[2] = c.IKs; cur[3] = Ito1; cur[4] = c.IK1; cur[5] = c.IKp;
     cur[6] = c.INaCa; cur[7] = c.INaK; cur[8] = c.IpCa; cur[9] = c.ICab; cur[10] = c.INab; cur[11] = ICa;
-    cur[12] = ICa * Acap / (-2.0 * Vmyo * Faraday * 1000.0);
-    cur[13] = ca.Jup; cur[14] = ca.Jtrpn; cur[15] = Jtr * VJSR / Vmyo; cur[16] = Jxfer * VSS / Vmyo;
+    cur[12] = rdiv(ICa * Acap, divk(-2.0 * Vmyo * Faraday * 1000.0));
+    cur[13] = ca.Jup; cur[14] = ca.Jtrpn; cur[15] = rdiv(Jtr * VJSR, divk(Vmyo));
+    cur[16] = rdiv(Jxfer * VSS, divk(Vmyo));
     cur[17] = c.IKv43; cur[18] = c.IKv14; cur[19] = c.IKv14_K; cur[20] = c.IKv14_Na; cur[21] = Ito2;
     cur[22] = Istim; cur[23] = Itot;
   }

@@ -299,32 +299,39 @@ GRW_HD void fcn_fru(const R conc[5], R V, R Cai, R CaNSR, const bool lopen[4], c
 
 // ---- reduced sums -> whole-cell fluxes (mpi_master.c:55-112); num = {sum CaSS,
 // sum CaJSR, sum CaSS over open LCCs, open LCC count, open Ito2 count}
-GRW_HD void flux_currents(const double num[5], double V, double Cai, double CaNSR, int nfru,
-                          double scale, double out4[4]) {
-  const double VF_over_RT = V / RT_over_F;
+// e1 = exp(V F/RT) (flux_exp), which a caller can evaluate before the sums exist.
+GRW_HD double flux_exp(double V) { return exp(rdiv(V, divk(RT_over_F))); }
+
+GRW_HD void flux_currents_e(const double num[5], double V, double e1, double Cai, double CaNSR, int nfru,
+                            double scale, double out4[4]) {
+  const double VF_over_RT = rdiv(V, divk(RT_over_F));
   const double VFsq_over_RT = (1000.0 * Faraday) * VF_over_RT;
-  const double e1 = exp(VF_over_RT);
   const double e2 = e1 * e1;
   double ICa, Ito2;
   if (fabs(V) < 1.e-6) {
     const double icanum = num[2] - num[3] * Cao * 0.341;
     ICa = PCa * 2.0 * 1000.0 * Faraday * icanum;
-    ICa = ICa / Acap;
+    ICa = rdiv(ICa, divk(Acap));
     Ito2 = ((double)num[4]) * PCl * 1000.0 * Faraday * (Clo - Cli);
-    Ito2 = Ito2 / Acap;
+    Ito2 = rdiv(Ito2, divk(Acap));
   } else {
     const double icanum = num[2] * e2 - num[3] * Cao * 0.341;
     ICa = PCa * 4.0 * VFsq_over_RT * icanum / (e2 - 1.0);
-    ICa = ICa / Acap;
+    ICa = rdiv(ICa, divk(Acap));
     Ito2 = ((double)num[4]) * PCl * VFsq_over_RT * (Cli - Clo * e1) / (1.0 - e1);
-    Ito2 = Ito2 / Acap;
+    Ito2 = rdiv(Ito2, divk(Acap));
   }
-  const double Jtr = (((double)nfru) * CaNSR - num[1]) / tautr;
-  const double Jxfer = (num[0] - ((double)nfru * (double)4) * Cai) / tauxfer;
+  const double Jtr = rdiv(((double)nfru) * CaNSR - num[1], divk(tautr));
+  const double Jxfer = rdiv(num[0] - ((double)nfru * (double)4) * Cai, divk(tauxfer));
   out4[0] = scale * Jxfer;
   out4[1] = scale * Jtr;
   out4[2] = scale * ICa;
   out4[3] = scale * Ito2;
+}
+
+GRW_HD void flux_currents(const double num[5], double V, double Cai, double CaNSR, int nfru,
+                          double scale, double out4[4]) {
+  flux_currents_e(num, V, flux_exp(V), Cai, CaNSR, nfru, scale, out4);
 }
 
 // ---- average outputs (mpi_master.c:175-190); labels of the last two outputs

@@ -39,6 +39,12 @@ DIVISORS = {
     "-6.8": -6.8,
     "-11.1": -11.1,
     "13.6": 13.6,
+    # volumes and the membrane area (fcn_assemble, flux_currents_e), the stage bounds
+    "Vmyo": 25.84e-6,
+    "VNSR": 0.53 * 2.10e-6,
+    "Acap": 1.534e-4,
+    "-2 Vmyo F 1000": -2.0 * 25.84e-6 * 96.5 * 1000.0,
+    "10": 10.0,
     # the Ito1 lattice constants (grw_global.h fcn_ito_lattice)
     "f43_1": 1.8936, "f43_2": 14.224647456, "f43_3": 158.574378389, "f43_4": 142.936645351,
     "b43_1": 6.77348, "b43_2": 15.6212705152, "b43_3": 28.7532603313, "b43_4": 524.576206679,
