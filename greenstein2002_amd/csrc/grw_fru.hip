@@ -1509,10 +1509,11 @@ __device__ __forceinline__ void fcn_block(double t, double* st, double* F, doubl
   FcnCurrents c;
   FcnCa ca;
   double V, Istim;
-  fcn_block_groups(t, st, F, prot, q, c, ca, V, Istim, &mset_sh);
+  const GrwProtocol p = prot;  // (loaded before the barriers)
+  fcn_block_groups(t, st, F, p, q, c, ca, V, Istim, &mset_sh);
   if (threadIdx.x == 0) {
     if (mset_sh >= 0.0) st[1] = mset_sh;
-    fcn_assemble(st, V, Istim, c, ca, F, cur, keepc, J, prot);
+    fcn_assemble(st, V, Istim, c, ca, F, cur, keepc, J, p);
   }
   __syncthreads();
 }
@@ -1734,6 +1735,10 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     return;
   }
   const double h = ctl->h, start = ctl->start;
+  // the protocol and the output step's end in registers before any barrier:
+  // read through ctl after this tail's stores, they would be loaded again
+  const GrwProtocol pl = ctl->prot;
+  const double tstep = ctl->tstep;
   if (j < kNglobal) {
     st[j] = st_j;
     yv[j] = y_j;
@@ -1763,7 +1768,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     c.INaK = pre[kPreCur + 10]; c.INaCa = pre[kPreCur + 11]; c.ICab = pre[kPreCur + 12];
     c.IpCa = pre[kPreCur + 13];
     const FcnCa ca{pre[kPreCa], pre[kPreCa + 1], pre[kPreCa + 2]};
-    fcn_assemble(yv, pre[kPreV], pre[kPreIstim], c, ca, F, nullptr, false, Jsh, ctl->prot);
+    fcn_assemble(yv, pre[kPreV], pre[kPreIstim], c, ca, F, nullptr, false, Jsh, pl);
     for (int q = 0; q < 4; ++q) ctl->J[q] = Jsh[q];
   }
   __syncthreads();
@@ -1831,13 +1836,12 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   // the stimulus alignment's remainders for either outcome (rk_pd54m.c:279,
   // :317), on wavefront 1 beside the norm
   if (j == 64) {
-    const GrwProtocol& p = ctl->prot;
-    tts_sh[0] = p.shift - fmod(start + h, p.period);
-    tts_sh[1] = p.shift - fmod(start, p.period);
+    tts_sh[0] = pl.shift - fmod(start + h, pl.period);
+    tts_sh[1] = pl.shift - fmod(start, pl.period);
   }
   __syncthreads();
   if (j == 0) {
-    const GrwProtocol& p = ctl->prot;
+    const GrwProtocol& p = pl;
     const bool accept = err < p.tolrk;
     double hn = h;
     if (accept || h <= p.step_min) {
@@ -1846,7 +1850,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       ctl->start = ns;
       ctl->success = 1;
       if (!accept) ctl->forced++;
-      if (ns >= ctl->tstep) {
+      if (ns >= tstep) {
         decision = 2;  // this rk54pd call is complete
       } else if (accept) {
         hn = rmin(0.85 * h * pow(p.tolrk / err, 0.2), 4.0 * h);
@@ -1860,7 +1864,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       if (!accept) {
         ctl->oh = hn;
         hn = rmin(p.step_max, rmax(p.step_min, hn));
-        hn = rmin(hn, ctl->tstep - ns);
+        hn = rmin(hn, tstep - ns);
       }
     } else {
       decision = 0;
@@ -1872,7 +1876,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       ctl->success = 0;
       ctl->resume = 1;
       hn = rmin(p.step_max, rmax(p.step_min, hn));
-      hn = rmin(hn, ctl->tstep - start);
+      hn = rmin(hn, tstep - start);
     }
     ctl->h = hn;
     ctl->stage = 0;
@@ -1908,8 +1912,8 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   if (ctl->out_cur && j < kNcur) ctl->out_cur[orow * kNcur + j] = ctl->cur[j];
   __syncthreads();
   if (j == 0) {
-    const GrwProtocol& p = ctl->prot;
-    ctl->pend_t = ctl->tstep;
+    const GrwProtocol& p = pl;
+    ctl->pend_t = tstep;
     ctl->pend_row = row;
     ctl->pend_V = yv[0];
     ctl->pend_Cai = yv[6];
@@ -1921,7 +1925,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
     ctl->avg_row = row;
     // the next call starts on the output grid (caf.c:150) with the step of
     // rk_pd54m.c:71-74; a later run continues from here
-    const double tc = ctl->tstep;
+    const double tc = tstep;
     ctl->tcall = tc;
     ctl->tstep = tc + p.time_step;
     double hn = rmin(ctl->oh, ctl->tstep - tc);
