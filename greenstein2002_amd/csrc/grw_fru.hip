@@ -1256,10 +1256,10 @@ __device__ __forceinline__ void fru_rollback(const SimArgs& a, const Dev& d, int
 constexpr int kBeatStats = 15;
 // The part of a stage's fcn that does not depend on the release units'
 // fluxes, as the stage's fcn block leaves it for the tail: F[0..36] (the
-// assembled components excepted), the 14 currents of FcnCurrents, the three
-// of FcnCa, V and Istim; and the flux conversion's exp at the stage's end
-// potential.
-constexpr int kPreCur = 37, kPreCa = 51, kPreV = 54, kPreIstim = 55, kPreE1 = 56, kPre = 57;
+// assembled components excepted), the FcnAsm terms, V and Istim; and the flux
+// conversion's exp at the stage's end potential.
+constexpr int kPreAsm = 37, kPreV = kPreAsm + kFcnAsm, kPreIstim = kPreV + 1, kPreE1 = kPreV + 2,
+              kPre = kPreV + 3;
 
 struct RkCtl {
   GrwProtocol prot;
@@ -1688,13 +1688,11 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       double* pre = ctl->pre;
       if (j < kNglobal) __hip_atomic_store(pre + j, F[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (j == 0) {
-        const double cv[14] = {c.INa, c.IKr, c.IKs, c.IKv43, c.IKv14, c.IKv14_K, c.IKv14_Na,
-                               c.IK1, c.INab, c.IKp, c.INaK, c.INaCa, c.ICab, c.IpCa};
-        for (int q = 0; q < 14; ++q)
-          __hip_atomic_store(pre + kPreCur + q, cv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(pre + kPreCa, ca.Jup, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(pre + kPreCa + 1, ca.Jtrpn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(pre + kPreCa + 2, ca.beta_i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const FcnAsm m = fcn_asm_pre(c, ca, Istim, ctl->prot.chf != 0);
+        const double mv[kFcnAsm] = {m.dNai, m.dKi, m.a3ca, m.mca, m.nsr, m.beta_i, m.Jup,
+                                    m.Jtrpn, m.INa, m.IKr, m.IKs, m.p2, m.a3v};
+        for (int q = 0; q < kFcnAsm; ++q)
+          __hip_atomic_store(pre + kPreAsm + q, mv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(pre + kPreV, V, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(pre + kPreIstim, Istim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
@@ -1761,14 +1759,9 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   }
   __syncthreads();
   if (j == 0) {  // fcn's assembly with this launch's fluxes
-    FcnCurrents c;
-    c.INa = pre[kPreCur]; c.IKr = pre[kPreCur + 1]; c.IKs = pre[kPreCur + 2]; c.IKv43 = pre[kPreCur + 3];
-    c.IKv14 = pre[kPreCur + 4]; c.IKv14_K = pre[kPreCur + 5]; c.IKv14_Na = pre[kPreCur + 6];
-    c.IK1 = pre[kPreCur + 7]; c.INab = pre[kPreCur + 8]; c.IKp = pre[kPreCur + 9];
-    c.INaK = pre[kPreCur + 10]; c.INaCa = pre[kPreCur + 11]; c.ICab = pre[kPreCur + 12];
-    c.IpCa = pre[kPreCur + 13];
-    const FcnCa ca{pre[kPreCa], pre[kPreCa + 1], pre[kPreCa + 2]};
-    fcn_assemble(yv, pre[kPreV], pre[kPreIstim], c, ca, F, nullptr, false, Jsh, pl);
+    const double* m = pre + kPreAsm;
+    const FcnAsm a{m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9], m[10], m[11], m[12]};
+    (void)fcn_asm_fin(a, yv, pre[kPreV], pre[kPreIstim], Jsh, pl.vclamp != 0, F);
     for (int q = 0; q < 4; ++q) ctl->J[q] = Jsh[q];
   }
   __syncthreads();

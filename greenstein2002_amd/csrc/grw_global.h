@@ -458,43 +458,88 @@ GRW_HD void fcn_ca(const double* state, const GrwProtocol& p, const FcnTrans& q,
   F[10] = dHTRPNCa;
 }
 
-// The CHF scaling (fcn.c:780-786), the concentration and voltage derivatives
-// F[0], F[4..7], F[31] and the currents record (fcn.c:788-1071).  Under
-// voltage clamp state[0] takes the clamp potential (fcn.c:811-814).
-GRW_HD void fcn_assemble(double* state, double V, double Istim, FcnCurrents c, FcnCa ca, double* F,
-                         double* cur, bool keepc, const double J[4], const GrwProtocol& p) {
-  const double Jxfer = J[0], Jtr = J[1], ICa = J[2], Ito2 = J[3];
-  double Ito1 = c.IKv43 + c.IKv14;
-  if (p.chf) {  // :780-786
+// The CHF scaling (fcn.c:780-786) of the currents and the uptake, and Ito1.
+GRW_HD void fcn_chf(FcnCurrents& c, FcnCa& ca, double& Ito1, bool chf) {
+  Ito1 = c.IKv43 + c.IKv14;
+  if (chf) {
     c.IK1 = 0.68 * c.IK1;
     ca.Jup = 0.38 * ca.Jup;
     c.INaCa = 1.75 * c.INaCa;
     c.IKv43 = 0.143 * c.IKv43;
     Ito1 = c.IKv43 + c.IKv14;
   }
-  double a1, a2, a3;
-  a1 = Acap / (Vmyo * Faraday * 1000.0);
-  const double dNai = -(c.INa + c.INab + 3.0 * (c.INaCa + c.INaK) + c.IKv14_Na) * a1;
-  a3 = c.IKr + c.IKs + c.IK1 + c.IKp;
-  const double dKi = -(a3 - 2.0 * c.INaK + (Ito1 - c.IKv14_Na) + Istim) * a1;
-  a3 = c.ICab - 2.0 * c.INaCa + c.IpCa;
-  const double dCai = ca.beta_i * (rdiv(Jxfer * VSS, divk(Vmyo)) - ca.Jup - ca.Jtrpn - a3 * 0.5 * a1);
-  const double dCaNSR = rdiv(ca.Jup * Vmyo, divk(VNSR)) - rdiv(Jtr * VJSR, divk(VNSR));
-  const double dCaTOT = -(a3 + ICa) * 0.5 * a1 * Vmyo * 1.0e6;
-  a1 = c.INa + ICa + c.IKr + c.IKs;
-  a2 = c.IK1 + c.IKp + c.INaCa + c.INaK + Ito1 + Ito2;
-  a3 = c.IpCa + c.ICab + c.INab;
+}
+
+// The terms of the concentration and voltage derivatives (fcn.c:791-822)
+// that do not depend on the release units' fluxes, each as far as the
+// reference's left-to-right evaluation gets before the first flux enters.
+struct FcnAsm {
+  double dNai, dKi;         // complete
+  double a3ca, mca;         // ICab - 2 INaCa + IpCa, and that times 0.5 Acap/(Vmyo F 1000)
+  double nsr;               // Jup Vmyo / VNSR
+  double beta_i, Jup, Jtrpn;
+  double INa, IKr, IKs;     // the operands of the first voltage sum around ICa
+  double p2;                // IK1 + IKp + INaCa + INaK + Ito1 (Ito2 follows)
+  double a3v;               // IpCa + ICab + INab
+};
+constexpr int kFcnAsm = 13;
+
+GRW_HD FcnAsm fcn_asm_pre(FcnCurrents c, FcnCa ca, double Istim, bool chf) {
+  double Ito1;
+  fcn_chf(c, ca, Ito1, chf);
+  const double a1 = Acap / (Vmyo * Faraday * 1000.0);
+  FcnAsm s;
+  s.dNai = -(c.INa + c.INab + 3.0 * (c.INaCa + c.INaK) + c.IKv14_Na) * a1;
+  const double a3 = c.IKr + c.IKs + c.IK1 + c.IKp;
+  s.dKi = -(a3 - 2.0 * c.INaK + (Ito1 - c.IKv14_Na) + Istim) * a1;
+  s.a3ca = c.ICab - 2.0 * c.INaCa + c.IpCa;
+  s.mca = s.a3ca * 0.5 * a1;
+  s.nsr = rdiv(ca.Jup * Vmyo, divk(VNSR));
+  s.beta_i = ca.beta_i;
+  s.Jup = ca.Jup;
+  s.Jtrpn = ca.Jtrpn;
+  s.INa = c.INa;
+  s.IKr = c.IKr;
+  s.IKs = c.IKs;
+  s.p2 = c.IK1 + c.IKp + c.INaCa + c.INaK + Ito1;
+  s.a3v = c.IpCa + c.ICab + c.INab;
+  return s;
+}
+
+// The rest with the fluxes J = {Jxfer, Jtr, ICa, Ito2}: F[0], F[4..7], F[31];
+// returns Itot.  Under voltage clamp state[0] takes the clamp potential
+// (fcn.c:811-814).
+GRW_HD double fcn_asm_fin(const FcnAsm& s, double* state, double V, double Istim, const double J[4], bool vclamp,
+                          double* F) {
+  const double Jxfer = J[0], Jtr = J[1], ICa = J[2], Ito2 = J[3];
+  const double a1c = Acap / (Vmyo * Faraday * 1000.0);
+  const double dCai = s.beta_i * (rdiv(Jxfer * VSS, divk(Vmyo)) - s.Jup - s.Jtrpn - s.mca);
+  const double dCaNSR = s.nsr - rdiv(Jtr * VJSR, divk(VNSR));
+  const double dCaTOT = -(s.a3ca + ICa) * 0.5 * a1c * Vmyo * 1.0e6;
+  const double a1 = s.INa + ICa + s.IKr + s.IKs;
+  const double a2 = s.p2 + Ito2;
   double dV, Itot;
-  if (p.vclamp) {
+  if (vclamp) {
     dV = 0.0;
     state[0] = V;
-    Itot = a1 + a2 + a3;
+    Itot = a1 + a2 + s.a3v;
   } else {
-    Itot = a1 + a2 + a3 + Istim;
+    Itot = a1 + a2 + s.a3v + Istim;
     dV = -Itot;
   }
-  F[0] = dV; F[4] = dNai; F[5] = dKi; F[6] = dCai; F[7] = dCaNSR; F[31] = dCaTOT;
+  F[0] = dV; F[4] = s.dNai; F[5] = s.dKi; F[6] = dCai; F[7] = dCaNSR; F[31] = dCaTOT;
+  return Itot;
+}
+
+// The CHF scaling, the concentration and voltage derivatives F[0], F[4..7],
+// F[31] and the currents record (fcn.c:780-1071).
+GRW_HD void fcn_assemble(double* state, double V, double Istim, FcnCurrents c, FcnCa ca, double* F,
+                         double* cur, bool keepc, const double J[4], const GrwProtocol& p) {
+  const double Itot = fcn_asm_fin(fcn_asm_pre(c, ca, Istim, p.chf != 0), state, V, Istim, J, p.vclamp != 0, F);
   if (keepc && cur) {  // :1046-1071
+    const double Jxfer = J[0], Jtr = J[1], ICa = J[2], Ito2 = J[3];
+    double Ito1;
+    fcn_chf(c, ca, Ito1, p.chf != 0);
     cur[0] = c.INa; cur[1] = c.IKr; cur[2] = c.IKs; cur[3] = Ito1; cur[4] = c.IK1; cur[5] = c.IKp;
     cur[6] = c.INaCa; cur[7] = c.INaK; cur[8] = c.IpCa; cur[9] = c.ICab; cur[10] = c.INab; cur[11] = ICa;
     cur[12] = rdiv(ICa * Acap, divk(-2.0 * Vmyo * Faraday * 1000.0));
