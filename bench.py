@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--graphs", type=int, default=0, choices=[0, 1],
+                    help="1: the device integrator replays RK attempts from a captured HIP graph")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs that measure roofline.traffic")
     ap.add_argument("--batch-cells", type=int, default=256, help="cells of the batched run (0 skips it)")
@@ -160,7 +162,7 @@ def measure_traffic(args) -> dict | None:
     win_ms = 35  # from t = 0 (no warmup, so that the launch count covers every dispatch)
     cell_cmd = [sys.executable, os.path.abspath(__file__), "--steps", str(win_ms), "--warmup", "0",
                 "--nfru", str(args.nfru), "--seed", str(args.seed), "--no-mixed", "--batch-cells", "0",
-                "--cpu-ms", "0", "--no-timing", "--no-traffic"]
+                "--cpu-ms", "0", "--no-timing", "--no-traffic", "--graphs", str(args.graphs)]
     copy_cmd = [sys.executable, os.path.abspath(__file__), "--calibrate-copy-mib", "256"]
     out = {}
     with tempfile.TemporaryDirectory(prefix="grw_pmc_", dir=env["TMPDIR"]) as tmp:
@@ -291,6 +293,7 @@ def run_cell(args, precision, timing):
     prot = Protocol()
     cell = Cell(args.nfru, seed=args.seed, precision=precision, protocol=prot, block=args.block,
                 lanes_per_fru=args.lanes)
+    cell.fru.set_graphs(bool(args.graphs))
     if args.warmup:
         cell.run(args.warmup, outputs=False, log_cap=0, driver=args.driver)
     cell.fru.set_timing(timing)
@@ -324,6 +327,7 @@ def batched(args):
     cl = rng.uniform(300.0, 2000.0, n)
     specs = [CellSpec(int(nf[k]), seed=1234 + k, protocol=Protocol(period=float(cl[k]))) for k in range(n)]
     b = Batch(specs, precision=args.precision)
+    b.set_graphs(bool(args.graphs))
     b.set_timing(True)
     b.reset_stats()
     t0 = time.perf_counter()
@@ -497,6 +501,7 @@ def sweep(args) -> dict:
 
     def run(cells, prec):
         b = Batch(cells, precision=prec)
+        b.set_graphs(bool(args.graphs))
         b.set_timing(True)
         b.reset_stats()
         t0 = time.perf_counter()
@@ -616,6 +621,7 @@ def main():
                                                       else "mixed FP32/FP64", args.warmup,
                                                       args.warmup + args.steps),
                    "nfru": args.nfru, "precision": args.precision, "lanes_per_fru": args.lanes,
+                   "hip_graphs": bool(args.graphs),
                    "driver": args.driver,
                    "rk_attempts": attempts, "rk_rejects": rejects,
                    "fru_steps": st["fru_steps"], "launches": st["launches"],

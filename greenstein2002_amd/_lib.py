@@ -35,6 +35,7 @@ EXPORTS = (
     "grw_get_stats",
     "grw_reset_stats",
     "grw_set_timing",
+    "grw_set_graphs",
 )
 
 
@@ -101,6 +102,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.grw_get_stats.argtypes = [vp, C.POINTER(GrwStats)]
     L.grw_reset_stats.argtypes = [vp]
     L.grw_set_timing.argtypes = [vp, C.c_int]
+    L.grw_set_graphs.argtypes = [vp, C.c_int]
     for name in EXPORTS:
         if name not in ("grw_last_error", "grw_config_default"):
             getattr(L, name).restype = C.c_int

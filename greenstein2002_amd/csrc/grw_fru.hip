@@ -2265,6 +2265,10 @@ struct grw_ctx {
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> evpool;  // per-launch timing of device-driven runs
+  bool graphs = false;             // device-driven runs replay captured attempts
+  hipGraphExec_t graph = nullptr;  // kGraphAttempts RK attempts, captured on first use
+  SimArgs graph_base{};            // the launch arguments the graph holds
+  Dev graph_dev{};
   std::vector<void*> allocs;
   // cells: one for a context made by grw_create, many for grw_batch_create
   int ncells = 1;
@@ -2462,6 +2466,53 @@ int reset_order(grw_ctx* c) {
   return 0;
 }
 
+// One RK attempt's launches: five k_dev_stage, then a batch's regrouping.
+// ek (may be null) holds the ten events that bracket the five launches.
+void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent_t* ek) {
+  for (int s5 = 0; s5 < 5; ++s5) {
+    if (ek) (void)hipEventRecord(ek[2 * s5], c->stream);
+    if (mixed)
+      hipLaunchKernelGGL(k_dev_stage<float>, dim3(c->ncells + c->grid_s), dim3(256), 0, c->stream, base, c->d,
+                         c->dctl);
+    else
+      hipLaunchKernelGGL(k_dev_stage<double>, dim3(c->ncells + c->grid_s), dim3(256), 0, c->stream, base, c->d,
+                         c->dctl);
+    if (ek) (void)hipEventRecord(ek[2 * s5 + 1], c->stream);
+  }
+  if (base.sorted) sort_and_order(c);
+}
+
+// Attempts per graph replay: with the attempt that run_device launches
+// directly (the one it may time), a group of kTimeEvery.
+constexpr int kGraphAttempts = kTimeEvery - 1;
+
+// Capture kGraphAttempts attempts into c->graph, once per context: every
+// attempt's launches are the same kernels with the same arguments (the
+// control blocks on the device decide what each one does), so one graph
+// serves every run.  Kernel launches only: nothing is cleared inside it.
+int ensure_graph(grw_ctx* c, const SimArgs& base, bool mixed) {
+  if (c->graph && std::memcmp(&c->graph_base, &base, sizeof base) == 0 &&
+      std::memcmp(&c->graph_dev, &c->d, sizeof c->d) == 0)
+    return 0;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  for (int k = 0; k < kGraphAttempts; ++k) enqueue_attempt(c, base, mixed, nullptr);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e == hipSuccess) e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->graph = nullptr;
+    return fail(GRW_EDEVICE, std::string("attempt graph capture failed: ") + hipGetErrorString(e));
+  }
+  c->graph_base = base;
+  c->graph_dev = c->d;
+  return 0;
+}
+
 // Enqueue RK attempts (five k_dev_stage each, after one k_dev_begin) until every active cell
 // of c->hctl has reached its ms_target; the host synchronises once per batch of
 // attempts sized to ~25 simulated ms of the slowest cell, and launches after a
@@ -2489,6 +2540,11 @@ int run_device(grw_ctx* c) {
     }
     if (K == 0) break;
     K = std::max(4, std::min(K, 4000));
+    const bool graph = c->graphs;
+    if (graph) {  // whole groups of kTimeEvery attempts (4000 is a multiple of 8)
+      K = (K + kTimeEvery - 1) / kTimeEvery * kTimeEvery;
+      if (int rc = ensure_graph(c, base, mixed)) return rc;
+    }
     // timing: HIP events around the five launches of one attempt in kTimeEvery
     // (events around every launch cost the run itself a few percent)
     const int nsamp = (K + kTimeEvery - 1) / kTimeEvery;
@@ -2510,20 +2566,17 @@ int run_device(grw_ctx* c) {
       if (base.sorted) sort_and_order(c);
     }
     first = false;
+    hipError_t eg = hipSuccess;
     for (int k = 0; k < K; ++k) {
-      const bool tk = timed && k % kTimeEvery == 0;
-      hipEvent_t* ek = evs.data() + 10 * (k / kTimeEvery);
-      for (int s5 = 0; s5 < 5; ++s5) {
-        if (tk) (void)hipEventRecord(ek[2 * s5], c->stream);
-        if (mixed)
-          hipLaunchKernelGGL(k_dev_stage<float>, dim3(nc + c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
-        else
-          hipLaunchKernelGGL(k_dev_stage<double>, dim3(nc + c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
-        if (tk) (void)hipEventRecord(ek[2 * s5 + 1], c->stream);
+      if (graph && k % kTimeEvery != 0) {  // the group's other attempts in one replay
+        if (k % kTimeEvery == 1 && eg == hipSuccess) eg = hipGraphLaunch(c->graph, c->stream);
+        continue;
       }
-      if (base.sorted) sort_and_order(c);
+      const bool tk = timed && k % kTimeEvery == 0;
+      enqueue_attempt(c, base, mixed, tk ? evs.data() + 10 * (k / kTimeEvery) : nullptr);
     }
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = eg;
     if (e == hipSuccess)
       e = hipMemcpyAsync(c->hctl.data(), c->dctl, sizeof(RkCtl) * nc, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -2684,6 +2737,7 @@ int grw_destroy(grw_ctx* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (hipEvent_t ev : c->evpool) (void)hipEventDestroy(ev);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -2911,6 +2965,12 @@ int grw_reset_stats(grw_ctx* c) {
 int grw_set_timing(grw_ctx* c, int on) {
   if (!c) return fail(GRW_EINVAL, "null context");
   c->timing = on != 0;
+  return 0;
+}
+
+int grw_set_graphs(grw_ctx* c, int on) {
+  if (!c) return fail(GRW_EINVAL, "null context");
+  c->graphs = on != 0;
   return 0;
 }
 

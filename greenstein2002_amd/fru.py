@@ -170,3 +170,8 @@ class FruCell:
 
     def set_timing(self, on: bool) -> None:
         check(self._L.grw_set_timing(self._h, int(bool(on))))
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay seven of every eight RK attempts of the device integrator
+        from one captured HIP graph (identical results)."""
+        check(self._L.grw_set_graphs(self._h, int(bool(on))))

@@ -114,6 +114,12 @@ int grw_reset_stats(grw_ctx *ctx);
 /* Enable device timing with HIP events: every launch of the host-driven path
  * (adds a sync per launch), one RK attempt in eight on the device integrator. */
 int grw_set_timing(grw_ctx *ctx, int on);
+/* Device integrator (grw_run_cell_device, grw_batch_run): launch seven of
+ * every eight RK attempts as one replay of a HIP graph captured on first use
+ * (the eighth, the one timed, is launched directly).  Results are identical
+ * either way.  No counterpart in the reference, whose attempts are host calls
+ * (rk_pd54m.c:93-345). */
+int grw_set_graphs(grw_ctx *ctx, int on);
 
 #ifdef __cplusplus
 }
