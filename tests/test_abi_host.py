@@ -55,6 +55,17 @@ def test_bad_config_rejected():
     assert b"block" in L.grw_last_error()
 
 
+def test_context_switches_reject_a_null_context():
+    """grw_set_timing and grw_set_graphs return GRW_EINVAL (-1) and set the
+    error text for a null context, without touching a device."""
+    from greenstein2002_amd import _lib
+
+    L = _lib.load()
+    for fn in (L.grw_set_timing, L.grw_set_graphs):
+        assert fn(None, 1) == -1
+        assert b"null context" in L.grw_last_error()
+
+
 def test_host_fcn_matches_reference_fixture(golden):
     from greenstein2002_amd.cell import fcn
 
