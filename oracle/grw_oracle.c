@@ -889,6 +889,26 @@ void orc_send_calc_fru_avg(orc_cell *c, const double state[ORC_NGLOBAL], double 
 void orc_save_state(orc_cell *c) { memcpy(c->hold, c->fru, sizeof(orc_fru) * (size_t)c->nfru); }
 void orc_resume_state(orc_cell *c) { memcpy(c->fru, c->hold, sizeof(orc_fru) * (size_t)c->nfru); }
 
+static void orc_ev(orc_cell *c, int kind, double t0, double t1, const double dep[3],
+                   const double out[4]) {
+  if (!c->ev) return;
+  if (c->ev_n >= c->ev_cap) { ++c->ev_n; return; }
+  orc_event *e = &c->ev[c->ev_n++];
+  memset(e, 0, sizeof *e);
+  e->kind = kind;
+  e->t0 = t0;
+  e->t1 = t1;
+  if (dep) memcpy(e->dep, dep, sizeof e->dep);
+  if (out) memcpy(e->out, out, sizeof e->out);
+}
+
+/* orc_distrib_simfru plus its trace record */
+static void orc_stage(orc_cell *c, double t0, double t1, const double dep0[3],
+                      const double depf[3], double out4[4]) {
+  orc_distrib_simfru(c, t0, t1, dep0, depf, out4);
+  orc_ev(c, ORC_EV_DISTRIB, t0, t1, depf, out4);
+}
+
 /* rk_pd54m.c:29-345: Prince-Dormand RK5(4)6M over [t, tstep] with one Monte Carlo
  * sub-interval per stage, FRU rollback on rejection and stimulus alignment. */
 double orc_rk54pd(orc_cell *c, double t, double tstep, double oldh, double state[ORC_NGLOBAL],
@@ -910,20 +930,22 @@ double orc_rk54pd(orc_cell *c, double t, double tstep, double oldh, double state
     dep0[0] = state[0]; dep0[1] = state[6]; dep0[2] = state[7];
     if (success) {
       orc_send_calc_fru_flux(c, dep0, J);
+      orc_ev(c, ORC_EV_FLUX, 0, 0, dep0, J);
       orc_fcn(start, state, F1, cur, keepc, J[0], J[1], J[2], J[3], p);
       keepc = 0;
       orc_save_state(c);
+      orc_ev(c, ORC_EV_SAVE, 0, 0, 0, 0);
     }
     /* stage 2 */
     for (int i = 0; i < N; ++i) { k1[i] = h * F1[i]; y[i] = state[i] + k1[i] / 5.0; }
     depf[0] = y[0]; depf[1] = y[6]; depf[2] = y[7];
-    orc_distrib_simfru(c, start, start + h / 5.0, dep0, depf, J);
+    orc_stage(c, start, start + h / 5.0, dep0, depf, J);
     dep0[0] = state[0]; dep0[1] = state[6]; dep0[2] = state[7];
     orc_fcn(start + h / 5.0, y, F, dummy, 0, J[0], J[1], J[2], J[3], p);
     /* stage 3 */
     for (int i = 0; i < N; ++i) { k2[i] = h * F[i]; y[i] = state[i] + (3.0 / 40.0 * k1[i] + 9.0 / 40.0 * k2[i]); }
     depf[0] = y[0]; depf[1] = y[6]; depf[2] = y[7];
-    orc_distrib_simfru(c, start + h / 5.0, start + h * 3.0 / 10.0, dep0, depf, J);
+    orc_stage(c, start + h / 5.0, start + h * 3.0 / 10.0, dep0, depf, J);
     dep0[0] = state[0]; dep0[1] = state[6]; dep0[2] = state[7];
     orc_fcn(start + h * 3.0 / 10.0, y, F, dummy, 0, J[0], J[1], J[2], J[3], p);
     /* stage 4 */
@@ -932,7 +954,7 @@ double orc_rk54pd(orc_cell *c, double t, double tstep, double oldh, double state
       y[i] = state[i] + (3.0 / 10.0 * k1[i] - 9.0 / 10.0 * k2[i] + 6.0 / 5.0 * k3[i]);
     }
     depf[0] = y[0]; depf[1] = y[6]; depf[2] = y[7];
-    orc_distrib_simfru(c, start + h * 3.0 / 10.0, start + h * 3.0 / 5.0, dep0, depf, J);
+    orc_stage(c, start + h * 3.0 / 10.0, start + h * 3.0 / 5.0, dep0, depf, J);
     dep0[0] = state[0]; dep0[1] = state[6]; dep0[2] = state[7];
     orc_fcn(start + h * 3.0 / 5.0, y, F, dummy, 0, J[0], J[1], J[2], J[3], p);
     /* stage 5 */
@@ -942,7 +964,7 @@ double orc_rk54pd(orc_cell *c, double t, double tstep, double oldh, double state
              55.0 / 729.0 * k4[i];
     }
     depf[0] = y[0]; depf[1] = y[6]; depf[2] = y[7];
-    orc_distrib_simfru(c, start + h * 3.0 / 5.0, start + h * 2.0 / 3.0, dep0, depf, J);
+    orc_stage(c, start + h * 3.0 / 5.0, start + h * 2.0 / 3.0, dep0, depf, J);
     dep0[0] = state[0]; dep0[1] = state[6]; dep0[2] = state[7];
     orc_fcn(start + h * 2.0 / 3.0, y, F, dummy, 0, J[0], J[1], J[2], J[3], p);
     /* stage 6 */
@@ -952,7 +974,7 @@ double orc_rk54pd(orc_cell *c, double t, double tstep, double oldh, double state
              91.0 / 27.0 * k4[i] + 189.0 / 55.0 * k5[i];
     }
     depf[0] = y[0]; depf[1] = y[6]; depf[2] = y[7];
-    orc_distrib_simfru(c, start + h * 2.0 / 3.0, start + h, dep0, depf, J);
+    orc_stage(c, start + h * 2.0 / 3.0, start + h, dep0, depf, J);
     orc_fcn(start + h, y, F, dummy, 0, J[0], J[1], J[2], J[3], p);
     for (int i = 0; i < N; ++i) k6[i] = h * F[i];
 
@@ -1016,6 +1038,7 @@ double orc_rk54pd(orc_cell *c, double t, double tstep, double oldh, double state
         success = 0;
         notdone = 1;
         orc_resume_state(c);
+        orc_ev(c, ORC_EV_RESUME, 0, 0, 0, 0);
       }
       h = ref_min(step_max, ref_max(step_min, h));
       h = ref_min(h, tstep - start);

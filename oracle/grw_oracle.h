@@ -15,9 +15,10 @@
  *   simfru step        -> reference objects (bit-exact traces), fixtures
  *   flux / avg sums    -> reference objects, brute force
  *   global fcn         -> reference fcn.o (bit-exact)
- *   rk54pd driver      -> restated from rk_pd54m.c (reference TU needs mpi.h and
- *                         is unbuildable here); pinned by the Dormand-Prince
- *                         order test and by the step-control invariants.
+ *   flux / avg outputs -> the reference's send_calc_fru_flux, distrib_simFRU and
+ *                         send_calc_fru_avg (mpi_master.c, one MPI rank)
+ *   rk54pd driver      -> the reference's rk54pd (rk_pd54m.c) call by call: the
+ *                         same sub-intervals, saves, rollbacks and states
  */
 #ifndef GRW_ORACLE_H
 #define GRW_ORACLE_H
@@ -122,6 +123,17 @@ double orc_total_ca(const orc_fru *f, int n, const double state[ORC_NGLOBAL], do
                                                             /* initialize_state.c:233-246 */
 
 /* ---- whole cell (caf.c + rk_pd54m.c + mpi_master.c with one process) ---- */
+
+/* One call orc_rk54pd makes into the FRU layer, in call order: the same record
+ * the reference harness writes for rk_pd54m.c's calls (oracle/ref_harness.c). */
+enum { ORC_EV_FLUX = 0, ORC_EV_SAVE = 1, ORC_EV_DISTRIB = 2, ORC_EV_RESUME = 3 };
+typedef struct {
+  int32_t kind, pad;
+  double t0, t1;     /* sub-interval (ORC_EV_DISTRIB) */
+  double dep[3];     /* dep (ORC_EV_FLUX) or depf (ORC_EV_DISTRIB) */
+  double out[4];     /* Jxfer, Jtr, ICa, Ito2 */
+} orc_event;
+
 typedef struct {
   int nfru;
   double scale;
@@ -131,6 +143,9 @@ typedef struct {
   double errw[ORC_NGLOBAL];
   /* counters */
   int64_t n_attempts, n_rejects, n_forced, n_fru_steps;
+  /* optional call trace of orc_rk54pd (NULL: off); ev_n counts past ev_cap */
+  orc_event *ev;
+  int32_t ev_cap, ev_n;
 } orc_cell;
 
 int orc_cell_init(orc_cell *c, int nfru, double scale, const orc_protocol *p);

@@ -89,6 +89,21 @@ PROT_DTYPE = np.dtype(
 )
 
 
+# struct orc_event / ref_event: one call the RK driver makes into the FRU layer
+EV_FLUX, EV_SAVE, EV_DISTRIB, EV_RESUME = 0, 1, 2, 3
+EVENT_DTYPE = np.dtype(
+    [
+        ("kind", "<i4"),
+        ("pad", "<i4"),
+        ("t0", "<f8"),
+        ("t1", "<f8"),
+        ("dep", "<f8", (3,)),
+        ("out", "<f8", (4,)),
+    ],
+    align=True,
+)
+
+
 class OrcCell(C.Structure):
     _fields_ = [
         ("nfru", C.c_int),
@@ -101,6 +116,9 @@ class OrcCell(C.Structure):
         ("n_rejects", C.c_int64),
         ("n_forced", C.c_int64),
         ("n_fru_steps", C.c_int64),
+        ("ev", C.c_void_p),
+        ("ev_cap", C.c_int32),
+        ("ev_n", C.c_int32),
     ]
 
 
@@ -317,6 +335,19 @@ class OracleCell:
                                 _p(log) if log_cap else None, log_cap, C.byref(n))
         return h, log[: n.value]
 
+    def trace(self, cap):
+        """Record the calls of the following rk54pd calls (cap records)."""
+        self._ev = np.zeros(max(cap, 1), EVENT_DTYPE)
+        self.c.ev = self._ev.ctypes.data if cap else None
+        self.c.ev_cap = cap
+        self.c.ev_n = 0
+
+    def events(self):
+        n = self.c.ev_n
+        if n > self.c.ev_cap:
+            raise OverflowError(f"{n} events for {self.c.ev_cap} records")
+        return self._ev[:n].copy()
+
 
 class Ref:
     """The reference's own objects (oracle/_ref/libgrw_ref.so).  Arrays use the
@@ -339,10 +370,23 @@ class Ref:
         L.ref_flux_sums.argtypes = [C.c_int, _DP, _DP, _DP, _DP]
         L.ref_avg_sums.argtypes = [C.c_int, _DP, _DP, _DP, _DP, _DP]
         L.ref_fcn.argtypes = [C.c_double, _DP, _DP, _DP, C.c_int] + [C.c_double] * 4
-        L.ref_set_block.argtypes = [C.c_int, _DP, _DP, _DP, C.c_double]
         L.ref_initialize_state.argtypes = [C.c_char_p, C.c_int, C.c_double] + [_DP] * 8
         L.ref_initialize_state.restype = C.c_int
-        L.send_calc_fru_flux.argtypes = [_DP] * 5
+        L.ref_mpi_init.restype = C.c_int
+        L.ref_load_block.argtypes = [C.c_int, C.c_double] + [_DP] * 6
+        L.ref_load_block.restype = C.c_int
+        L.ref_get_block.argtypes = [C.c_int] + [_DP] * 6
+        L.ref_get_block.restype = C.c_int
+        L.ref_send_calc_fru_flux.argtypes = [_DP, _DP]
+        L.ref_send_calc_fru_avg.argtypes = [_DP, _DP]
+        L.ref_distrib_simfru.argtypes = [C.c_double, C.c_double, _DP, _DP, _DP]
+        L.ref_set_errweight.argtypes = [_DP]
+        L.ref_rk54pd.argtypes = [C.c_double, C.c_double, C.c_double, _DP, _DP]
+        L.ref_rk54pd.restype = C.c_double
+        L.ref_trace.argtypes = [_DP, C.c_int]
+        L.ref_trace_n.restype = C.c_int
+        L.ref_dynamic_fru.argtypes = [C.c_double] * 4 + [_DP] * 3
+        L.ref_set_nfru.argtypes = [C.c_int, C.c_double]
 
     # FRU arrays in reference layout, for n FRUs
     @staticmethod
@@ -434,12 +478,64 @@ class Ref:
                             _p(a["ito2"]), _p(out))
         return out
 
-    def send_calc_fru_flux(self, a, dep, scale):
-        self.L.ref_set_block(len(a["fru"]), _p(a["ltype"]), _p(a["ito2"]), _p(a["fru"]), scale)
-        outs = [C.c_double() for _ in range(4)]
-        dep = np.ascontiguousarray(dep, np.float64)
-        self.L.send_calc_fru_flux(_p(dep), *[C.cast(C.byref(o), C.c_void_p) for o in outs])
-        return np.array([o.value for o in outs])
+    # -- the MPI master surface (mpi_master.c) over one rank
+    def load_block(self, a, scale):
+        """initialize_mpi_state (mpi_master.c:199) of array set a (<= 1000 FRUs)."""
+        rc = self.L.ref_load_block(len(a["fru"]), scale, _p(a["fru"]), _p(a["ltype"]), _p(a["ryr"]),
+                                   _p(a["ito2"]), _p(a["mti"]), _p(a["mt"]))
+        if rc:
+            raise RuntimeError(f"ref_load_block rc={rc}")
+
+    def get_block(self, n):
+        """parallel_get_FRUs (mpi_master.c:509) into a fresh array set."""
+        a = self.alloc(n)
+        rc = self.L.ref_get_block(n, _p(a["fru"]), _p(a["ltype"]), _p(a["ryr"]), _p(a["ito2"]),
+                                  _p(a["mti"]), _p(a["mt"]))
+        if rc:
+            raise RuntimeError(f"ref_get_block rc={rc}")
+        return a
+
+    def send_calc_fru_flux(self, dep):
+        out = np.zeros(4)
+        self.L.ref_send_calc_fru_flux(_p(np.ascontiguousarray(dep, np.float64)), _p(out))
+        return out
+
+    def send_calc_fru_avg(self, state):
+        out = np.zeros(NOTHER)
+        self.L.ref_send_calc_fru_avg(_p(np.ascontiguousarray(state, np.float64)), _p(out))
+        return out
+
+    def distrib_simfru(self, t0, t1, dep0, depf):
+        out = np.zeros(4)
+        self.L.ref_distrib_simfru(t0, t1, _p(np.ascontiguousarray(dep0, np.float64)),
+                                  _p(np.ascontiguousarray(depf, np.float64)), _p(out))
+        return out
+
+    def set_errweight(self, w):
+        self._errw = np.ascontiguousarray(w, np.float64)
+        self.L.ref_set_errweight(_p(self._errw))
+
+    def trace(self, cap):
+        self._ev = np.zeros(max(cap, 1), EVENT_DTYPE)
+        self.L.ref_trace(_p(self._ev) if cap else None, cap)
+
+    def events(self):
+        n = self.L.ref_trace_n()
+        if n > len(self._ev):
+            raise OverflowError(f"{n} events for {len(self._ev)} records")
+        return self._ev[:n].copy()
+
+    def rk54pd(self, t, tstep, oldh, state, cur):
+        """rk54pd (rk_pd54m.c:29) on the loaded block; state and cur updated in place."""
+        return self.L.ref_rk54pd(t, tstep, oldh, _p(state), _p(cur))
+
+    def dynamic_fru(self, t, V, pryr, plcc, phase, t_last):
+        """dynamicFRU's phase decision (dynamicFRU.c:26-150), factor 1."""
+        ph = np.array([phase], np.int32)
+        tl = np.array([t_last], np.float64)
+        tts = np.zeros(1)
+        self.L.ref_dynamic_fru(t, V, pryr, plcc, _p(ph), _p(tl), _p(tts))
+        return int(ph[0]), float(tl[0]), float(tts[0])
 
     def fcn(self, t, state, J4, keepc=1):
         st = np.array(state, np.float64)

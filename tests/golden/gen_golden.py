@@ -1,12 +1,12 @@
 """Write the golden fixtures under tests/golden/ from the reference's own code.
 
 Runs only where the reference tree exists (default /root/reference): it builds
-oracle/_ref/libgrw_ref.so from the reference's unmodified hot-path sources
-(``make -C oracle``) and records inputs and outputs of those compiled
-functions.  The fixtures are data (inputs and expected outputs) and are
-committed; this script is how they were made.
+oracle/_ref/libgrw_ref.so from the reference's unmodified sources, its MPI
+layer and global integrator included (``make -C oracle``), and records inputs
+and outputs of those compiled functions.  The fixtures are data (inputs and
+expected outputs) and are committed; this script is how they were made.
 
-    python tests/golden/gen_golden.py [--ref /root/reference]
+    python tests/golden/gen_golden.py [--ref /root/reference] [--only NAME ...]
 """
 from __future__ import annotations
 
@@ -156,11 +156,120 @@ def gen_sums(r: Ref, ic, rng):
     b["ito2"][:] = rng.integers(1, 3, (1250, 4))
     flux_b = r.flux_sums(b)
     avg_b = r.avg_sums(b)
-    deps = np.array([[-93.1, 1.4e-4, 0.9], [0.0, 1.4e-4, 0.9], [1e-7, 1.4e-4, 0.9], [20.0, 5e-4, 0.7]])
-    conv = np.array([r.send_calc_fru_flux(b, d, 10.0) for d in deps])
     np.savez_compressed(os.path.join(HERE, "sums_ref.npz"), flux=flux, avg=avg,
                         b_fru=b["fru"], b_ltype=b["ltype"], b_ryr=b["ryr"], b_ito2=b["ito2"],
-                        flux_b=flux_b, avg_b=avg_b, deps=deps, conv=conv, scale=10.0)
+                        flux_b=flux_b, avg_b=avg_b)
+
+
+def seeds_o3(a0: int, n: int) -> np.ndarray:
+    """initialize_ran.c:30-34 as gcc -O3 compiles it, for master seed a0 (the
+    closed form the fixture rng_ref.npz pins for a0 = 12350232)."""
+    j = np.arange(n, dtype=np.int64)
+    return ((a0 + 23 * (j * (j + 1) // 2) + 4093 * j * j + 1) % 2 ** 32).astype(np.uint32)
+
+
+def seeded_block(r: Ref, ic, n: int, a0: int):
+    """ic/stable FRUs 0..n-1 with MT streams from sgrnd_local of the -O3 seeds."""
+    a = _fresh(n)
+    for key in ("fru", "ltype", "ryr", "ito2"):
+        a[key][:] = ic[key][:n]
+    for k, sd in enumerate(seeds_o3(a0, n)):
+        mt, mti = r.sgrnd(int(sd))
+        a["mt"][k] = mt
+        a["mti"][k] = mti[0]
+    return a
+
+
+def gen_mpi_master(r: Ref, ic, state0, rng):
+    """The master-side surface of mpi_master.c over one MPI rank: the flux
+    conversion of send_calc_fru_flux (:41-113) and of distrib_simFRU (:401-468,
+    zero-length and real sub-intervals), and send_calc_fru_avg's 12 outputs
+    (:122-191), on the ic/stable block and a scrambled block of 1,000 FRUs (the
+    reference's MAX_LOCAL_NFRU)."""
+    n = 1000
+    blocks = []
+    a = _fresh(n)
+    for key in ("fru", "ltype", "ryr", "ito2"):
+        a[key][:] = ic[key][:n]
+    blocks.append(a)
+    b = _fresh(n)
+    b["fru"][:] = ic["fru"][:n] * rng.uniform(0.5, 2.0, (n, 5))
+    b["ltype"][:, :, 0] = rng.integers(1, 13, (n, 4))
+    b["ltype"][:, :, 1] = rng.integers(1, 3, (n, 4))
+    b["ryr"][:] = rng.integers(1, 9, (n, 4, 5))
+    b["ito2"][:] = rng.integers(1, 3, (n, 4))
+    blocks.append(b)
+    # V on both sides of the |V| < 1e-6 Taylor branch (mpi_master.c:92,448)
+    Vs = [-93.1, -40.0, -1.0e-6, -9.99e-7, -3.0e-7, 0.0, 1.0e-7, 9.99e-7, 1.0e-6, 1.01e-6,
+          0.37, 20.0, 55.0]
+    deps = np.array([[v, 1.4e-4 * (1 + 0.1 * k), 0.9 - 0.02 * k] for k, v in enumerate(Vs)])
+    scales = [12.5, 1.0]
+    flux = np.zeros((2, len(deps), 4))
+    conv = np.zeros((2, len(deps), 4))
+    for i, blk in enumerate(blocks):
+        r.load_block(blk, scales[i])
+        for k, d in enumerate(deps):
+            flux[i, k] = r.send_calc_fru_flux(d)
+            dep0 = d * [1.0, 0.9, 1.0] + [1.5, 0.0, 0.01]
+            conv[i, k] = r.distrib_simfru(2.5, 2.5, dep0, d)
+    states = state0 * rng.uniform(0.9, 1.1, (6, 37))
+    avg = np.zeros((2, len(states), 12))
+    for i, blk in enumerate(blocks):
+        r.load_block(blk, scales[i])
+        for k, st in enumerate(states):
+            avg[i, k] = r.send_calc_fru_avg(st)
+    # real sub-intervals through distrib_simFRU on 200 seeded FRUs
+    m = 200
+    c = seeded_block(r, ic, m, 1234)
+    r.load_block(c, 12500.0 / m)
+    sub = [(0.0, 0.05, -93.0, -92.0), (0.05, 0.2, -92.0, 1.0e-7), (0.2, 0.35, 1.0e-7, 30.0),
+           (0.35, 0.5, 30.0, 10.0)]
+    sub_out = np.zeros((len(sub), 4))
+    for k, (t0, t1, v0, v1) in enumerate(sub):
+        sub_out[k] = r.distrib_simfru(t0, t1, [v0, 1.5e-4, 0.92], [v1, 2.5e-4, 0.91])
+    after = r.get_block(m)
+    np.savez_compressed(
+        os.path.join(HERE, "mpi_master_ref.npz"), scales=np.array(scales),
+        b_fru=b["fru"], b_ltype=b["ltype"], b_ryr=b["ryr"], b_ito2=b["ito2"], deps=deps,
+        flux=flux, conv=conv, states=states, avg=avg, sub=np.array(sub), sub_out=sub_out,
+        sub_nfru=m, sub_a0=1234, after_fru=after["fru"], after_ltype=after["ltype"],
+        after_ryr=after["ryr"], after_ito2=after["ito2"], after_mti=after["mti"])
+
+
+def gen_rk54pd(r: Ref, ref_root: str, ms=25):
+    """rk54pd (rk_pd54m.c:29-345) driving the reference's MPI layer over one
+    rank: a 250-FRU cell (NFRU_scale 50) from ic/stable, master seed 1234, one
+    call per 1 ms output step (caf.c:150-161) through the stimulus at 10 ms.
+    Recorded: every call rk54pd makes (send_calc_fru_flux, send_save_state,
+    distrib_simFRU with its sub-interval, send_resume_state) with its outputs,
+    the returned step size, state and currents of each call, and the final FRU
+    state."""
+    n = 250
+    state0, ic, errw = r.initialize_state(ref_root, n, 12500.0 / n)
+    a = seeded_block(r, ic, n, 1234)
+    r.load_block(a, 12500.0 / n)
+    r.set_errweight(errw)
+    state = np.array(state0, np.float64)
+    cur = np.zeros(24)
+    h = 0.1
+    hs, states, curs, nev = [], [], [], []
+    r.trace(200000)
+    for k in range(ms):
+        h = r.rk54pd(float(k), float(k + 1), h, state, cur)
+        hs.append(h)
+        states.append(state.copy())
+        curs.append(cur.copy())
+        nev.append(r.L.ref_trace_n())
+    ev = r.events()
+    r.trace(0)
+    after = r.get_block(n)
+    np.savez_compressed(
+        os.path.join(HERE, "rk54pd_ref.npz"), nfru=n, a0=1234, ms=ms, h=np.array(hs),
+        states=np.array(states), curs=np.array(curs), nev=np.array(nev), ev_kind=ev["kind"],
+        ev_t=np.stack([ev["t0"], ev["t1"]], 1), ev_dep=ev["dep"], ev_out=ev["out"],
+        after_fru=after["fru"], after_mti=after["mti"],
+        after_chan=np.concatenate([after["ltype"].reshape(n, -1), after["ryr"].reshape(n, -1),
+                                   after["ito2"]], 1))
 
 
 def gen_fcn(r: Ref, state0, rng):
@@ -183,17 +292,30 @@ def gen_fcn(r: Ref, state0, rng):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--only", nargs="*", default=None,
+                    help="subset of: ic rng rates fcn_fru simfru sums fcn mpi_master rk54pd")
     args = ap.parse_args()
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), f"REF={args.ref}"], check=True)
+    want = lambda name: args.only is None or name in args.only  # noqa: E731
     r = Ref()
     rng = np.random.default_rng(20021)
-    state0, ic = gen_ic(r, args.ref)
-    mt_all, mti_all = gen_rng(r)
-    gen_rates(r, mt_all, rng)
-    gen_fcn_fru(r, rng)
-    gen_simfru(r, ic, mt_all, mti_all)
-    gen_sums(r, ic, rng)
-    gen_fcn(r, state0, rng)
+    state0, ic = gen_ic(r, args.ref)  # also the IC set the others start from
+    if want("rng") or want("rates") or want("simfru"):
+        mt_all, mti_all = gen_rng(r)
+    if want("rates"):
+        gen_rates(r, mt_all, rng)
+    if want("fcn_fru"):
+        gen_fcn_fru(r, rng)
+    if want("simfru"):
+        gen_simfru(r, ic, mt_all, mti_all)
+    if want("sums"):
+        gen_sums(r, ic, rng)
+    if want("fcn"):
+        gen_fcn(r, state0, rng)
+    if want("mpi_master"):
+        gen_mpi_master(r, ic, state0, np.random.default_rng(20022))
+    if want("rk54pd"):
+        gen_rk54pd(r, args.ref)
     print("fixtures written to", HERE)
 
 

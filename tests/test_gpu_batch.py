@@ -97,9 +97,9 @@ def test_device_beat_records_match_host_reduction():
 
 
 def test_ensemble_matches_cpu_oracle():
-    """16 seeds x 250 FRUs x 2 beats (1 Hz) in one batch against the CPU oracle's
-    runs on the same seeds (tests/golden/ensemble_c1.npz, reduced by the same
-    beat_stats).  The FP64 path follows the oracle operation for operation
+    """16 seeds x 250 FRUs x 2 beats (1 Hz) in one batch against the reference's
+    own runs on the same seeds (tests/golden/ensemble_c1.npz: its rk54pd over
+    its MPI layer, reduced by the same beat_stats).  The FP64 path follows the oracle operation for operation
     (concentrations within an ulp of exp), so per seed the statistics agree
     closely; the ensemble means of APD90, peak V, peak Cai, peak ICa and the
     SR / JSR loads agree within 0.5%."""
@@ -118,6 +118,30 @@ def test_ensemble_matches_cpu_oracle():
     assert close.all(axis=(1, 2)).sum() >= len(seeds) - 2, close.all(axis=(1, 2))
     m_got, m_ref = got[:, :, cols].mean(axis=0), ref[:, :, cols].mean(axis=0)
     np.testing.assert_allclose(m_got, m_ref, rtol=5e-3)
+
+
+def test_c1_output_traces_match_reference():
+    """The 1 ms output traces of the 250-FRU shape: V, Cai, the FRU-flux ICa of
+    each sample's last accepted step, and the whole-cell JRyR of
+    send_calc_fru_avg (mpi_master.c:177), for master seeds 1234 and 1235 over
+    the first beat (t = 1..300 ms), against the reference's own runs
+    (tests/golden/ensemble_c1.npz, written by the reference's rk54pd over its
+    MPI layer; stored as float32, so rtol 1e-5)."""
+    from pathlib import Path
+
+    g = np.load(Path(__file__).parent / "golden" / "ensemble_c1.npz")
+    ms = 300
+    for k in (0, 1):
+        seed = int(g["seeds"][k])
+        with Cell(int(g["nfru"]), seed=seed) as c:
+            r = c.run(ms, log_cap=400000)
+        idx = np.searchsorted(r.log[:, 0], r.t, side="right") - 1
+        ica = r.log[np.maximum(idx, 0), 3]
+        for name, got in (("V", r.states[:, 0]), ("Cai", r.states[:, 6]), ("ICa", ica),
+                          ("JRyR", r.other[:, 2])):
+            ref = g[name][k, :ms].astype(np.float64)
+            np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max(),
+                                       err_msg=f"seed {seed} {name}")
 
 
 def test_full_size_beat_matches_cpu_oracle():

@@ -229,13 +229,178 @@ def test_sums_match_reference_fixture(oracle, golden):
     b["ito2"] = g["b_ito2"]
     assert np.array_equal(oracle.flux_sums(b), g["flux_b"])
     assert np.array_equal(oracle.avg_sums(b), g["avg_b"])
-    for k, dep in enumerate(g["deps"]):
-        out = oracle.flux_currents(oracle.flux_sums(b), dep, 1250, float(g["scale"]))
-        assert np.array_equal(out, g["conv"][k])
     # brute force of the counts
     op = ((b["lcc"] == 6) | (b["lcc"] == 12)) & (b["yinact"] == 2)
     assert oracle.flux_sums(b)[3] == op.sum()
     assert oracle.flux_sums(b)[4] == (b["ito2"] == 2).sum()
+
+
+def _orc_block(ic_n=None, fru=None, ltype=None, ryr=None, ito2=None):
+    from greenstein2002_amd.ic import load_stable
+
+    if ic_n is not None:
+        ic = load_stable(ic_n)
+        fru, ltype, ryr, ito2 = ic["fru"], ic["ltype"], ic["ryr"], ic["ito2"]
+    f = np.zeros(len(fru), FRU_DTYPE)
+    f["conc"] = fru
+    f["lcc"] = ltype[:, :, 0]
+    f["yinact"] = ltype[:, :, 1]
+    f["ryr"] = ryr
+    f["ito2"] = ito2
+    return f
+
+
+def _seeded_cell_frus(oracle, n, a0):
+    f = _orc_block(ic_n=n)
+    for j in range(n):
+        m = oracle.mt_seed(oracle.seed_for_fru(a0, j))
+        f["mt"][j] = m["mt"]
+        f["mti"][j] = m["mti"]
+    return f
+
+
+def test_flux_and_avg_outputs_match_mpi_master_fixture(oracle, golden):
+    """orc_flux_currents and orc_avg_outputs against the reference's own
+    send_calc_fru_flux (mpi_master.c:41-113), distrib_simFRU's conversion
+    (:401-468, at the end-of-interval potential) and send_calc_fru_avg
+    (:122-191), compiled with its MPI layer and run as one rank
+    (tests/golden/mpi_master_ref.npz): 13 potentials on both sides of the
+    |V| < 1e-6 Taylor branch, two 1,000-FRU blocks, six global states -- bit-exact."""
+    g = golden("mpi_master_ref.npz")
+    blocks = [_orc_block(ic_n=1000),
+              _orc_block(fru=g["b_fru"], ltype=g["b_ltype"], ryr=g["b_ryr"], ito2=g["b_ito2"])]
+    taylor = 0
+    for i, b in enumerate(blocks):
+        scale = float(g["scales"][i])
+        num = oracle.flux_sums(b)
+        for k, dep in enumerate(g["deps"]):
+            out = oracle.flux_currents(num, dep, len(b), scale)
+            assert np.array_equal(out, g["flux"][i, k]), (i, k)
+            assert np.array_equal(out, g["conv"][i, k]), (i, k)
+            taylor += abs(dep[0]) < 1e-6
+        avg = oracle.avg_sums(b)
+        for k, st in enumerate(g["states"]):
+            assert np.array_equal(oracle.avg_outputs(avg, st, len(b), scale), g["avg"][i, k]), (i, k)
+    assert taylor == 2 * 5  # the Taylor branch is exercised on both blocks
+    # the swapped labels of mpi_master.c:189-190 are kept: CaJSRtot holds the CaSS total
+    b = blocks[1]
+    st = g["states"][0]
+    o = oracle.avg_outputs(oracle.avg_sums(b), st, len(b), float(g["scales"][1]))
+    css = oracle.avg_sums(b)[5]
+    assert o[10] == 1.e6 * css * (0.5 * 0.2 * 2.03e-12) * float(g["scales"][1])
+
+
+def test_distrib_subintervals_match_mpi_master_fixture(oracle, golden):
+    """orc_distrib_simfru over four sub-intervals of 200 seeded FRUs (one through
+    V = 1e-7) against the reference's distrib_simFRU: the four flux outputs and
+    every FRU's final state, bit-exact."""
+    from oracle.pyoracle import OracleCell
+
+    g = golden("mpi_master_ref.npz")
+    n = int(g["sub_nfru"])
+    cell = OracleCell(oracle, _seeded_cell_frus(oracle, n, int(g["sub_a0"])), 12500.0 / n)
+    for k, (t0, t1, v0, v1) in enumerate(g["sub"]):
+        out = cell.distrib(t0, t1, [v0, 1.5e-4, 0.92], [v1, 2.5e-4, 0.91])
+        assert np.array_equal(out, g["sub_out"][k]), k
+    f = cell.fru
+    assert np.array_equal(f["conc"], g["after_fru"])
+    assert np.array_equal(f["lcc"], g["after_ltype"][:, :, 0])
+    assert np.array_equal(f["yinact"], g["after_ltype"][:, :, 1])
+    assert np.array_equal(f["ryr"], g["after_ryr"])
+    assert np.array_equal(f["ito2"], g["after_ito2"])
+    assert np.array_equal(f["mti"], g["after_mti"])
+    cell.close()
+
+
+def _run_oracle_rk(oracle, n, a0, ms, state0):
+    from oracle.pyoracle import OracleCell
+
+    cell = OracleCell(oracle, _seeded_cell_frus(oracle, n, a0), 12500.0 / n)
+    cell.trace(200000)
+    state = np.array(state0, np.float64)
+    cur = np.zeros(24)
+    h = 0.1
+    hs, states, curs, nev = [], [], [], []
+    for k in range(ms):
+        h, _ = cell.rk54pd(float(k), float(k + 1), h, state, cur)
+        hs.append(h)
+        states.append(state.copy())
+        curs.append(cur.copy())
+        nev.append(cell.c.ev_n)
+    return cell, np.array(hs), np.array(states), np.array(curs), np.array(nev), cell.events()
+
+
+def test_rk54pd_matches_reference_call_by_call(oracle, golden):
+    """orc_rk54pd against the reference's rk54pd (rk_pd54m.c:29-345) driving its
+    own MPI layer (tests/golden/rk54pd_ref.npz): a 250-FRU cell, master seed
+    1234, 25 calls of 1 ms through the stimulus at 10 ms.  Every call the
+    driver makes -- flux, save, each stage's sub-interval (t0, t1, end-point
+    estimate, fluxes), rollback -- in the same order with the same values, the
+    same accepted / rejected steps, step sizes, states and currents, and the
+    same final FRU state: bit-exact."""
+    from greenstein2002_amd.ic import load_stable
+
+    g = golden("rk54pd_ref.npz")
+    n, ms = int(g["nfru"]), int(g["ms"])
+    cell, h, states, curs, nev, ev = _run_oracle_rk(oracle, n, int(g["a0"]), ms,
+                                                    load_stable(n)["state"])
+    assert np.array_equal(h, g["h"])
+    assert np.array_equal(nev, g["nev"])
+    assert np.array_equal(ev["kind"], g["ev_kind"])
+    assert np.array_equal(np.stack([ev["t0"], ev["t1"]], 1), g["ev_t"])
+    assert np.array_equal(ev["dep"], g["ev_dep"])
+    assert np.array_equal(ev["out"], g["ev_out"])
+    assert np.array_equal(states, g["states"])
+    assert np.array_equal(curs, g["curs"])
+    f = cell.fru
+    assert np.array_equal(f["conc"], g["after_fru"])
+    assert np.array_equal(f["mti"], g["after_mti"])
+    chan = np.concatenate([np.stack([f["lcc"], f["yinact"]], -1).reshape(n, -1),
+                           f["ryr"].reshape(n, -1), f["ito2"]], 1)
+    assert np.array_equal(chan, g["after_chan"])
+    # the run covers the control logic's branches: rejections with rollback and
+    # the step shortened to land just before the stimulus (rk_pd54m.c:279-285)
+    kinds = np.bincount(g["ev_kind"], minlength=4)
+    assert kinds[3] > 0 and kinds[2] == 5 * (kinds[1] + kinds[3])
+    t1 = g["ev_t"][g["ev_kind"] == 2][4::5, 1]
+    assert np.any((t1 > 10.0 - 1e-5) & (t1 < 10.0))
+    cell.close()
+
+
+@pytest.mark.skipif(not __import__("os").path.exists("/root/reference/rk_pd54m.c"),
+                    reason="reference tree not present")
+def test_rk54pd_equals_live_reference_driver(oracle):
+    """Where oracle/_ref is built: another seed and 40 ms, oracle against the
+    reference's rk54pd + MPI layer live, call by call, plus the 12 outputs of
+    send_calc_fru_avg after every call."""
+    from greenstein2002_amd.ic import load_stable
+    from oracle.pyoracle import Ref
+
+    r = Ref()
+    n, a0, ms = 180, 4321, 40
+    ic = load_stable(n)
+    f = _seeded_cell_frus(oracle, n, a0)
+    r.load_block(Ref.from_orc(f), 12500.0 / n)
+    r.set_errweight(oracle.errweight())
+    r.trace(200000)
+    cell, h, states, curs, nev, ev = _run_oracle_rk(oracle, n, a0, 0, ic["state"])
+    state_r = np.array(ic["state"], np.float64)
+    state_o = state_r.copy()
+    cur_r, cur_o = np.zeros(24), np.zeros(24)
+    h_r = h_o = 0.1
+    for k in range(ms):
+        h_r = r.rk54pd(float(k), float(k + 1), h_r, state_r, cur_r)
+        h_o, _ = cell.rk54pd(float(k), float(k + 1), h_o, state_o, cur_o)
+        assert h_r == h_o and np.array_equal(state_r, state_o) and np.array_equal(cur_r, cur_o), k
+        assert np.array_equal(r.send_calc_fru_avg(state_r), cell.avg(state_o)), k
+    er, eo = r.events(), cell.events()
+    assert len(er) == len(eo)
+    for key in ("kind", "t0", "t1", "dep", "out"):
+        assert np.array_equal(er[key], eo[key]), key
+    g = Ref.to_orc(r.get_block(n))
+    for key in ("conc", "lcc", "yinact", "ryr", "ito2", "mt", "mti"):
+        assert np.array_equal(g[key], cell.fru[key]), key
+    cell.close()
 
 
 def test_global_fcn_matches_reference_fixture(oracle, golden):

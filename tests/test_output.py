@@ -106,7 +106,7 @@ def test_restart_dump_matches_reference(tmp_path):
     ref.mkdir()
     r = run_ref(ref, """
         n = len(d["fru"])
-        L.ref_set_block(n, vp(d["lt"]), vp(d["ito2"]), vp(d["fru"]), C.c_double(1.0))
+        L.ref_set_nfru(n, C.c_double(1.0))
         L.restart_data(3, vp(d["state"]), vp(d["fru"]), vp(d["lt"]), vp(d["ryr"]), vp(d["ito2"]),
                        vp(d["mt"]), vp(d["mti"]))
     """, state=state, fru=fru, lt=ltype, ryr=ryr, ito2=ito2, mt=mt, mti=mti)
@@ -178,7 +178,7 @@ def test_fru_props_match_reference(tmp_path, V):
     ref.mkdir()
     r = run_ref(ref, f"""
         n = len(d["fru"])
-        L.ref_set_block(n, vp(d["lt"]), vp(d["ito2"]), vp(d["fru"]), C.c_double(1.0))
+        L.ref_set_nfru(n, C.c_double(1.0))
         per = (C.c_void_p * 200)(); avg = (C.c_void_p * 200)()
         L.open_fru_props(2, per); L.open_fru_props_avg(2, avg)
         tot = L.open_fru_props_avgtot(2)
