@@ -14,9 +14,10 @@ stimulus, the Ca release and diastole.
 Prints one JSON line.  ``roofline`` is measured for the dominant kernel (the
 FRU Monte Carlo sub-interval kernel with the integrator stage in its tail) from
 HIP events recorded around every launch on the library's stream;
-``cpu_baseline`` times the CPU oracle (a single-threaded restatement of the
-reference, bit-exact with it) on a bounded sample of the same workload on this
-host.  Two further measurements share the line:
+``cpu_baseline`` times the reference's own MPI program (oracle/_ref/stoch,
+built from its unmodified sources) on this host's cores over the same simulated
+window, at its compiled 1,000-FRU size scaled to 12,500 FRUs (the CPU oracle,
+single-threaded, where that program is not built).  Two further measurements share the line:
 
 * ``mixed``: the same beat with the mixed-precision form (FP32 rates and local
   Ca right-hand side), and its speed relative to FP64;
@@ -68,7 +69,8 @@ def parse():
     ap.add_argument("--driver", choices=["device", "host"], default="device",
                     help="integrator on the GPU (default) or on the host calling the FRU ABI")
     ap.add_argument("--cpu-ms", type=float, default=3.0,
-                    help="simulated ms of the CPU-oracle sample (0 disables)")
+                    help="0 disables the CPU baseline; otherwise the simulated ms of the CPU-oracle "
+                         "sample used when the reference's MPI program is not built")
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
@@ -249,12 +251,75 @@ def summarize(folder: str) -> None:
                 f"{sum(v[c]) / len(v[c]):.1f}" if v.get(c) else "" for c in names) + " |")
 
 
+def reference_cpu_baseline(t0: float, t1: float, ranks: int) -> dict | None:
+    """The reference's own CPU path: its whole MPI program (caf.c + rk_pd54m.c +
+    mpi_master.c / mpi_slave.c + the hot path, unmodified, built by
+    oracle/Makefile into oracle/_ref/stoch) under MPICH's mpiexec with `ranks`
+    processes on this host, started from ic/stable (written here from the
+    package's copy, exact to the bit), timed between the progress lines it
+    prints at simulated t0 and t1 (caf.c:152, one line per 1 ms output step).
+    Its configuration is compiled in: 1,000 FRUs, NFRU_scale 12.5 (parameters.h:
+    43-51), the same 1 Hz current clamp.  Returns None when the program or
+    mpiexec is not there."""
+    stoch = os.path.join(ROOT, "oracle", "_ref", "stoch")
+    mpiexec = shutil.which("mpiexec", path=os.pathsep.join(["/opt/conda/bin", os.environ.get("PATH", "")]))
+    stdbuf = shutil.which("stdbuf")
+    if not (os.path.exists(stoch) and mpiexec and stdbuf):
+        return None
+    import signal
+
+    from greenstein2002_amd.ic import write_ic_dir
+
+    with tempfile.TemporaryDirectory(prefix="grw_ref_") as run:
+        write_ic_dir(os.path.join(run, "ic", "stable"))
+        p = subprocess.Popen([mpiexec, "-n", str(ranks), stdbuf, "-oL", stoch], cwd=run,
+                             stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             start_new_session=True)
+        marks = {}
+        deadline = time.perf_counter() + 600.0
+        try:
+            for line in p.stdout:
+                tok = line.split()
+                if len(tok) == 4 and tok[2] == "1000":
+                    try:
+                        t = float(tok[0])
+                    except ValueError:
+                        continue
+                    marks[t] = time.perf_counter()
+                    if t >= t1 or time.perf_counter() > deadline:
+                        break
+        finally:
+            try:
+                os.killpg(p.pid, signal.SIGTERM)
+            except ProcessLookupError:
+                pass
+            try:
+                p.wait(timeout=30)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.wait()
+    if t0 not in marks or t1 not in marks:
+        return None
+    wall = marks[t1] - marks[t0]
+    rate = (t1 - t0) / wall
+    return dict(value=round(rate * 1000.0 / 12500.0, 5), unit="simulated ms per wall-second", cores=ranks,
+                kind="reference",
+                sample=f"the reference's MPI program (oracle/_ref/stoch: caf.c, rk_pd54m.c, mpi_master.c, "
+                       f"mpi_slave.c and the hot path, unmodified, gcc -O3, MPICH) on {ranks} ranks of this "
+                       f"host, its compiled 1,000-FRU cell, 1 Hz current clamp from ic/stable, timed over "
+                       f"t = {t0:g}..{t1:g} ms (the GPU line's window): {rate:.3f} simulated ms/s at 1,000 FRUs "
+                       f"in {wall:.2f} s, scaled by 1,000/12,500 to the 12,500-FRU cell (its run time is "
+                       f"per-FRU work: >= 95% in the hot path)",
+                value_1000fru=round(rate, 4), wall_s=round(wall, 3))
+
+
 def cpu_baseline(nfru: int, ms: float, seed: int):
     """The CPU oracle (oracle/libgrw_oracle.so, one thread) on the same cell
-    for the first `ms` simulated ms from ic/stable."""
+    for the first `ms` simulated ms from ic/stable; used when the reference's
+    program is not built."""
     import numpy as np
 
-    from greenstein2002_amd.ic import errweight, load_stable
+    from greenstein2002_amd.ic import load_stable
     from oracle.pyoracle import FRU_DTYPE, Oracle, OracleCell
 
     o = Oracle()
@@ -281,9 +346,17 @@ def cpu_baseline(nfru: int, ms: float, seed: int):
     steps = int(cell.c.n_fru_steps)
     cell.close()
     return dict(value=n / dt, unit="simulated ms per wall-second", cores=1, kind="port",
-                sample=f"{nfru}-FRU cell, t = 0..{n} ms from ic/stable (diastole before the "
-                       f"stimulus, the cheapest phase of the beat), CPU oracle single-threaded, "
+                sample=f"{nfru}-FRU cell, t = 0..{n} ms from ic/stable, CPU oracle single-threaded, "
                        f"{steps} FRU-steps in {dt:.1f} s = {dt / max(steps, 1) * 1e9:.0f} ns/FRU-step")
+
+
+def host_cores() -> int:
+    """Cores this process may use, capped at 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def run_cell(args, precision, timing):
@@ -652,9 +725,14 @@ def main():
             roof["traffic"] = tr["bytes_per_launch"]
             roof["traffic_detail"] = tr
     if args.cpu_ms > 0:
-        line["cpu_baseline"] = cpu_baseline(args.nfru, args.cpu_ms, args.seed)
-        line["cpu_baseline"]["value"] = round(line["cpu_baseline"]["value"], 5)
-        line["vs_cpu_baseline"] = round(value / line["cpu_baseline"]["value"], 2)
+        ref = None
+        if args.nfru == 12500 and args.precision == "fp64":
+            ref = reference_cpu_baseline(float(args.warmup), float(args.warmup + args.steps), host_cores())
+        if ref is None:
+            ref = cpu_baseline(args.nfru, args.cpu_ms, args.seed)
+            ref["value"] = round(ref["value"], 5)
+        line["cpu_baseline"] = ref
+        line["vs_cpu_baseline"] = round(value / ref["value"], 2)
     print(json.dumps(line), flush=True)
 
 

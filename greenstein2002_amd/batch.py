@@ -133,7 +133,9 @@ class Batch:
         check(self._L.grw_batch_run(self._h, _vp(ns), _vp(out), C.cast(st, C.c_void_p)))
         self.attempts += np.array([s.attempts for s in st], np.int64)
         self.rejects += np.array([s.rejects for s in st], np.int64)
-        self.t = self.t + ns * np.array([sp.protocol.time_step for sp in self.specs])
+        for k, sp in enumerate(self.specs):  # one addition of time_step per output step (caf.c:150)
+            for _ in range(int(ns[k])):
+                self.t[k] += sp.protocol.time_step
         self.state = out
         return out
 

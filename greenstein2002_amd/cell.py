@@ -195,8 +195,15 @@ class Cell:
                                    _vp(self.state), C.byref(h), _vp(states), _vp(cur), _vp(other),
                                    _vp(log) if log_cap else None, int(log_cap), C.byref(nlog),
                                    C.byref(st)))
-        t_out = self.t + self.protocol.time_step * np.arange(1, nsteps + 1)
-        self.t = self.t + nsteps * self.protocol.time_step
+        # the clock advances by adding time_step once per output step, as
+        # caf.c:150 and the integrators do, so a run split into pieces restarts
+        # each piece at the same time as an unbroken run reaches it
+        t_out = np.zeros(nsteps)
+        t = float(self.t)
+        for k in range(nsteps):
+            t += self.protocol.time_step
+            t_out[k] = t
+        self.t = t
         self.h = h.value
         return RunResult(t=t_out, states=states, currents=cur, other=other, log=log[: nlog.value],
                          stats=dict(attempts=st.attempts, rejects=st.rejects, forced=st.forced))

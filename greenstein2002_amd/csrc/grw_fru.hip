@@ -14,6 +14,7 @@
 // with every continuous and discrete state in registers; each lane draws from
 // its own stream, so lanes never wait on each other except at the flux
 // reduction, which is fused into the tail of the same launch.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -314,7 +315,13 @@ __device__ __forceinline__ bool reduce_range(double v[NV], const Dev& d, unsigne
         __hip_atomic_store(&d.part[(size_t)blk * kSlot + slot_off + k], s, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
       }
+    // s_waitcnt is an intrinsic without memory semantics: the compiler-only
+    // fences (they emit nothing) keep the compiler from moving the partial
+    // stores past it or the ticket before it; the wait itself orders them on
+    // the hardware.
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     last = (t == (unsigned)(nb + extra) - 1u);
   }
@@ -1698,7 +1705,11 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       }
       if (j == 64)
         __hip_atomic_store(pre + kPreE1, flux_exp(a.depf[0]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // complete the stores before this block arrives at the cell's ticket;
+      // the compiler-only fences keep them on this side of the wait
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
       __builtin_amdgcn_s_waitcnt(0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
     }
   } else {
     count_steps(d, simfru4_body<R>(a, d, acc, blk));
@@ -2311,16 +2322,21 @@ int check_err(grw_ctx* c) {
   return 0;
 }
 
+// Timed launches go through hipExtLaunchKernelGGL, whose start / stop events
+// take the dispatch's own begin and end timestamps: the kernel's execution
+// time, without the gap to the launch before it (what a profiler's kernel
+// trace reports), rather than the span between two event packets.
 template <class R, bool SIM, bool REDUCE>
 int launch_sim(grw_ctx* c, const SimArgs& a) {
-  if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->stream));
+  hipEvent_t e0 = c->timing ? c->ev0 : nullptr, e1 = c->timing ? c->ev1 : nullptr;
   if (SIM && c->cfg.lanes_per_fru == 4)
-    hipLaunchKernelGGL((k_simfru4<R, REDUCE>), dim3(c->grid_sim), dim3(c->block), 0, c->stream, a, c->d);
+    hipExtLaunchKernelGGL((k_simfru4<R, REDUCE>), dim3(c->grid_sim), dim3(c->block), 0, c->stream, e0, e1, 0, a,
+                          c->d);
   else
-    hipLaunchKernelGGL((k_simfru<R, SIM, REDUCE>), dim3(c->grid), dim3(c->block), 0, c->stream, a, c->d);
+    hipExtLaunchKernelGGL((k_simfru<R, SIM, REDUCE>), dim3(c->grid), dim3(c->block), 0, c->stream, e0, e1, 0, a,
+                          c->d);
   HIPCHK(hipGetLastError());
   if (c->timing) {
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -2467,17 +2483,17 @@ int reset_order(grw_ctx* c) {
 }
 
 // One RK attempt's launches: five k_dev_stage, then a batch's regrouping.
-// ek (may be null) holds the ten events that bracket the five launches.
+// ek (may be null) holds ten events, the start and stop of each of the five
+// launches (hipExtLaunchKernelGGL: the dispatch's own begin / end timestamps).
 void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent_t* ek) {
   for (int s5 = 0; s5 < 5; ++s5) {
-    if (ek) (void)hipEventRecord(ek[2 * s5], c->stream);
+    hipEvent_t e0 = ek ? ek[2 * s5] : nullptr, e1 = ek ? ek[2 * s5 + 1] : nullptr;
     if (mixed)
-      hipLaunchKernelGGL(k_dev_stage<float>, dim3(c->ncells + c->grid_s), dim3(256), 0, c->stream, base, c->d,
-                         c->dctl);
+      hipExtLaunchKernelGGL(k_dev_stage<float>, dim3(c->ncells + c->grid_s), dim3(256), 0, c->stream, e0, e1, 0,
+                            base, c->d, c->dctl);
     else
-      hipLaunchKernelGGL(k_dev_stage<double>, dim3(c->ncells + c->grid_s), dim3(256), 0, c->stream, base, c->d,
-                         c->dctl);
-    if (ek) (void)hipEventRecord(ek[2 * s5 + 1], c->stream);
+      hipExtLaunchKernelGGL(k_dev_stage<double>, dim3(c->ncells + c->grid_s), dim3(256), 0, c->stream, e0, e1, 0,
+                            base, c->d, c->dctl);
   }
   if (base.sorted) sort_and_order(c);
 }
@@ -3116,6 +3132,7 @@ int grw_batch_create(const grw_batch_config* cfg, const grw_cell_spec* spec, grw
 int grw_batch_load_ic(grw_ctx* c, const double* states, const double errw[37], int nic, const double* fru,
                       const int32_t* ltype, const int32_t* ryr, const int32_t* ito2) {
   if (!c || !states || !errw || !fru || !ltype || !ryr || !ito2) return fail(GRW_EINVAL, "null argument");
+  if (!c->batch) return fail(GRW_EINVAL, "grw_batch_load_ic needs a batch context (grw_batch_create)");
   if (nic <= 0) return fail(GRW_EINVAL, "empty initial FRU set");
   std::vector<double> icc(5 * (size_t)nic);
   std::vector<uint32_t> ich(4 * (size_t)nic);
@@ -3175,6 +3192,7 @@ int grw_batch_load_ic(grw_ctx* c, const double* states, const double errw[37], i
 
 int grw_batch_run(grw_ctx* c, const int32_t* nsteps, double* states, GrwRkStats* stt) {
   if (!c || !nsteps) return fail(GRW_EINVAL, "null argument");
+  if (!c->batch) return fail(GRW_EINVAL, "grw_batch_run needs a batch context (grw_batch_create)");
   for (int k = 0; k < c->ncells; ++k) {
     if (nsteps[k] < 0) return fail(GRW_EINVAL, "negative step count for cell " + std::to_string(k));
     RkCtl& hc = c->hctl[k];

@@ -31,9 +31,7 @@ def _tokens(path: str):
 
 def read_ic_dir(path: str, nfru: int, names=None) -> dict:
     """Read ic_{states,FRU,LCh,RyR,Ito2}_*.txt from ``path`` for ``nfru`` FRUs."""
-    names = names or dict(states="ic_states_stable.txt", fru="ic_FRU_stable.txt",
-                          lch="ic_LCh_stable.txt", ryr="ic_RyR_stable.txt",
-                          ito2="ic_Ito2_stable.txt")
+    names = names or IC_NAMES
     rew = min(nfru, REWIND)
 
     def records(name, per, conv):
@@ -53,6 +51,43 @@ def read_ic_dir(path: str, nfru: int, names=None) -> dict:
     tok = _tokens(os.path.join(path, names["states"]))
     out["state"] = np.array([float(t) for t in tok[:37]])
     return out
+
+
+IC_NAMES = dict(states="ic_states_stable.txt", fru="ic_FRU_stable.txt", lch="ic_LCh_stable.txt",
+                ryr="ic_RyR_stable.txt", ito2="ic_Ito2_stable.txt")
+
+
+def write_ic_dir(path: str, ic: dict | None = None, names=None) -> list[str]:
+    """Write an initial-condition set in the reference's ic/ text formats (the
+    files initialize_state.c:43-133 reads: the 37 global states, then a record
+    count line and per FRU the 5 concentrations, 4 x "LCC gate", 4 x 5 RyR
+    states and 4 Ito2 states).  Doubles are written with %.17g, so the
+    reference's tokenizer (data_io.c:142-166, strtod) reads back the same
+    values bit for bit.  ``ic`` defaults to the packaged ic/stable set."""
+    if ic is None:
+        d = np.load(STABLE_NPZ)
+        ic = {k: d[k] for k in ("state", "fru", "ltype", "ryr", "ito2")}
+    names = names or IC_NAMES
+    os.makedirs(path, exist_ok=True)
+    fru = np.asarray(ic["fru"], np.float64).reshape(-1, 5)
+    n = len(fru)
+    lt = np.asarray(ic["ltype"]).reshape(n, 4, 2)
+    ryr = np.asarray(ic["ryr"]).reshape(n, 4, 5)
+    ito2 = np.asarray(ic["ito2"]).reshape(n, 4)
+    body = dict(
+        states="".join("%.17g\n" % v for v in np.asarray(ic["state"], np.float64).reshape(-1)),
+        fru="%d\n" % n + "".join(" ".join("%.17g" % v for v in row) + "\n" for row in fru),
+        lch="%d\n" % n + "".join("%d %d\n" % (a, b) for a, b in lt.reshape(-1, 2)),
+        ryr="%d\n" % n + "".join(" ".join("%d" % v for v in row) + "\n" for row in ryr.reshape(-1, 5)),
+        ito2="%d\n" % n + "".join("%d\n" % v for v in ito2.reshape(-1)),
+    )
+    paths = []
+    for key, text in body.items():
+        p = os.path.join(path, names[key])
+        with open(p, "w") as f:
+            f.write(text)
+        paths.append(p)
+    return paths
 
 
 def read_seeds(path: str, nfru: int, word_bytes: int = 8) -> dict:

@@ -165,3 +165,40 @@ def test_full_size_beat_matches_cpu_oracle():
     cols = [0, 1, 2, 4, 5, 7, 8, 9, 10]  # t_stim, V_rest, V_peak, APD90, Cai_peak, Cai_rest, ICa_peak, loads
     np.testing.assert_allclose(got[:, :, cols], ref[:, :, cols], rtol=1e-7, atol=0)
     assert np.array_equal(got[:, :, 11], ref[:, :, 11])  # the same number of accepted steps
+
+
+def test_batch_calls_reject_a_single_cell_context():
+    """grw_batch_load_ic / grw_batch_run on a context from grw_create fail with
+    GRW_EINVAL and say why, instead of running with an empty protocol."""
+    import ctypes as C
+
+    from greenstein2002_amd import _lib
+    from greenstein2002_amd.batch import _declare
+
+    L = _lib.load()
+    _declare(L)
+    with Cell(64, seed=1) as c:
+        ns = np.ones(1, np.int32)
+        rc = L.grw_batch_run(c.fru._h, ns.ctypes.data_as(C.c_void_p), None, None)
+        assert rc == -1 and b"batch context" in L.grw_last_error()
+        z = np.zeros(64)
+        rc = L.grw_batch_load_ic(c.fru._h, z.ctypes.data_as(C.c_void_p), z.ctypes.data_as(C.c_void_p), 1,
+                                 z.ctypes.data_as(C.c_void_p), z.ctypes.data_as(C.c_void_p),
+                                 z.ctypes.data_as(C.c_void_p), z.ctypes.data_as(C.c_void_p))
+        assert rc == -1 and b"batch context" in L.grw_last_error()
+
+
+def test_piecewise_run_equals_unbroken_run_with_fractional_time_step():
+    """time_step = 0.1 ms: a run in pieces of 7, 13 and 20 output steps equals
+    one unbroken run of 40 bit for bit (the clock adds time_step per step)."""
+    prot = Protocol(time_step=0.1, shift=2.0)
+    with Cell(128, seed=21, protocol=prot) as a:
+        parts = [a.run(n, log_cap=0) for n in (7, 13, 20)]
+        ta = a.t
+    with Cell(128, seed=21, protocol=prot) as b:
+        whole = b.run(40, log_cap=0)
+        tb = b.t
+    assert ta == tb
+    assert np.array_equal(np.concatenate([p.t for p in parts]), whole.t)
+    assert np.array_equal(np.concatenate([p.states for p in parts]), whole.states)
+
