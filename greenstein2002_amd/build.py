@@ -3,8 +3,9 @@
     python -m greenstein2002_amd.build
 
 Compiles csrc/*.hip into greenstein2002_amd/libgrw_fru.so with hipcc for
-gfx950, and the CPU checker libraries under oracle/ (oracle/Makefile; the
-reference build there happens only where the reference tree exists).
+gfx950, csrc/grw_mpi.cpp into libgrw_mpi.so (the reference's MPI-layer names
+over it), and the CPU checker libraries under oracle/ (oracle/Makefile; the
+reference builds there happen only where the reference tree exists).
 """
 from __future__ import annotations
 
@@ -56,6 +57,25 @@ def build_native(verbose: bool = False) -> str:
     return LIB
 
 
+MPI_SHIM = os.path.join(HERE, "libgrw_mpi.so")
+
+
+def build_mpi_shim(verbose: bool = False) -> str:
+    """libgrw_mpi.so: the reference's MPI-layer entry points (include/grw_mpi.h)
+    over libgrw_fru.so; host code only, found next to it through $ORIGIN."""
+    tmp = MPI_SHIM + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", tmp,
+           os.path.join(CSRC, "grw_mpi.cpp"), "-L" + HERE, "-l:libgrw_fru.so", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed on grw_mpi.cpp")
+    os.replace(tmp, MPI_SHIM)
+    return MPI_SHIM
+
+
 def build_oracle(verbose: bool = False) -> None:
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], capture_output=True, text=True)
     if verbose and r.stdout:
@@ -67,6 +87,7 @@ def build_oracle(verbose: bool = False) -> None:
 
 def build_all(verbose: bool = False) -> None:
     build_native(verbose)
+    build_mpi_shim(verbose)
     build_oracle(verbose)
 
 

@@ -865,10 +865,17 @@ void orc_cell_free(orc_cell *c) {
 
 void orc_distrib_simfru(orc_cell *c, double t0, double t1, const double dep0[3],
                         const double depf[3], double out4[4]) {
+  /* FRUs are independent within a sub-interval (mpi_master.c:409-421 and the
+   * slaves each run their own share): with OpenMP they run on the host's
+   * threads, and the sums below stay in sequential FRU order, so the result
+   * is the same for any thread count. */
+  int64_t steps = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : steps)
   for (int k = 0; k < c->nfru; ++k) {
     int n = orc_simfru(t0, t1, dep0, depf, &c->fru[k], &c->prot, 0, 0);
-    if (n > 0) c->n_fru_steps += n;
+    if (n > 0) steps += n;
   }
+  c->n_fru_steps += steps;
   double num[5];
   orc_flux_sums(c->nfru, c->fru, num);
   orc_flux_currents(num, depf, c->nfru, c->scale, out4); /* V from depf: :401-407 */

@@ -173,3 +173,52 @@ def test_beat_stats_closed_form():
     assert rec[1][0] == 110.0 and np.isnan(rec[1][4])
     # without a release-flux column the release fields are NaN
     assert np.isnan(beat_stats(log, rows_t, states, other, shift, period)[0][13])
+
+
+def test_mpi_shim_exports_the_reference_mpi_layer():
+    """libgrw_mpi.so exports mpi_master.c's nine entry points and the slave loop
+    caf.c names (include/grw_mpi.h), and nothing else."""
+    import re
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "grw_mpi.h")).read()
+    declared = set(re.findall(r"^void (\w+)\(", hdr, re.M))
+    assert declared == {"initialize_mpi", "end_mpi", "parallel", "initialize_mpi_state",
+                        "parallel_get_FRUs", "distrib_simFRU", "send_calc_fru_flux",
+                        "send_calc_fru_avg", "send_save_state", "send_resume_state"}
+    so = os.path.join(root, "greenstein2002_amd", "libgrw_mpi.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", so], capture_output=True, text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert exported == declared
+    import ctypes as C
+    L = C.CDLL(so)  # loads without the program's NFRU_sim / NFRU_scale (weak references)
+    for name in declared:
+        assert getattr(L, name)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                                    "oracle", "_ref", "stoch_grw")),
+                    reason="oracle/_ref/stoch_grw not built (needs the reference tree)")
+def test_reference_driver_links_against_the_shim():
+    """oracle/_ref/stoch_grw -- the reference's caf.c, rk_pd54m.c, fcn.c, IC and
+    output code -- takes the MPI layer's entry points from libgrw_mpi.so, has
+    no MPI library and none of the reference's FRU kernels."""
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "oracle", "_ref", "stoch_grw")
+    und = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True, text=True).stdout
+    need = {ln.split()[-1] for ln in und.splitlines()}
+    for name in ("initialize_mpi", "initialize_mpi_state", "distrib_simFRU", "send_calc_fru_flux",
+                 "send_calc_fru_avg", "send_save_state", "send_resume_state", "parallel_get_FRUs",
+                 "end_mpi", "parallel"):
+        assert name in need, name
+    syms = subprocess.run(["nm", exe], capture_output=True, text=True).stdout
+    for kernel in ("simfru_local", "fru_rates_local", "fcn_fru", "calc_fru_flux_local", "MPI_Init"):
+        assert kernel not in syms, kernel
+    needed = lambda f: re.findall(r"NEEDED.*\[(.*)\]", subprocess.run(  # noqa: E731
+        ["readelf", "-d", f], capture_output=True, text=True).stdout)
+    assert "libgrw_mpi.so" in needed(exe) and not any("libmpi" in n for n in needed(exe))
+    assert "libgrw_fru.so" in needed(os.path.join(root, "greenstein2002_amd", "libgrw_mpi.so"))
+
