@@ -893,8 +893,16 @@ void orc_send_calc_fru_avg(orc_cell *c, const double state[ORC_NGLOBAL], double 
   orc_avg_outputs(num, state, c->nfru, c->scale, other);
 }
 
-void orc_save_state(orc_cell *c) { memcpy(c->hold, c->fru, sizeof(orc_fru) * (size_t)c->nfru); }
-void orc_resume_state(orc_cell *c) { memcpy(c->fru, c->hold, sizeof(orc_fru) * (size_t)c->nfru); }
+/* mpi_slave.c:150-226: every process copies its own FRUs; here the copies
+ * are split over the OpenMP threads the same way. */
+void orc_save_state(orc_cell *c) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < c->nfru; ++k) c->hold[k] = c->fru[k];
+}
+void orc_resume_state(orc_cell *c) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < c->nfru; ++k) c->fru[k] = c->hold[k];
+}
 
 static void orc_ev(orc_cell *c, int kind, double t0, double t1, const double dep[3],
                    const double out[4]) {
