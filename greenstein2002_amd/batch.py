@@ -28,7 +28,8 @@ class GrwBatchConfig(C.Structure):
 
 class GrwCellSpec(C.Structure):
     _fields_ = [("nfru", C.c_int32), ("seed", C.c_uint32), ("nfru_scale", C.c_double),
-                ("k23_scale", C.c_double), ("gamma_scale", C.c_double), ("prot", GrwProtocol)]
+                ("k23_scale", C.c_double), ("gamma_scale", C.c_double), ("prot", GrwProtocol),
+                ("t0", C.c_double)]
 
 
 @dataclass
@@ -39,6 +40,7 @@ class CellSpec:
     k23_scale: float = 1.0            # RyR opening rate (fru_rates_local.c:103)
     gamma_scale: float = 1.0          # LCC mode switch (fru_rates_local.c:70)
     protocol: Protocol = field(default_factory=Protocol)
+    t0: float = 0.0                   # the cell's start time (ms)
 
     @property
     def scale(self) -> float:
@@ -64,7 +66,7 @@ def _vp(a):
 
 
 class Batch:
-    """Cells ``specs`` on one GPU, started from ic/stable at t = 0."""
+    """Cells ``specs`` on one GPU, each started from ic/stable at its t0."""
 
     def __init__(self, specs, precision: str = "fp64", beat_cap: int = 64, device: int = 0):
         if precision not in ("fp64", "mixed"):
@@ -84,12 +86,13 @@ class Batch:
             arr[k].k23_scale = float(sp.k23_scale)
             arr[k].gamma_scale = float(sp.gamma_scale)
             arr[k].prot = sp.protocol.to_c()
+            arr[k].t0 = float(sp.t0)
         h = C.c_void_p()
         check(L.grw_batch_create(C.byref(cfg), C.cast(arr, C.c_void_p), C.byref(h)))
         self._h = h
         self.ncells = n
         self.beat_cap = beat_cap
-        self.t = np.zeros(n)
+        self.t = np.array([float(sp.t0) for sp in self.specs])
         self.attempts = np.zeros(n, np.int64)
         self.rejects = np.zeros(n, np.int64)
         base = load_stable(REWIND)

@@ -10,7 +10,7 @@ coarse for APD90 or the ICaL peak, and has no such reduction.
 
 Record fields (``FIELDS``): beat b starts at the stimulus t_s = shift + b period
 (parameters.h:114-117) and holds the accepted steps ending in [t_s, t_s + period);
-under voltage clamp the clamp period is used (vclamp_shift, vclamp_period).
+under a voltage-clamp protocol its periods are used (``beat_window``).
 Ca_in (mM of myoplasm, -ICa Acap / (2 Vmyo F), fcn.c:1058) and Ca_rel (the
 whole-cell JRyR of send_calc_fru_avg integrated) are trapezoids over the
 accepted steps; Ca_rel / Ca_in is the EC-coupling gain.
@@ -27,6 +27,20 @@ FIELDS = ("t_stim", "V_rest", "V_peak", "t_V_peak", "APD90", "Cai_peak", "t_Cai_
           "Cai_rest", "ICa_peak", "CaNSR_rest", "CaJSR_rest", "steps", "Ca_in", "Ca_rel", "JRyR_peak")
 NSTAT = len(FIELDS)
 NAN = float("nan")
+
+
+def beat_window(prot) -> tuple[float, float]:
+    """(shift, period) of the beats of a run under ``prot``: the pacing period,
+    or the period of the voltage-clamp protocol in force (the I-V steps, the
+    clamp with a prepulse, the plain clamp, in that precedence, as the device
+    integrator's records use)."""
+    if prot.ivclamp:
+        return prot.iv_shift, prot.iv_period
+    if prot.vppclamp:
+        return prot.vpp_shift, prot.vpp_period
+    if prot.vclamp:
+        return prot.vclamp_shift, prot.vclamp_period
+    return prot.shift, prot.period
 
 
 def ca_influx(ica):

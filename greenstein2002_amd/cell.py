@@ -41,7 +41,26 @@ class GrwProtocol(C.Structure):
         ("step_min", C.c_double),
         ("step_max", C.c_double),
         ("tolrk", C.c_double),
+        ("vppclamp", C.c_int32),
+        ("ivclamp", C.c_int32),
+        ("vpp_ppset", C.c_double),
+        ("vpp_ppduration", C.c_double),
+        ("vpp_shift", C.c_double),
+        ("vpp_set", C.c_double),
+        ("vpp_duration", C.c_double),
+        ("vpp_hold", C.c_double),
+        ("vpp_period", C.c_double),
+        ("iv_duration", C.c_double),
+        ("iv_period", C.c_double),
+        ("iv_shift", C.c_double),
+        ("iv_set", C.c_double),
+        ("iv_hold", C.c_double),
+        ("iv_step", C.c_double),
+        ("iv_n", C.c_int32),
+        ("reserved", C.c_int32),
     ]
+
+_INT_FIELDS = ("vclamp", "chf", "vppclamp", "ivclamp", "iv_n", "reserved")
 
 
 class GrwRkStats(C.Structure):
@@ -95,12 +114,45 @@ class Protocol:
     step_min: float = 1.e-6
     step_max: float = 0.1
     tolrk: float = 1.e-6
+    # voltage clamp with a prepulse, parameters.h:128-137 (fcn.c:485-516)
+    vppclamp: bool = False
+    vpp_ppset: float = -40.0
+    vpp_ppduration: float = 100.0
+    vpp_shift: float = 10000.0
+    vpp_set: float = 0.0
+    vpp_duration: float = 300.0
+    vpp_hold: float = -80.0
+    vpp_period: float = 500.0
+    # voltage clamp for I-V studies, parameters.h:141-147 (fcn.c:521-535); the
+    # test potential is iv_set + iv_n * iv_step, iv_n the step number (caf.c:234)
+    ivclamp: bool = False
+    iv_duration: float = 50.0
+    iv_period: float = 200.0
+    iv_shift: float = 10.0
+    iv_set: float = -40.0
+    iv_hold: float = -100.0
+    iv_step: float = 10.0
+    iv_n: int = 0
+    reserved: int = 0
+
+    @property
+    def clamped(self) -> bool:
+        return bool(self.vclamp or self.vppclamp or self.ivclamp)
+
+    def initial_V(self, V: float) -> float:
+        """initialize_state.c:222-228: V starts at the holding potential under
+        the voltage clamp and under the I-V clamp."""
+        if self.vclamp:
+            V = self.vclamp_hold
+        if self.ivclamp:
+            V = self.iv_hold
+        return V
 
     def to_c(self) -> GrwProtocol:
         p = GrwProtocol()
         for name, _ in GrwProtocol._fields_:
             v = getattr(self, name)
-            setattr(p, name, int(v) if name in ("vclamp", "chf") else float(v))
+            setattr(p, name, int(v) if name in _INT_FIELDS else float(v))
         return p
 
 
@@ -140,18 +192,27 @@ class Cell:
         self.fru = FruCell(nfru, nfru_scale=scale, precision=precision, block=block,
                            sequential_sums=sequential_sums, k23_scale=k23_scale,
                            gamma_scale=gamma_scale, lanes_per_fru=lanes_per_fru)
-        ic = ic or load_stable(nfru, scale)
-        self.state = np.array(ic["state"], np.float64)
-        if self.protocol.vclamp:  # initialize_state.c:222-224
-            self.state[0] = self.protocol.vclamp_hold
+        self.ic = ic or load_stable(nfru, scale)
+        self.seed = seed
         self.errw = errweight()
-        self.fru.seed(seed)
-        st = self.fru.parallel_get_FRUs()
-        self.fru.initialize_mpi_state(ic["fru"], ic["ltype"], ic["ryr"], ic["ito2"], st["mti"], st["mt"])
-        self.t = 0.0
-        self.h = self.protocol.step_max
         self._L = _lib.load()
         _declare(self._L)
+        self.restart(0.0)
+
+    def restart(self, t: float | None = None) -> None:
+        """Back to the initial condition with the same MT19937 seeds, keeping
+        the clock (or setting it to t): initialize_ran, initialize_state and
+        initialize_mpi_state, as caf.c:232-244 does at every I-V step, with the
+        next step size step_max (caf.c:236)."""
+        ic = self.ic
+        self.state = np.array(ic["state"], np.float64)
+        self.state[0] = self.protocol.initial_V(self.state[0])  # initialize_state.c:222-228
+        self.fru.seed(self.seed)
+        st = self.fru.parallel_get_FRUs()
+        self.fru.initialize_mpi_state(ic["fru"], ic["ltype"], ic["ryr"], ic["ito2"], st["mti"], st["mt"])
+        if t is not None:
+            self.t = float(t)
+        self.h = self.protocol.step_max
 
     def close(self):
         self.fru.close()

@@ -38,6 +38,22 @@ void grw_protocol_default(GrwProtocol* p) {
   p->step_min = 1.e-6;
   p->step_max = 0.1;
   p->tolrk = 1.e-6;
+  p->vppclamp = 0;  // parameters.h:128-137
+  p->vpp_ppset = -40.0;
+  p->vpp_ppduration = 100.0;
+  p->vpp_shift = 10000.0;
+  p->vpp_set = 0.0;
+  p->vpp_duration = 300.0;
+  p->vpp_hold = -80.0;
+  p->vpp_period = 500.0;
+  p->ivclamp = 0;  // parameters.h:141-147
+  p->iv_duration = 50.0;
+  p->iv_period = 200.0;
+  p->iv_shift = 10.0;
+  p->iv_set = -40.0;
+  p->iv_hold = -100.0;
+  p->iv_step = 10.0;
+  p->iv_n = 0;
 }
 
 int grw_fcn(const GrwProtocol* p, double t, double state[37], double F[37], double cur[24],

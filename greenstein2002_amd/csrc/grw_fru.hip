@@ -1258,8 +1258,9 @@ __device__ __forceinline__ void fru_rollback(const SimArgs& a, const Dev& d, int
 //   11 accepted steps in the beat, 12 Ca entered through the LCCs and 13 Ca
 //   released by the RyRs over the beat (mM of myoplasm, trapezoids over the
 //   accepted steps, the first one from the step before t_s), 14 peak release
-//   flux (mM/ms).  Under voltage clamp the beats are the clamp periods
-//   (vclamp_shift, vclamp_period).
+//   flux (mM/ms).  Under a voltage-clamp protocol the beats are its periods
+//   (iv_shift / iv_period, vpp_shift / vpp_period or vclamp_shift /
+//   vclamp_period, the first of these protocols that is on).
 constexpr int kBeatStats = 15;
 // The part of a stage's fcn that does not depend on the release units'
 // fluxes, as the stage's fcn block leaves it for the tail: F[0..36] (the
@@ -1311,9 +1312,10 @@ struct BeatLocal {
 __device__ __forceinline__ BeatLocal beat_load(const RkCtl* c) {
   BeatLocal b;
   b.out_beats = c->out_beats;
-  const bool vc = c->prot.vclamp != 0;
-  b.shift = vc ? c->prot.vclamp_shift : c->prot.shift;
-  b.period = vc ? c->prot.vclamp_period : c->prot.period;
+  // beats are the pacing periods, or the periods of the clamp protocol in force
+  const GrwProtocol& pr = c->prot;
+  b.shift = pr.ivclamp ? pr.iv_shift : pr.vppclamp ? pr.vpp_shift : pr.vclamp ? pr.vclamp_shift : pr.shift;
+  b.period = pr.ivclamp ? pr.iv_period : pr.vppclamp ? pr.vpp_period : pr.vclamp ? pr.vclamp_period : pr.period;
   b.beat_cap = c->beat_cap;
   b.beat_n = c->beat_n;
   b.beat_cur = c->beat_cur;
@@ -1772,7 +1774,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   if (j == 0) {  // fcn's assembly with this launch's fluxes
     const double* m = pre + kPreAsm;
     const FcnAsm a{m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9], m[10], m[11], m[12]};
-    (void)fcn_asm_fin(a, yv, pre[kPreV], pre[kPreIstim], Jsh, pl.vclamp != 0, F);
+    (void)fcn_asm_fin(a, yv, pre[kPreV], pre[kPreIstim], Jsh, clamped(pl), F);
     for (int q = 0; q < 4; ++q) ctl->J[q] = Jsh[q];
   }
   __syncthreads();
@@ -2285,6 +2287,7 @@ struct grw_ctx {
   int ncells = 1;
   bool batch = false;
   std::vector<CellInfo> cells;
+  std::vector<double> t0s;      // each cell's start time (grw_cell_spec.t0)
   int grid_b = 0, grid_s = 0;   // integrator grids: one thread per FRU / 64 FRUs per block
   RkCtl* dctl = nullptr;        // [ncells] integrator control blocks, persistent across runs
   std::vector<RkCtl> hctl;      // host copies
@@ -3093,6 +3096,9 @@ int grw_batch_create(const grw_batch_config* cfg, const grw_cell_spec* spec, grw
       return fail(GRW_EINVAL, "cell " + std::to_string(k) + ": bad protocol time controls");
     if (!(sp.k23_scale > 0.0) || !(sp.gamma_scale > 0.0) || !(sp.nfru_scale >= 0.0) || !(p.serca_scale > 0.0))
       return fail(GRW_EINVAL, "cell " + std::to_string(k) + ": scales must be positive");
+    if (!std::isfinite(sp.t0) || (p.vppclamp && !(p.vpp_period > 0.0)) || (p.ivclamp && !(p.iv_period > 0.0)) ||
+        (p.vclamp && !(p.vclamp_period > 0.0)))
+      return fail(GRW_EINVAL, "cell " + std::to_string(k) + ": bad start time or clamp period");
     cells[k].nfru = sp.nfru;
     cells[k].scale = sp.nfru_scale > 0.0 ? sp.nfru_scale : 12500.0 / (double)sp.nfru;
     cells[k].k23c = sp.k23_scale != 1.0 ? k23cf * sp.k23_scale : k23cf;
@@ -3119,7 +3125,11 @@ int grw_batch_create(const grw_batch_config* cfg, const grw_cell_spec* spec, grw
     grw_destroy(c);
     return rc;
   }
-  for (int k = 0; k < c->ncells; ++k) c->hctl[k].prot = spec[k].prot;
+  c->t0s.resize(c->ncells);
+  for (int k = 0; k < c->ncells; ++k) {
+    c->hctl[k].prot = spec[k].prot;
+    c->t0s[k] = spec[k].t0;
+  }
   reset_tracking(c);
   if (int rc2 = reset_order(c)) {
     grw_destroy(c);
@@ -3177,9 +3187,10 @@ int grw_batch_load_ic(grw_ctx* c, const double* states, const double errw[37], i
       hc.errw[q] = errw[q];
       hc.state[q] = states[(size_t)k * kNglobal + q];
     }
-    if (p.vclamp) hc.state[0] = p.vclamp_hold;  // initialize_state.c:222-224
-    hc.tcall = hc.start = 0.0;
-    hc.tstep = p.time_step;
+    if (p.vclamp) hc.state[0] = p.vclamp_hold;  // initialize_state.c:222-228
+    if (p.ivclamp) hc.state[0] = p.iv_hold;
+    hc.tcall = hc.start = c->t0s[k];
+    hc.tstep = c->t0s[k] + p.time_step;
     hc.h = hc.oh = std::min(p.step_max, p.time_step);
     hc.success = 1;
     hc.keepc = 1;

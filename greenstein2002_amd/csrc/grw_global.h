@@ -98,18 +98,62 @@ GRW_HD double pow2(double x) {
 #endif
 }
 
-// The membrane potential the transcendental arguments use: the state's, or
-// the clamp's under voltage clamp (fcn.c:448-483).
-GRW_HD double fcn_prep_V(double t, const double* state, const GrwProtocol& p) {
-  double V = state[0];
+// fcn.c:448-535: the membrane potential under the voltage-clamp protocols --
+// the plain clamp (:448-483), the clamp with a prepulse (:485-516) and the I-V
+// steps (:521-535) -- in the reference's order, each overriding V when on.
+// div is the division the caller uses (IEEE either way).  Returns whether a
+// clamp is on, in which case fcn holds V: dV = 0, state[V] = V and the total
+// current leaves out the stimulus (fcn.c:811-838).
+template <class Div>
+GRW_HD bool clamp_V(double t, const GrwProtocol& p, double& V, Div div) {
   if (p.vclamp) {
-    const double von = floor(rdiv(t - p.vclamp_shift, p.vclamp_period)) * p.vclamp_period;
+    const double von = floor(div(t - p.vclamp_shift, p.vclamp_period)) * p.vclamp_period;
     const double voff = von + p.vclamp_duration;
     if (((t - p.vclamp_shift) >= von) && (von >= 0.0) && ((t - p.vclamp_shift) < voff))
       V = p.vclamp_set;
     else
       V = p.vclamp_hold;
   }
+  if (p.vppclamp) {
+    const double on = floor(div(t - p.vpp_shift, p.vpp_period)) * p.vpp_period;
+    const double ppoff = on + p.vpp_ppduration;
+    const double off = on + p.vpp_duration + p.vpp_ppduration;
+    const double ts = t - p.vpp_shift;
+    if ((ts >= on) && (on >= 0.0) && (ts < ppoff))
+      V = p.vpp_ppset;
+    else if ((ts >= ppoff) && (on >= 0.0) && (ts < off))
+      V = p.vpp_set;
+    else if ((ts < on) || (on < 0.0) || (ts >= off))
+      V = p.vpp_hold;
+  }
+  if (p.ivclamp) {
+    const double start = (floor(div(t + 1.e-20, p.iv_period))) * p.iv_period;
+    const double end = (1.0 + floor(div(t + 1.e-20, p.iv_period))) * p.iv_period;
+    const double on = p.iv_shift + start;
+    double off = on + p.iv_duration;
+    off = (end < off) ? end : off;  // min(period_end, time_iv_clamp_off)
+    if ((t >= on) && (t < off))
+      V = p.iv_set + p.iv_n * p.iv_step;
+    else
+      V = p.iv_hold;
+  }
+  return p.vclamp || p.vppclamp || p.ivclamp;
+}
+
+GRW_HD bool clamped(const GrwProtocol& p) { return p.vclamp || p.vppclamp || p.ivclamp; }
+
+struct DivRdiv {
+  GRW_HD double operator()(double a, double b) const { return rdiv(a, b); }
+};
+struct DivIeee {
+  GRW_HD double operator()(double a, double b) const { return a / b; }
+};
+
+// The membrane potential the transcendental arguments use: the state's, or
+// the clamp's under a voltage-clamp protocol.
+GRW_HD double fcn_prep_V(double t, const double* state, const GrwProtocol& p) {
+  double V = state[0];
+  (void)clamp_V(t, p, V, DivRdiv());
   return V;
 }
 
@@ -226,14 +270,7 @@ GRW_HD void fcn_voltage(double t, const double* state, const GrwProtocol& p, dou
   Istim = 0.0;
   if ((t - p.shift) >= ton && (t - p.shift) <= toff) Istim = Istim + p.pulse_amplitude;
   V = state[0];
-  if (p.vclamp) {  // :448-483
-    const double von = floor((t - p.vclamp_shift) / p.vclamp_period) * p.vclamp_period;
-    const double voff = von + p.vclamp_duration;
-    if (((t - p.vclamp_shift) >= von) && (von >= 0.0) && ((t - p.vclamp_shift) < voff))
-      V = p.vclamp_set;
-    else
-      V = p.vclamp_hold;
-  }
+  (void)clamp_V(t, p, V, DivIeee());  // :448-535
 }
 
 struct FcnCurrents {
@@ -507,8 +544,8 @@ GRW_HD FcnAsm fcn_asm_pre(FcnCurrents c, FcnCa ca, double Istim, bool chf) {
 }
 
 // The rest with the fluxes J = {Jxfer, Jtr, ICa, Ito2}: F[0], F[4..7], F[31];
-// returns Itot.  Under voltage clamp state[0] takes the clamp potential
-// (fcn.c:811-814).
+// returns Itot.  Under a voltage-clamp protocol state[0] takes the clamp
+// potential (fcn.c:811-814, 825-838).
 GRW_HD double fcn_asm_fin(const FcnAsm& s, double* state, double V, double Istim, const double J[4], bool vclamp,
                           double* F) {
   const double Jxfer = J[0], Jtr = J[1], ICa = J[2], Ito2 = J[3];
@@ -535,7 +572,7 @@ GRW_HD double fcn_asm_fin(const FcnAsm& s, double* state, double V, double Istim
 // F[31] and the currents record (fcn.c:780-1071).
 GRW_HD void fcn_assemble(double* state, double V, double Istim, FcnCurrents c, FcnCa ca, double* F,
                          double* cur, bool keepc, const double J[4], const GrwProtocol& p) {
-  const double Itot = fcn_asm_fin(fcn_asm_pre(c, ca, Istim, p.chf != 0), state, V, Istim, J, p.vclamp != 0, F);
+  const double Itot = fcn_asm_fin(fcn_asm_pre(c, ca, Istim, p.chf != 0), state, V, Istim, J, clamped(p), F);
   if (keepc && cur) {  // :1046-1071
     const double Jxfer = J[0], Jtr = J[1], ICa = J[2], Ito2 = J[3];
     double Ito1;

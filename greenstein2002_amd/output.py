@@ -309,9 +309,10 @@ def run_caf(cell, time_end: float, output_dir: str = "./", time_start: float = 0
     under the next file number unless the run is within a step of its end
     (caf.c:196-228).  After the loop the currents of the final state
     (lastcall, fcn.c:1077-1092) go to currents.N under time_end (caf.c:254-255).
-    The IV-curve restarts (iv_flag, caf.c:232-244) and the FRU-count switching
-    (dynamicFRU, caf.c:246-250) are off in the reference's configuration
-    (parameters.h:141, 43-46) and not offered."""
+    Under the I-V clamp (protocol.ivclamp) every iv_period of simulated time
+    the cell restarts from its initial condition and seeds with the next test
+    potential (iv_n + 1), the clock running on (caf.c:232-244); a
+    ``dynamic.DynamicCell`` then decides on its FRU count (caf.c:246-250)."""
     from .cell import fcn as host_fcn
 
     step = cell.protocol.time_step
@@ -324,6 +325,7 @@ def run_caf(cell, time_end: float, output_dir: str = "./", time_start: float = 0
     dumps = []
     calls = 0
     time_now = time_start
+    iv_time = 0.0
     try:
         while time_now <= time_end - .0001 * step:
             n_restart = n_restart + 1
@@ -360,6 +362,13 @@ def run_caf(cell, time_end: float, output_dir: str = "./", time_start: float = 0
                     of = open_otherstates(output_dir, filenumber)
                     if fru_props:
                         fp = FruPropsFiles(output_dir, filenumber, fru_start, fru_end, ts_sec)
+            iv_time = iv_time + step
+            if cell.protocol.ivclamp and iv_time >= cell.protocol.iv_period:  # caf.c:232-244
+                iv_time = 0.0
+                cell.protocol.iv_n = cell.protocol.iv_n + 1
+                cell.restart()
+            if hasattr(cell, "dynamic_step"):  # dynamicFRU after the step (caf.c:246-250)
+                cell.dynamic_step(tstep, state, other)
             time_now += step
         # lastcall (fcn.c:1077-1092): fluxes of the final state, then fcn's currents
         st = np.array(cell.state, np.float64)

@@ -40,6 +40,7 @@ typedef struct {
   double k23_scale;    /* RyR opening-rate scale on k23cf (fru_rates_local.c:103); 1 = reference */
   double gamma_scale;  /* LCC mode-switch scale on gammacf (fru_rates_local.c:70); 1 = reference */
   GrwProtocol prot;    /* pacing, clamp, SERCA scale, integrator controls */
+  double t0;           /* the cell's start time (ms): grw_batch_load_ic sets its clock there */
 } grw_cell_spec;
 
 int grw_batch_create(const grw_batch_config *cfg, const grw_cell_spec *cells, grw_ctx **out);
@@ -50,7 +51,8 @@ int grw_batch_create(const grw_batch_config *cfg, const grw_cell_spec *cells, gr
  * (fru [nic][5], ltype [nic][4][2], ryr [nic][4][5], ito2 [nic][4], the layouts
  * of grw_load_state) that FRU j of every cell takes as record j mod nic
  * (initialize_state.c:38-41,62-65).  Seeds every cell's streams from its master
- * seed and restarts its clock at t = 0. */
+ * seed and restarts its clock at its spec's t0; under the voltage clamp or the
+ * I-V clamp V starts at the holding potential (initialize_state.c:222-228). */
 int grw_batch_load_ic(grw_ctx *ctx, const double *states, const double errw[37], int nic,
                       const double *fru, const int32_t *ltype, const int32_t *ryr,
                       const int32_t *ito2);

@@ -30,6 +30,18 @@ typedef struct {
   double serca_scale;
   /* integrator controls, parameters.h:95-98 */
   double time_step, step_min, step_max, tolrk;
+  /* voltage clamp with a prepulse, parameters.h:128-137 (vppclamp != 0
+     enables; fcn.c:485-516, 825-838) */
+  int32_t vppclamp;
+  /* voltage clamp for I-V studies, parameters.h:141-147 (ivclamp != 0
+     enables; fcn.c:521-535); the test potential is iv_set + iv_n * iv_step */
+  int32_t ivclamp;
+  double vpp_ppset, vpp_ppduration, vpp_shift, vpp_set, vpp_duration, vpp_hold, vpp_period;
+  double iv_duration, iv_period, iv_shift, iv_set, iv_hold, iv_step;
+  /* the I-V step number (caf.c:31,140,234: iv_n, counted up at every
+     iv_period restart) */
+  int32_t iv_n;
+  int32_t reserved;
 } GrwProtocol;
 
 /* Per-run counters of the integrator. */
@@ -94,8 +106,9 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
  *     both trapezoids over the accepted steps (the first from the step before
  *     t_s); 13 / 12 is the EC-coupling gain
  *  14 peak JRyR (mM/ms)
- * Under voltage clamp the beats are the clamp periods: t_s = vclamp_shift +
- * b * vclamp_period.  Writes up to cap finished beats (and, if
+ * Under a voltage-clamp protocol the beats are its periods: t_s = vclamp_shift
+ * + b * vclamp_period (vpp_shift / vpp_period with the prepulse, iv_shift /
+ * iv_period for the I-V steps, which take precedence in that order).  Writes up to cap finished beats (and, if
  * include_current, the beat in progress) to out and their count to *n. */
 #define GRW_BEAT_STATS 15
 int grw_get_beats(grw_ctx *ctx, double *out, int cap, int *n, int include_current);

@@ -526,6 +526,10 @@ void orc_protocol_default(orc_protocol *p) {
   p->vclamp_duration = 200.0; p->vclamp_set = 0.0; p->vclamp_shift = 300.0;
   p->vclamp_hold = -80.0; p->vclamp_period = 500.0;
   p->serca_scale = 1.0; p->k23_scale = 1.0; p->gamma_scale = 1.0;
+  p->vppclamp = 0; p->vpp_ppset = -40.0; p->vpp_ppduration = 100.0; p->vpp_shift = 10000.0;
+  p->vpp_set = 0.0; p->vpp_duration = 300.0; p->vpp_hold = -80.0; p->vpp_period = 500.0;
+  p->ivclamp = 0; p->iv_duration = 50.0; p->iv_period = 200.0; p->iv_shift = 10.0;
+  p->iv_set = -40.0; p->iv_hold = -100.0; p->iv_step = 10.0; p->iv_n = 0;
 }
 
 /* initialize_state.c:274-310 */
@@ -663,6 +667,28 @@ void orc_fcn(double t, double state[ORC_NGLOBAL], double F[ORC_NGLOBAL], double 
       V = p->vclamp_set;
     else
       V = p->vclamp_hold;
+  }
+  /* voltage clamp with a prepulse :485-516 */
+  if (p->vppclamp) {
+    const double on = floor((t - p->vpp_shift) / p->vpp_period) * p->vpp_period;
+    const double ppoff = on + p->vpp_ppduration;
+    const double off = on + p->vpp_duration + p->vpp_ppduration;
+    if (((t - p->vpp_shift) >= on) && (on >= 0.0) && ((t - p->vpp_shift) < ppoff)) {
+      V = p->vpp_ppset;
+    } else if (((t - p->vpp_shift) >= ppoff) && (on >= 0.0) && ((t - p->vpp_shift) < off)) {
+      V = p->vpp_set;
+    } else if (((t - p->vpp_shift) < on) || (on < 0.0) || ((t - p->vpp_shift) >= off)) {
+      V = p->vpp_hold;
+    }
+  }
+  /* voltage clamp for I-V relationship studies :521-535 */
+  if (p->ivclamp) {
+    const double pstart = (floor((t + 1.e-20) / p->iv_period)) * p->iv_period;
+    const double pend = (1.0 + floor((t + 1.e-20) / p->iv_period)) * p->iv_period;
+    const double on = p->iv_shift + pstart;
+    const double off = ref_min(pend, on + p->iv_duration);
+    if ((t >= on) && (t < off)) V = p->iv_set + p->iv_n * p->iv_step;
+    else V = p->iv_hold;
   }
 
   /* reversal potentials :549-559 */
@@ -806,7 +832,7 @@ void orc_fcn(double t, double state[ORC_NGLOBAL], double F[ORC_NGLOBAL], double 
   a2 = IK1 + IKp + INaCa + INaK + Ito1 + Ito2;
   a3 = IpCa + ICab + INab;
   double dV, Itot;
-  if (p->vclamp) {
+  if (p->vclamp || p->ivclamp || p->vppclamp) { /* :811-838 */
     dV = 0.0;
     state[0] = V;
     Itot = a1 + a2 + a3;

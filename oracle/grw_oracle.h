@@ -87,6 +87,12 @@ typedef struct {
   double serca_scale;     /* multiplies vmaxf and vmaxr (fcn.c:218-219); 1 = reference */
   double k23_scale;       /* multiplies k23cf (fru_rates_local.c:103); 1 = reference */
   double gamma_scale;     /* multiplies gammacf (fru_rates_local.c:70); 1 = reference */
+  int32_t vppclamp;       /* :128-137, clamp with a prepulse */
+  int32_t ivclamp;        /* :141-147, I-V steps */
+  double vpp_ppset, vpp_ppduration, vpp_shift, vpp_set, vpp_duration, vpp_hold, vpp_period;
+  double iv_duration, iv_period, iv_shift, iv_set, iv_hold, iv_step;
+  int32_t iv_n;           /* caf.c:31: the I-V step number */
+  int32_t pad;
 } orc_protocol;
 
 /* ---- MT19937 (mt19937_local.c) ---- */

@@ -84,6 +84,23 @@ PROT_DTYPE = np.dtype(
         ("serca_scale", "<f8"),
         ("k23_scale", "<f8"),
         ("gamma_scale", "<f8"),
+        ("vppclamp", "<i4"),
+        ("ivclamp", "<i4"),
+        ("vpp_ppset", "<f8"),
+        ("vpp_ppduration", "<f8"),
+        ("vpp_shift", "<f8"),
+        ("vpp_set", "<f8"),
+        ("vpp_duration", "<f8"),
+        ("vpp_hold", "<f8"),
+        ("vpp_period", "<f8"),
+        ("iv_duration", "<f8"),
+        ("iv_period", "<f8"),
+        ("iv_shift", "<f8"),
+        ("iv_set", "<f8"),
+        ("iv_hold", "<f8"),
+        ("iv_step", "<f8"),
+        ("iv_n", "<i4"),
+        ("pad", "<i4"),
     ],
     align=True,
 )

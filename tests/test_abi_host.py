@@ -96,6 +96,77 @@ def test_host_fcn_matches_oracle_under_clamp_and_chf(oracle):
             assert np.array_equal(F1, F2) and np.array_equal(c1, c2) and np.array_equal(s1, s2)
 
 
+def _clamp_closed_form(kind, t, p):
+    """The clamp potential as a piecewise function of the phase within the
+    protocol's period (fcn.c:448-535 read as intervals), or None outside."""
+    if kind == "vclamp":
+        x = t - p["vclamp_shift"]
+        if x < 0:
+            return p["vclamp_hold"]
+        ph = x - np.floor(x / p["vclamp_period"]) * p["vclamp_period"]
+        return p["vclamp_set"] if ph < p["vclamp_duration"] else p["vclamp_hold"]
+    if kind == "vpp":
+        x = t - p["vpp_shift"]
+        if x < 0:
+            return p["vpp_hold"]
+        ph = x - np.floor(x / p["vpp_period"]) * p["vpp_period"]
+        if ph < p["vpp_ppduration"]:
+            return p["vpp_ppset"]
+        if ph < p["vpp_ppduration"] + p["vpp_duration"]:
+            return p["vpp_set"]
+        return p["vpp_hold"]
+    k = np.floor(t / p["iv_period"])
+    ph = t - k * p["iv_period"]
+    on = ph >= p["iv_shift"] and ph < min(p["iv_period"], p["iv_shift"] + p["iv_duration"])
+    return p["iv_set"] + p["iv_n"] * p["iv_step"] if on else p["iv_hold"]
+
+
+@pytest.mark.parametrize("kind", ["vclamp", "vpp", "iv"])
+def test_clamp_protocols_hold_the_closed_form_potential(oracle, kind):
+    """The plain clamp, the clamp with a prepulse (fcn.c:485-516) and the I-V
+    steps (fcn.c:521-535): at times on a grid through several periods (away
+    from the switching instants by 1e-6 ms, and exactly on the integer ones),
+    the host fcn and the oracle hold V at the closed-form potential, with
+    dV = 0 (fcn.c:811-838); and the two agree bit for bit on every output."""
+    from greenstein2002_amd.cell import Protocol, fcn
+
+    params = dict(
+        vclamp=dict(vclamp=1, vclamp_shift=7.0, vclamp_period=60.0, vclamp_duration=25.0,
+                    vclamp_set=12.0, vclamp_hold=-80.0),
+        vpp=dict(vppclamp=1, vpp_shift=5.0, vpp_period=90.0, vpp_ppduration=20.0, vpp_duration=30.0,
+                 vpp_ppset=-40.0, vpp_set=15.0, vpp_hold=-85.0),
+        iv=dict(ivclamp=1, iv_period=70.0, iv_shift=10.0, iv_duration=80.0, iv_set=-40.0,
+                iv_hold=-100.0, iv_step=10.0, iv_n=3),
+    )[kind]
+    prot = Protocol(**{k: (bool(v) if k in ("vclamp", "vppclamp", "ivclamp") else v) for k, v in params.items()})
+    op = oracle.protocol(**params)
+    pc = {f: getattr(prot, f) for f in prot.__dataclass_fields__}
+    st0 = np.load(os.path.join(ROOT, "tests", "golden", "ic_stable.npz"))["state"]
+    rng = np.random.default_rng(3)
+    ts = np.concatenate([np.arange(0.0, 400.0, 1.0), np.arange(0.5e-6, 400.0, 2.5) + 1e-6,
+                         rng.uniform(0.0, 400.0, 200)])
+    levels = set()
+    for t in ts:
+        s = st0 * rng.uniform(0.97, 1.03, 37)
+        J = rng.normal(size=4)
+        F1, c1, s1 = fcn(t, s, J, prot)
+        F2, c2, s2 = oracle.fcn(t, s, J, prot=op)
+        assert np.array_equal(F1, F2) and np.array_equal(c1, c2) and np.array_equal(s1, s2), t
+        want = _clamp_closed_form(kind, t, pc)
+        assert s1[0] == want, (t, s1[0], want)
+        assert F1[0] == 0.0
+        levels.add(want)
+    assert len(levels) == (2 if kind != "vpp" else 3)  # every phase of the protocol was visited
+    # the I-V test potential steps with iv_n
+    if kind == "iv":
+        prot.iv_n = 5
+        _, _, s5 = fcn(25.0, st0, np.zeros(4), prot)
+        assert s5[0] == -40.0 + 5 * 10.0
+    # without a clamp V is the state's and the stimulus drives dV
+    F0, _, s0 = fcn(10.5, st0, np.zeros(4), Protocol())
+    assert s0[0] == st0[0] and F0[0] != 0.0
+
+
 def test_ic_reader_and_packaged_ic():
     """The text reader (initialize_state.c:43-133 rules) on the reference's files
     gives the packaged ic/stable data; records repeat every 1250 FRUs."""

@@ -109,3 +109,105 @@ def test_mixed_cell_action_potential():
     assert np.all(np.isfinite(res.states))
     assert res.states[:, 0].max() > 0.0
     assert res.states[:, 6].max() > 2e-4
+
+
+def _closed_vpp(t, p):
+    x = t - p.vpp_shift
+    if x < 0:
+        return p.vpp_hold
+    ph = x - np.floor(x / p.vpp_period) * p.vpp_period
+    if ph < p.vpp_ppduration:
+        return p.vpp_ppset
+    return p.vpp_set if ph < p.vpp_ppduration + p.vpp_duration else p.vpp_hold
+
+
+def test_prepulse_clamp_on_device_follows_host_driver():
+    """Voltage clamp with a prepulse (fcn.c:485-516): the device integrator
+    holds V at the closed-form potential at every output sample, and follows
+    the host integrator (which calls the FRU ABI per sub-interval) within the
+    round-off of the flux-sum order."""
+    prot = Protocol(vppclamp=True, vpp_shift=5.0, vpp_period=90.0, vpp_ppduration=20.0,
+                    vpp_duration=30.0, vpp_ppset=-40.0, vpp_set=20.0, vpp_hold=-85.0)
+    with Cell(300, seed=8, protocol=prot) as c:
+        rd = c.run(100, driver="device")
+    with Cell(300, seed=8, protocol=prot) as c:
+        rh = c.run(30, driver="host")
+    want = np.array([_closed_vpp(t, prot) for t in rd.t])
+    assert np.array_equal(rd.states[:, 0], want)
+    assert {-85.0, -40.0, 20.0} <= set(want.tolist())
+    np.testing.assert_allclose(rd.states[:30], rh.states, rtol=1e-8, atol=1e-13)
+    assert rd.currents[30:50, 11].min() < -0.5  # ICa flows during the step to +20 mV
+
+
+def test_iv_steps_in_one_batch_equal_the_serial_restart_loop():
+    """I-V clamp (fcn.c:521-535) the reference's way -- one cell restarted from
+    its initial condition and seeds every iv_period with the next test
+    potential (caf.c:232-244, output.run_caf) -- against the MI355X way: every
+    step its own cell of one batch, cell k starting at t0 = k iv_period with
+    iv_n = k.  Each batch cell ends where the serial run ends that step (within
+    the flux-sum round-off: single-cell and batch contexts sum in different
+    orders), and equals itself run alone as a one-cell batch bit for bit."""
+    from greenstein2002_amd.batch import Batch, CellSpec
+
+    P, n, nfru = 40.0, 3, 200
+    base = dict(ivclamp=True, iv_period=P, iv_shift=5.0, iv_duration=20.0, iv_set=-30.0,
+                iv_step=30.0, iv_hold=-90.0)
+    serial = []
+    with Cell(nfru, seed=31, protocol=Protocol(**base)) as c:
+        for k in range(n):
+            r = c.run(int(P), log_cap=0)
+            serial.append(r.states[-1].copy())
+            if k == 0:
+                V0 = r.states[:, 0]
+            c.protocol.iv_n += 1
+            c.restart()
+    # V of step 0: hold, then the test potential on [5, 25), then hold
+    assert np.all(V0[:4] == -90.0) and np.all(V0[4:24] == -30.0) and np.all(V0[24:] == -90.0)
+    specs = [CellSpec(nfru, seed=31, protocol=Protocol(**base, iv_n=k), t0=k * P) for k in range(n)]
+    with Batch(specs) as b:
+        b.run(int(P))
+        got = b.state.copy()
+        assert np.allclose(b.t, [(k + 1) * P for k in range(n)])
+    for k in range(n):
+        np.testing.assert_allclose(got[k], serial[k], rtol=1e-8, atol=1e-13, err_msg=f"step {k}")
+        with Batch([specs[k]]) as b1:
+            b1.run(int(P))
+            assert np.array_equal(b1.state[0], got[k]), k
+    # a higher test potential drives more Ca entry by the end of its step
+    assert not np.allclose(got[0], got[2])
+
+
+def test_dynamic_fru_number_switches_on_the_device(tmp_path):
+    """dynamicFRU (caf.c:246-250, dynamicFRU.c:26-152) with 250 low / 1,000 high
+    FRUs and the stimulus at 300 ms: the cell starts high, drops to the low
+    count at rest, returns to the high count 100 ms before the stimulus, and
+    fires.  Every decision equals phase_decision replayed on the run's own
+    outputs; a down switch keeps the whole-cell FRU Ca; run_caf drives it."""
+    from greenstein2002_amd.dynamic import HIGH, LOW, DynamicCell, phase_decision
+    from greenstein2002_amd.output import run_caf
+
+    prot = Protocol(shift=300.0)
+    rows = []
+    with DynamicCell(250, 1000, seed=5, protocol=prot) as dc:
+        for _ in range(330):
+            ph0 = dc.phase
+            r = dc.run(1, log_cap=0)
+            g0 = dc.fru.parallel_get_FRUs()["fru"] if ph0 == HIGH else None
+            changed = dc.dynamic_step(dc.t, r.states[0], r.other[0])
+            rows.append((dc.t, r.states[0, 0], r.other[0, 3], r.other[0, 7], ph0, changed))
+            if changed and ph0 == HIGH:  # down: the FRU Ca of the cell is unchanged
+                g1 = dc.fru.parallel_get_FRUs()["fru"]
+                np.testing.assert_allclose(g1.sum(0) * (12500 / 250), g0.sum(0) * (12500 / 1000), rtol=1e-12)
+        sw = list(dc.switches)
+        V = np.array([x[1] for x in rows])
+    ph, tl = HIGH, 0.0
+    for t, v, pr, pl, ph0, changed in rows:
+        assert ph == ph0
+        ph, tl, ch, _ = phase_decision(t, v, pr, pl, ph, tl, prot)
+        assert ch == changed, t
+    assert sw and sw[0][1] == LOW and sw[0][0] < 100.0, sw
+    assert any(s[1] == HIGH and 200.0 <= s[0] < 300.0 for s in sw), sw
+    assert np.all(np.isfinite(V)) and V[300:].max() > 0.0  # the action potential at 300 ms
+    with DynamicCell(250, 1000, seed=6, protocol=prot) as dc:
+        out = run_caf(dc, time_end=12.0, output_dir=str(tmp_path) + "/")
+        assert out.n_steps == 12 and dc.switches and dc.switches[0][1] == LOW
