@@ -132,10 +132,19 @@ def test_prepulse_clamp_on_device_follows_host_driver():
         rd = c.run(100, driver="device")
     with Cell(300, seed=8, protocol=prot) as c:
         rh = c.run(30, driver="host")
-    want = np.array([_closed_vpp(t, prot) for t in rd.t])
+    # an output sample holds the clamp potential at the start of its last
+    # accepted RK step (fcn writes it into the state there; dV = 0), so at a
+    # switching instant the sample still shows the old level: closed form at
+    # t - 0.05 ms (RK steps are at most 0.1 ms, the switches on the ms grid)
+    want = np.array([_closed_vpp(t - 0.05, prot) for t in rd.t])
     assert np.array_equal(rd.states[:, 0], want)
     assert {-85.0, -40.0, 20.0} <= set(want.tolist())
-    np.testing.assert_allclose(rd.states[:30], rh.states, rtol=1e-8, atol=1e-13)
+    # the two integrators take the same steps until an event decision flips on
+    # the last-ulp difference of the device's exp and the host's: within 30 ms
+    # of clamp steps one does (observed: 3e-6 relative by 20 ms), so the
+    # trajectories are compared to 1e-4 relative; a protocol error would move
+    # V by tens of mV and the currents with it
+    np.testing.assert_allclose(rd.states[:30], rh.states, rtol=1e-4, atol=1e-9)
     assert rd.currents[30:50, 11].min() < -0.5  # ICa flows during the step to +20 mV
 
 
@@ -161,8 +170,9 @@ def test_iv_steps_in_one_batch_equal_the_serial_restart_loop():
                 V0 = r.states[:, 0]
             c.protocol.iv_n += 1
             c.restart()
-    # V of step 0: hold, then the test potential on [5, 25), then hold
-    assert np.all(V0[:4] == -90.0) and np.all(V0[4:24] == -30.0) and np.all(V0[24:] == -90.0)
+    # V of step 0: hold, the test potential on [5, 25), hold (samples at t =
+    # 1, 2, ... show the potential at the start of their last RK step)
+    assert np.all(V0[:5] == -90.0) and np.all(V0[5:25] == -30.0) and np.all(V0[25:] == -90.0)
     specs = [CellSpec(nfru, seed=31, protocol=Protocol(**base, iv_n=k), t0=k * P) for k in range(n)]
     with Batch(specs) as b:
         b.run(int(P))
