@@ -32,6 +32,7 @@ EXPORTS = (
     "grw_simfru_local",
     "grw_calc_fru_flux_local",
     "grw_calc_fru_avg_local",
+    "grw_calc_fru_flux_local_noalloc",
     "grw_get_stats",
     "grw_reset_stats",
     "grw_set_timing",
@@ -99,6 +100,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.grw_simfru_local.argtypes = [vp, C.c_double, C.c_double, vp, vp]
     L.grw_calc_fru_flux_local.argtypes = [vp, vp]
     L.grw_calc_fru_avg_local.argtypes = [vp, vp]
+    L.grw_calc_fru_flux_local_noalloc.argtypes = [C.c_int, vp, vp, vp, vp, vp]
     L.grw_get_stats.argtypes = [vp, C.POINTER(GrwStats)]
     L.grw_reset_stats.argtypes = [vp]
     L.grw_set_timing.argtypes = [vp, C.c_int]

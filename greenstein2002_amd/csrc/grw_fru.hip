@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -384,6 +385,46 @@ __device__ __forceinline__ void flux_terms(const double conc[5], const uint32_t 
     }
     if (ch_ito2(ch[c]) == 2) v[4] += 1.0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free flux sums (grw_calc_fru_flux_local_noalloc): the block
+// partials and the last-block ticket of the fixed-order reduction live in one
+// file-scope device array instead of memory a context allocates.  Blocks of
+// 256 threads, one FRU per thread, as the context path of
+// grw_calc_fru_flux_local launches them, so the sums are the same bit for bit.
+constexpr int kNoAllocBlocks = 4096;  // at most 4096 x 256 = 1,048,576 FRUs
+constexpr int kNoAllocMaxFru = kNoAllocBlocks * 256;
+struct NoAllocScratch {
+  double part[kNoAllocBlocks * kSlot];  // block partial sums, kSlot doubles per block (256 B)
+  unsigned int ticket;                  // arrivals; zero at load, reset by every call's last block
+};
+static __device__ __align__(256) NoAllocScratch scratch;
+
+// calc_fru_flux_local.c:28-77 over the reference's arrays (fru [n][5],
+// ltype [n][4][2], ito2 [n][4]) in device memory; num5 (device) gets the sums.
+__global__ void __launch_bounds__(256) k_flux_sums_ref(int n, const int32_t* ltype, const int32_t* ito2,
+                                                       const double* fru, double* num5, NoAllocScratch* sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    double conc[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) conc[k] = fru[(size_t)5 * i + k];
+    uint32_t ch[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {  // the fields flux_terms reads: LCC state, gate, Ito2
+      const int lcc = ltype[(size_t)8 * i + 2 * c], y = ltype[(size_t)8 * i + 2 * c + 1];
+      const int it = ito2[(size_t)4 * i + c];
+      ch[c] = ((uint32_t)lcc & 15u) | ((uint32_t)(y - 1) & 1u) << 4 | ((uint32_t)(it - 1) & 1u) << 20;
+    }
+    flux_terms(conc, ch, acc);
+  }
+  Dev d{};
+  d.part = sc->part;
+  __shared__ double tot[5];
+  if (reduce_range<5>(acc, d, &sc->ticket, blockIdx.x, 0, gridDim.x, 0, tot) && threadIdx.x == 0)
+    for (int k = 0; k < 5; ++k) num5[k] = tot[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -2914,6 +2955,36 @@ int grw_calc_fru_flux_local(grw_ctx* c, double num5[5]) {
   int rc = flux_now(c, dep);
   if (rc) return rc;
   return fetch_out(c, 4, 5, num5);
+}
+
+int grw_calc_fru_flux_local_noalloc(int nfru, const int32_t* ltype, const int32_t* ito2, const double* fru,
+                                    double* num5, void* stream) {
+  if (nfru < 0 || nfru > kNoAllocMaxFru)
+    return fail(GRW_EINVAL, "grw_calc_fru_flux_local_noalloc: nfru " + std::to_string(nfru) +
+                                " outside [0, " + std::to_string(kNoAllocMaxFru) + "], the allocation-free maximum");
+  if (!num5 || (nfru > 0 && (!ltype || !ito2 || !fru))) return fail(GRW_EINVAL, "null argument");
+  // The scratch array's address on the current device, looked up once per
+  // device (each GPU holds its own copy of the array).
+  static std::mutex mu;
+  static void* ptr[64] = {};
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(GRW_EDEVICE, "device ordinal beyond 63");
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return fail(GRW_EDEVICE, std::string("scratch symbol address: ") + hipGetErrorString(e));
+      ptr[dev] = p;
+    }
+  }
+  const int grid = nfru > 0 ? (nfru + 255) / 256 : 1;
+  hipLaunchKernelGGL(k_flux_sums_ref, dim3(grid), dim3(256), 0, (hipStream_t)stream, nfru, ltype, ito2, fru, num5,
+                     (NoAllocScratch*)ptr[dev]);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 int grw_calc_fru_avg_local(grw_ctx* c, double num11[11]) {

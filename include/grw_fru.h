@@ -109,6 +109,21 @@ int grw_simfru_local(grw_ctx *ctx, double t0, double t1, const double dep0[3],
 int grw_calc_fru_flux_local(grw_ctx *ctx, double num5[5]);
 int grw_calc_fru_avg_local(grw_ctx *ctx, double num11[11]);
 
+/* The five sums of calc_fru_flux_local.c:28-77 without any allocation: an
+ * entry point with no counterpart in the reference, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * Inputs are nfru FRUs in the reference's layouts in DEVICE memory (fru
+ * [n][5], ltype [n][4][2], ito2 [n][4]); the sums go to num5 (DEVICE, 5
+ * doubles); the kernel is enqueued on stream (a hipStream_t; NULL: the null
+ * stream) and the call returns without waiting.  Its reduction keeps its block
+ * partials and ticket in a fixed device array of the library (1 MB per GPU,
+ * held until the process ends), so nfru is at most 1,048,576 (GRW_EINVAL
+ * beyond).  The sums equal grw_calc_fru_flux_local's on the same FRUs bit for
+ * bit.  Calls that overlap on one GPU share that array and race: serialise
+ * them (one stream, or events). */
+int grw_calc_fru_flux_local_noalloc(int nfru, const int32_t *ltype, const int32_t *ito2,
+                                    const double *fru, double *num5, void *stream);
+
 int grw_get_stats(grw_ctx *ctx, grw_stats *st);
 int grw_reset_stats(grw_ctx *ctx);
 /* Enable device timing with HIP events: every launch of the host-driven path
