@@ -80,6 +80,8 @@ def parse():
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--layout", choices=["quad", "waves"], default="quad",
+                    help="Monte Carlo kernel of the device integrator (grw_set_layout)")
     ap.add_argument("--graphs", type=int, default=0, choices=[0, 1],
                     help="1: the device integrator replays RK attempts from a captured HIP graph")
     ap.add_argument("--no-traffic", action="store_true",
@@ -168,7 +170,8 @@ def _cell_cmd(args, steps, warmup):
     HIP events (the child a profiler watches)."""
     return [sys.executable, os.path.abspath(__file__), "--steps", str(steps), "--warmup", str(warmup),
             "--nfru", str(args.nfru), "--seed", str(args.seed), "--no-mixed", "--batch-cells", "0",
-            "--cpu-ms", "0", "--no-timing", "--no-traffic", "--no-trace", "--graphs", str(args.graphs)]
+            "--cpu-ms", "0", "--no-timing", "--no-traffic", "--no-trace", "--graphs", str(args.graphs),
+            "--layout", args.layout]
 
 
 def _run_profiled(prof_args, cmd, folder, timeout=300):
@@ -472,6 +475,8 @@ def run_cell(args, precision, timing):
     cell = Cell(args.nfru, seed=args.seed, precision=precision, protocol=prot, block=args.block,
                 lanes_per_fru=args.lanes)
     cell.fru.set_graphs(bool(args.graphs))
+    if args.driver == "device":
+        cell.set_layout(args.layout)
     if args.warmup:
         cell.run(args.warmup, outputs=False, log_cap=0, driver=args.driver)
     cell.fru.set_timing(timing)
@@ -812,7 +817,7 @@ def main():
                                                       else "mixed FP32/FP64", args.warmup,
                                                       args.warmup + args.steps),
                    "nfru": args.nfru, "precision": args.precision, "lanes_per_fru": args.lanes,
-                   "hip_graphs": bool(args.graphs),
+                   "hip_graphs": bool(args.graphs), "layout": args.layout,
                    "driver": args.driver,
                    "rk_attempts": attempts, "rk_rejects": rejects,
                    "fru_steps": st["fru_steps"], "launches": st["launches"],

@@ -21,6 +21,7 @@ from .ic import errweight, load_stable
 NGLOBAL = 37
 NCUR = 24
 NOTHER = 12
+LAYOUT_QUAD, LAYOUT_WAVES = 1, 2  # grw_set_layout
 
 
 class GrwProtocol(C.Structure):
@@ -87,6 +88,8 @@ def _declare(L):
     L.grw_run_cell_device.restype = C.c_int
     L.grw_get_beats.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.c_int]
     L.grw_get_beats.restype = C.c_int
+    L.grw_set_layout.argtypes = [vp, C.c_int]
+    L.grw_set_layout.restype = C.c_int
     L._grw_cell_declared = True
 
 
@@ -197,6 +200,7 @@ class Cell:
         self.errw = errweight()
         self._L = _lib.load()
         _declare(self._L)
+        self.layout = "quad"
         self.restart(0.0)
 
     def restart(self, t: float | None = None) -> None:
@@ -222,6 +226,17 @@ class Cell:
 
     def __exit__(self, *exc):
         self.close()
+
+    def set_layout(self, layout: str) -> None:
+        """The Monte Carlo kernel of the device integrator (grw_set_layout):
+        "quad", four lanes per FRU (the default), or "waves", the
+        wave-specialised kernel (four wavefronts per 16 FRUs).  Both give
+        the same results bit for bit."""
+        codes = {"quad": LAYOUT_QUAD, "waves": LAYOUT_WAVES}
+        if layout not in codes:
+            raise ValueError("layout must be 'quad' or 'waves'")
+        check(self._L.grw_set_layout(self.fru._h, codes[layout]))
+        self.layout = layout
 
     def beats(self, include_current: bool = True) -> np.ndarray:
         """Per-beat statistics of the device-driven runs so far (grw_get_beats;

@@ -1220,6 +1220,502 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
   return steps;
 }
 
+// ---------------------------------------------------------------------------
+// Wave-specialised sub-interval (single cells).  A single cell is bound by the
+// micro-steps of its slowest FRU, each issued by one wavefront at one
+// instruction per issue slot; splitting a micro-step over the lanes of that
+// wavefront would not shorten it (lanes that take different work serialise
+// under exec masks), but splitting it over the four wavefronts of a block
+// does: each runs on its own SIMD.  Here a block holds 16 FRUs, and lane l of
+// every wavefront is cleft l & 3 of FRU l >> 2; the wavefronts take roles:
+//   L  the LCC, its voltage gate and Ito2 (rates, draws, moves; MT words 0, 1, 7)
+//   RA RyRs 0-1 (rates, draws, moves; MT words 2, 3)
+//   RB RyRs 2-4 (MT words 4, 5, 6)
+//   H  the Heun step of the Ca concentrations (fcn_fru twice)
+// and exchange through LDS twice per micro-step: after the rates (the largest
+// exit rate, so that every wavefront takes the same step length) and after the
+// step (the concentrations and the channel fields each role changed).  Every
+// value is computed with the operations of simfru4_body, so the results are
+// the same bit for bit; the flux sums are grouped by reduce_ws into the
+// partials of the 64-FRU blocks of the four-lane kernel.
+constexpr int kWsL = 0, kWsRA = 1, kWsRB = 2, kWsH = 3;
+constexpr int kWsFru = 16;  // FRUs per block
+constexpr uint32_t kWsMaskL = 15u | (1u << 4) | (1u << 20);        // LCC, gate, Ito2
+constexpr uint32_t kWsMaskRA = (7u << 5) | (7u << 8);              // RyR 0, 1
+constexpr uint32_t kWsMaskRB = (7u << 11) | (7u << 14) | (7u << 17);  // RyR 2, 3, 4
+
+// The RyR regime of ryr_k (fru_rates_local.c:384,505,622) alone.
+template <class R>
+__device__ __forceinline__ int ryr_regime(R ca) {
+  const R cm = rmin(ca, R(threshMAXCa));
+  return (cm < R(threshCa56to8)) ? 1 : (cm < R(threshCa34to7)) ? 2 : 3;
+}
+
+// One role's words of a cleft's eight: positions given as offsets into the
+// cleft's 8-word window (K words; the twist of word k also reads word k + 1).
+template <int K>
+struct MtPart {
+  uint32_t cur[K], nxt[K], far[K];
+  uint32_t wv[K];  // the words after regeneration
+};
+template <int... O>
+__device__ __forceinline__ void mt_fetch_part(const uint32_t* row, int g0, MtPart<sizeof...(O)>& w) {
+  constexpr int off[] = {O...};
+  const int q = mt_row_pos(g0);
+  int qf = q + 397;
+  qf -= (qf >= kMtN) ? kMtN : 0;
+  const uint32_t* const pc = row + q;
+  const uint32_t* const pf = row + qf;
+#pragma unroll
+  for (int k = 0; k < (int)sizeof...(O); ++k) {
+    w.cur[k] = pc[off[k]];
+    w.nxt[k] = pc[off[k] + 1];
+    w.far[k] = pf[off[k]];
+  }
+}
+// The regenerated words (kept for mt_store_part) and their tempered draws.
+template <int... O>
+__device__ __forceinline__ void mt_twist_part(int g0, MtPart<sizeof...(O)>& w, uint32_t* out) {
+  constexpr int off[] = {O...};
+#pragma unroll
+  for (int k = 0; k < (int)sizeof...(O); ++k) {
+    const uint32_t m = 0u - (uint32_t)(g0 + off[k] >= kMtN);
+    w.wv[k] = (mt_twist(w.cur[k], w.nxt[k], w.far[k]) & m) | (w.cur[k] & ~m);
+    out[k] = MtLane::temper(w.wv[k]);
+  }
+}
+template <int... O>
+__device__ __forceinline__ void mt_store_part(uint32_t* row, int g0, const MtPart<sizeof...(O)>& w) {
+  constexpr int off[] = {O...};
+  const int q = mt_row_pos(g0);
+  uint32_t* const p = row + q;
+#pragma unroll
+  for (int k = 0; k < (int)sizeof...(O); ++k) p[off[k]] = w.wv[k];
+  if (q < kMtMirror || q > kMtN - kMtMirror) {
+#pragma unroll
+    for (int k = 0; k < (int)sizeof...(O); ++k) {
+      const int pos = q + off[k];
+      if (pos < kMtMirror) row[pos + kMtN] = w.wv[k];
+      if (pos >= kMtN) row[pos - kMtN] = w.wv[k];
+    }
+  }
+}
+
+// A wavefront's exec-uniform role (readfirstlane: the compiler then branches
+// on a scalar instead of masking lanes).
+__device__ __forceinline__ int ws_role() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+template <class R>
+__device__ __forceinline__ unsigned simfru_ws(const SimArgs& a, const Dev& d, double acc[17], int blk) {
+  const int role = ws_role();
+  const int lane = threadIdx.x & 63;
+  const int slot = (blk - 4 * a.b0) * kWsFru + (lane >> 2);
+  const int c = lane & 3;
+  const bool on = slot < a.nfru;  // a FRU's four lanes are on or off together, in every wavefront
+  const int f = a.f0 + (on ? slot : 0);
+  uint32_t* const row = d.mt + (size_t)f * kMtStride;
+  // the exchange, one entry per lane (the same lane in every wavefront)
+  __shared__ double x_max[3][64];
+  __shared__ int x_bad[3][64];
+  __shared__ double x_cj[64], x_css[64];
+  __shared__ uint32_t x_w[3][64];
+  __shared__ uint32_t x_res[64];  // RyR fields resolved by merged-state draws (rare)
+  __shared__ R rtot[8][256];      // lane-private RyR totals (roles RA, RB), as in simfru4_body
+  __shared__ R lom[5][256];       // lane-private omega / bL^k (role L)
+  unsigned steps = 0;
+  double cj = 0.0, css = 0.0;
+  uint32_t w = 0u;
+  int idx = 0, nreg = 0;
+  if (on) {
+    if (a.pro == 2) {  // rollback from the checkpoint (mpi_slave.c:150-184)
+      cj = d.conc_h[f];
+      css = d.conc_h[(c + 1) * a.ld + f];
+      w = d.chan_h[c * a.ld + f];
+      idx = d.mtidx_h[f];
+      if (role == kWsL && c == 0) mt_rollback(row, idx, d.nreg[f], d.mtw_h[f]);
+    } else {
+      cj = d.conc[f];
+      css = d.conc[(c + 1) * a.ld + f];
+      w = d.chan[c * a.ld + f];
+      idx = d.mtidx[f];
+      nreg = a.pro ? 0 : d.nreg[f];
+      if (a.pro == 1 && role == kWsH) {  // checkpoint (mpi_slave.c:192-226)
+        d.conc_h[(c + 1) * a.ld + f] = css;
+        d.chan_h[c * a.ld + f] = w;
+        if (c == 0) {
+          d.conc_h[f] = cj;
+          d.mtidx_h[f] = idx;
+          d.mtw_h[f] = row[mt_first_pos(idx)];
+        }
+      }
+    }
+  }
+  __syncthreads();  // the rolled-back row before any wavefront reads it
+  const double t0 = in_vgpr(a.t0), t1 = in_vgpr(a.t1);
+  const double o0 = in_vgpr(a.dep0[0]), o1 = in_vgpr(a.dep0[1]), o2 = in_vgpr(a.dep0[2]);
+  const double span0 = in_vgpr(a.depf[0] - a.dep0[0]);
+  const double span1 = in_vgpr(a.depf[1] - a.dep0[1]);
+  const double span2 = in_vgpr(a.depf[2] - a.dep0[2]);
+  const double tint = a.t1 - a.t0;
+  const DivK kint{in_vgpr(tint), in_vgpr(1.0 / tint)};
+  double dV = o0, dCai = o1, dNSR = o2;
+  bool lopen = ch_lcc_open(w);
+  int nopen = ch_ryr_open(w);
+  const R k23c = in_vgpr((R)a.k23c), gcf = in_vgpr((R)a.gcf);
+  double t = on ? t0 : t1;
+  int bad = 0, nguard = 0;
+  const int tl = threadIdx.x;
+  if (role == kWsRA || role == kWsRB) {
+    rtot[3][tl] = R(k43 + k45);
+    rtot[5][tl] = R(k65);
+  }
+  if (role == kWsL) {
+    lom[0][tl] = R(omega);
+    lom[1][tl] = R(omega / bL);
+    lom[2][tl] = R(omega / bL2);
+    lom[3][tl] = R(omega / bL3);
+    lom[4][tl] = R(omega / bL4);
+  }
+  // the role's words of the next step, fetched a step ahead
+  // (L: words 0, 1, 7 of the cleft; RA: 2, 3; RB: 4, 5, 6)
+  MtPart<3> winL{};
+  MtPart<2> winA{};
+  MtPart<3> winB{};
+  if (t < t1) {
+    if (role == kWsL) mt_fetch_part<0, 1, 7>(row, idx + 8 * c, winL);
+    if (role == kWsRA) mt_fetch_part<2, 3>(row, idx + 8 * c, winA);
+    if (role == kWsRB) mt_fetch_part<4, 5, 6>(row, idx + 8 * c, winB);
+  }
+  // A role's twists read words that another role regenerates in the same step
+  // (word k + 1 of the cleft, or the next cleft's word 0), so no role stores
+  // its words before barrier 1, by which every role has used its window.  The
+  // stores after it touch none of the words that the windows of the next step,
+  // fetched at the same time, read (positions idx .. idx+32 and words at least
+  // 195 positions older than idx).
+  Ghk<R> gv{};
+  if (role == kWsH) gv = ghk_terms<R>((R)dV);
+  while (t < t1) {
+    const R V = (R)dV;
+    const R cc = (R)css;
+    // ---- merged-state draws (fru_rates_local.c:452,476,581): every role finds
+    // the counts from the step-start state it holds, so all agree on whether
+    // any lane draws (the condition below is the same in every wavefront)
+    const int regime = ryr_regime<R>(cc);
+    uint32_t rbits = ryr_resolve_bits(3);
+    rbits = (regime == 2) ? ryr_resolve_bits(2) : rbits;
+    rbits = (regime == 1) ? ryr_resolve_bits(1) : rbits;
+    int st[kNryr];
+    int nd = 0;
+    unsigned need = 0;
+#pragma unroll
+    for (int r = 0; r < kNryr; ++r) {
+      const uint32_t b = (rbits >> (4u * ((w >> (5 + 3 * r)) & 7u))) & 15u;
+      st[r] = (int)(b & 7u) + 1;
+      nd += (int)(b >> 3);
+      need |= (b >> 3) << r;
+    }
+    const int n0 = qperm<QB0>(nd), n1 = qperm<QB1>(nd), n2 = qperm<QB2>(nd), n3 = qperm<QB3>(nd);
+    const int E = n0 + n1 + n2 + n3;
+    const int lazy0 = idx > kMtN ? idx : kMtN;
+    RyrK<R> k{};
+    R T7 = R(0.0), a7 = R(0.0), a8 = R(0.0);
+    if (role == kWsRA || role == kWsRB) {
+      k = ryr_k(cc, k23c);
+      const R T2 = (R(k21) + k.k23 + k.k25);
+      const R T3 = (R(k32) + k.k34);
+      const R T5 = (R(k52) + k.k54 + k.k56);
+      a7 = rdiv(k.k34, R(k43) + k.k34);
+      T7 = ((R(1.0) - a7) * R(k32)) + (a7 * R(k45));
+      a8 = rdiv(R(k65), k.k56 + R(k65));
+      const R T8 = (a8 * R(k52)) + (a8 * k.k54);
+      rtot[0][tl] = k.k12;
+      rtot[1][tl] = T2;
+      rtot[2][tl] = T3;
+      rtot[4][tl] = T5;
+      rtot[6][tl] = T7;
+      rtot[7][tl] = T8;
+    }
+    if (__any(nd != 0)) {  // rare: role RA draws for the cleft
+      if (role == kWsRA && nd) {
+        const RyrK<R> kk = k;
+        const int Ec = (c > 0 ? n0 : 0) + (c > 1 ? n1 : 0) + (c > 2 ? n2 : 0);
+        uint32_t vw[kNryr];
+        mt_run<kNryr>(row, idx + Ec, nd, vw);
+#pragma unroll
+        for (int r = 0; r < kNryr; ++r) {
+          if ((need >> r) & 1u) {
+            st[r] = ryr_split(st[r], unif(vw[0]), kk);
+#pragma unroll
+            for (int j = 0; j < kNryr - 1; ++j) vw[j] = vw[j + 1];
+          }
+        }
+        uint32_t res = 0;
+#pragma unroll
+        for (int r = 0; r < kNryr; ++r) res |= (uint32_t)(st[r] - 1) << (5 + 3 * r);
+        x_res[lane] = res;
+      }
+      __syncthreads();  // the drawn states, and the words stored for them
+      if (nd) {
+#pragma unroll
+        for (int r = 0; r < kNryr; ++r) st[r] = (int)((x_res[lane] >> (5 + 3 * r)) & 7u) + 1;
+      }
+      // the step's own words follow the merged-state draws
+      if (E != 0) {
+        if (role == kWsL) mt_fetch_part<0, 1, 7>(row, idx + E + 8 * c, winL);
+        if (role == kWsRA) mt_fetch_part<2, 3>(row, idx + E + 8 * c, winA);
+        if (role == kWsRB) mt_fetch_part<4, 5, 6>(row, idx + E + 8 * c, winB);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kNryr; ++r) w = ch_set_ryr(w, r, st[r]);
+    // ---- each role's rates and words of the step
+    R lb = R(0.0), lf = R(0.0), lm = R(0.0), lt = R(0.0), vr = R(0.0), ir = R(0.0);
+    int ls = 1;
+    R tr[kNryr] = {};
+    uint32_t dw[3] = {0u, 0u, 0u};
+    R djs1 = R(0.0), dss1 = R(0.0);
+    R mpart = R(0.0);
+    if (role == kWsL) {
+      const R vm35 = V - R(35.0);
+      const R e0 = R(0.012) * vm35, e1 = R(-0.05) * vm35;
+      const R e2 = rdiv(V + R(12.5), divk(5.0)), e3 = rdiv(V + R(30.0), divk(12.0));
+      R earg = e3;
+      earg = (c == 2) ? e2 : earg;
+      earg = (c == 1) ? e1 : earg;
+      earg = (c == 0) ? e0 : earg;
+      const R ex = xexp(earg);
+      const R alpha = R(alphacf) * qperm<QB0>(ex);
+      const R beta = R(betacf) * qperm<QB1>(ex);
+      const R yinf = rdiv(R(yCa_frac), R(1.0) + qperm<QB2>(ex)) + (R(1.0) - R(yCa_frac));
+      const R tau = R(60.0) + rdiv(R(340.0), R(1.0) + qperm<QB3>(ex));
+      const R bp = rdiv(beta, divk(bL * 1.0));
+      ls = ch_lcc(w);
+      bad |= (ls < 1 || ls > 12);
+      const int lk = ls - 7;
+      lcc_arith<R>(ls, alpha, beta, bp, gcf * cc, lom[(lk >= 0 && lk <= 4) ? lk : 0][tl], lb, lf, lm, lt);
+      vr = rdiv(ch_y(w) == 2 ? (R(1.0) - yinf) : yinf, tau);
+      ir = (ch_ito2(w) == 1) ? R(kfIto2) * cc : R(kbIto2);
+      mpart = rmax(rmax(rmax(R(0.0), lt), vr), ir);
+      mt_twist_part<0, 1, 7>(idx + E + 8 * c, winL, dw);
+    } else if (role == kWsRA) {
+      tr[0] = rtot[st[0] - 1][tl];
+      tr[1] = rtot[st[1] - 1][tl];
+      mpart = rmax(tr[0], tr[1]);
+      mt_twist_part<2, 3>(idx + E + 8 * c, winA, dw);
+    } else if (role == kWsRB) {
+      tr[2] = rtot[st[2] - 1][tl];
+      tr[3] = rtot[st[3] - 1][tl];
+      tr[4] = rtot[st[4] - 1][tl];
+      mpart = rmax(rmax(tr[2], tr[3]), tr[4]);
+      mt_twist_part<4, 5, 6>(idx + E + 8 * c, winB, dw);
+    } else {  // H: the first evaluation of the Heun step needs no step length
+      fcn_fru4<R>((R)cj, cc, gv, (R)dCai, (R)dNSR, lopen, nopen, djs1, dss1);
+    }
+    if (role != kWsH) {
+      x_max[role][lane] = (double)mpart;
+      x_bad[role][lane] = bad;
+    }
+    __syncthreads();  // barrier 1: the rates of every role
+    // ---- step length (simfru_local.c:135-146), the same in every wavefront
+    R maxr = rmax(rmax((R)x_max[kWsL][lane], (R)x_max[kWsRA][lane]), (R)x_max[kWsRB][lane]);
+    bad = x_bad[kWsL][lane] | x_bad[kWsRA][lane] | x_bad[kWsRB][lane];
+    maxr = rmax(maxr, qperm<QX1>(maxr));
+    maxr = rmax(maxr, qperm<QX2>(maxr));
+    bad |= qperm<QX1>(bad);
+    bad |= qperm<QX2>(bad);
+    double dt = 0.1 / (double)maxr;
+    dt = rmin(dt, 10.001e-3);
+    double cm = rmax(css, qperm<QX1>(css));
+    cm = rmax(cm, qperm<QX2>(cm));
+    const double cmax = rmax(0.e-3, cm);
+    if (cmax > 1.5e-3) dt = rmin(dt, 5.001e-3);
+    dt = rmin(dt, t1 - t);
+    if (bad || !(t + dt > t) || ++nguard > kStepCap) {
+      bad = 1;
+      break;
+    }
+    const int gs = idx + E + 8 * c;  // this step's words
+    {
+      const int idn = idx + E + 32;
+      const int adv = idn - lazy0;
+      nreg += (adv > 0 ? adv : 0);
+      idx = (idn >= 2 * kMtN) ? idn - kMtN : idn;
+    }
+    // ---- decisions, moves, the Heun step; the next step's words fetched early
+    double cj_n = cj, css_n = css;
+    if (role == kWsL) {
+      mt_fetch_part<0, 1, 7>(row, idx + 8 * c, winL);
+      mt_store_part<0, 1, 7>(row, gs, winL);
+      const double u0 = unif(dw[0]);
+      const bool mv_lcc = u0 > 1.0 - dt * (double)lt;
+      const bool mv_y = unif(dw[1]) >= 1.0 - dt * (double)vr;
+      const bool mv_ito = unif(dw[2]) >= 1.0 - dt * (double)ir;
+      if (mv_lcc) w = ch_set_lcc(w, lcc_move<R>(ls, u0, dt, lb, lf, lm, lt));
+      if (mv_y) w = ch_flip_y(w);
+      if (mv_ito) w = ch_flip_ito2(w);
+    } else if (role == kWsRA || role == kWsRB) {
+      if (role == kWsRA) {
+        mt_fetch_part<2, 3>(row, idx + 8 * c, winA);
+        mt_store_part<2, 3>(row, gs, winA);
+      } else {
+        mt_fetch_part<4, 5, 6>(row, idx + 8 * c, winB);
+        mt_store_part<4, 5, 6>(row, gs, winB);
+      }
+      const int r0 = role == kWsRA ? 0 : 2, nr = role == kWsRA ? 2 : 3;
+      double ur[3];
+      unsigned mm = 0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        ur[q] = unif(dw[q]);
+        const R trq = role == kWsRA ? (q == 0 ? tr[0] : tr[1]) : tr[2 + q];
+        if (q < nr && ur[q] > 1.0 - dt * (double)trq) mm |= 1u << q;
+      }
+      while (mm) {
+        const int q = __builtin_ctz(mm);
+        mm &= mm - 1u;
+        const int r = r0 + q;
+        int sr = st[0];
+        double ru = ur[0];
+#pragma unroll
+        for (int qq = 1; qq < kNryr; ++qq) sr = (r == qq) ? st[qq] : sr;
+#pragma unroll
+        for (int qq = 1; qq < 3; ++qq) ru = (q == qq) ? ur[qq] : ru;
+        Exits<R> e;
+        ryr_exits_sel(sr, k, a7, a8, rtot[sr - 1][tl], e);
+        w = ch_set_ryr(w, r, pick<R>(ru, dt, e));
+      }
+    }
+    const double tn = t + dt;
+    const double frac = rdiv(tn - t0, kint);
+    if (role == kWsH) {
+      const double k1j = dt * (double)djs1, k1s = dt * (double)dss1;
+      const double y1j = cj + k1j, y1s = css + k1s;
+      const double dVn = o0 + frac * span0, dCain = o1 + frac * span1, dNSRn = o2 + frac * span2;
+      gv = ghk_terms<R>((R)dVn);
+      R djs, dss;
+      fcn_fru4<R>((R)y1j, (R)y1s, gv, (R)dCain, (R)dNSRn, lopen, nopen, djs, dss);
+      cj_n = cj + (k1j + dt * (double)djs) / 2.0;
+      css_n = css + (k1s + dt * (double)dss) / 2.0;
+      x_cj[lane] = cj_n;
+      x_css[lane] = css_n;
+      steps += (c == 0);
+    } else {
+      x_w[role][lane] = w;
+    }
+    t = tn;
+    dV = o0 + frac * span0;
+    dCai = o1 + frac * span1;
+    dNSR = o2 + frac * span2;
+    __syncthreads();  // barrier 2: the new state of every role
+    cj = x_cj[lane];
+    css = x_css[lane];
+    w = (x_w[kWsL][lane] & kWsMaskL) | (x_w[kWsRA][lane] & kWsMaskRA) | (x_w[kWsRB][lane] & kWsMaskRB);
+    lopen = ch_lcc_open(w);
+    nopen = ch_ryr_open(w);
+  }
+  if (on) {
+    if (bad && role == kWsH && c == 0) atomicOr(d.err, 1);
+    if (role == kWsH) {
+      d.conc[(c + 1) * a.ld + f] = css;
+      d.chan[c * a.ld + f] = w;
+      if (c == 0) d.conc[f] = cj;
+      acc[0] = css;
+      acc[1] = (c == 0) ? cj : 0.0;
+      acc[2] = lopen ? css : 0.0;
+      acc[3] = lopen ? 1.0 : 0.0;
+      acc[4] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
+      acc[5] = JRyRmax * ((double)nopen) * (cj - css);
+      if (a.avg) {
+        int nr = 0;
+#pragma unroll
+        for (int r = 0; r < kNryr; ++r) {
+          const int s = ch_ryr(w, r);
+          nr += (s == 1 || s == 2);
+        }
+        acc[6] = css;
+        acc[7] = (c == 0) ? cj : 0.0;
+        acc[8] = JRyRmax * ((double)nopen) * (cj - css);
+        acc[9] = (double)nopen;
+        acc[10] = (double)nr;
+        acc[11] = css * (1.0 + BSRtot / (KBSR + css) + BSLtot / (KBSL + css));
+        acc[12] = (c == 0) ? cj * (1.0 + CSQNtot / (KmCSQN + cj)) : 0.0;
+        acc[13] = (ch_lcc(w) <= 6) ? 1.0 : 0.0;
+        acc[14] = (ch_y(w) == 2) ? 1.0 : 0.0;
+        acc[15] = lopen ? 1.0 : 0.0;
+        acc[16] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
+      }
+    }
+    if (role == kWsL && c == 0) {
+      d.mtidx[f] = idx;
+      d.nreg[f] = nreg;
+    }
+  }
+  return steps;
+}
+
+// The flux sums of a wave-specialised launch: the H wavefront's butterfly of
+// each block's 16 FRUs into the block's slot, the last block (ticket) adding
+// four consecutive slots in the order the four-lane kernel's block adds its
+// four wavefronts, then those in that kernel's order -- the same numbers.
+template <int NV>
+__device__ __forceinline__ bool reduce_ws(double v[NV], const Dev& d, unsigned int* ticket, int blk, int b0, int nb,
+                                          double* tot_sh, int extra = 0) {
+  static_assert(NV <= kSlot, "block partials overflow their slot row");
+  __shared__ double wsum[4][NV];
+  __shared__ bool last;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  // A block's arrival at the ticket follows the completed stores of all its
+  // wavefronts (the fcn block's stage terms come from wavefronts 0 and 1).
+  __syncthreads();
+  if (ws_role() == kWsH) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum_xor(v[k]);
+    if (lane == 0) {
+      if (blk >= 0)
+        for (int k = 0; k < NV; ++k)
+          __hip_atomic_store(&d.part[(size_t)blk * kSlot + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = (t == (unsigned)(4 * nb + extra) - 1u);
+    }
+  }
+  __syncthreads();
+  if (!last) return false;
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  for (int m = (int)threadIdx.x; m < nb; m += blockDim.x) {
+    const size_t g = (size_t)(4 * (b0 + m)) * kSlot;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const double p0 = __hip_atomic_load(&d.part[g + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double p1 = __hip_atomic_load(&d.part[g + kSlot + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double p2 = __hip_atomic_load(&d.part[g + 2 * kSlot + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double p3 = __hip_atomic_load(&d.part[g + 3 * kSlot + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s[k] += ((p0 + p1) + p2) + p3;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = wave_sum_xor(s[k]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) wsum[wave][k] = s[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NV; ++k) {
+      double a = wsum[0][k];
+      for (int w = 1; w < nw; ++w) a += wsum[w][k];
+      tot_sh[k] = a;
+    }
+    *ticket = 0u;
+  }
+  __syncthreads();
+  return true;
+}
+
 // (a lane's count is at most kStepCap + 1, so the wavefront's sum fits 32 bits)
 static_assert((unsigned long long)(kStepCap + 1) * 64ull < (1ull << 32), "step counts overflow");
 __device__ __forceinline__ void count_steps(const Dev& d, unsigned steps) {
@@ -1656,10 +2152,10 @@ __device__ __forceinline__ int mc_block(const SimArgs& base, const Dev& d, int f
 // checkpoints it and sums the FRU state with the lanes, blocks and order of a
 // stage tail, so that a run continued in pieces adds the same numbers as one
 // unbroken run; then attempt_start in the cell's last block.
-template <class R>
+template <class R, bool WS>
 __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev d, RkCtl* ctls) {
   const int blk = mc_block(base, d, 0);
-  const int cell = base.ncells == 1 ? 0 : d.smap[blk];  // (one cell: no dependent load)
+  const int cell = (WS || base.ncells == 1) ? 0 : d.smap[blk];  // (one cell: no dependent load)
   const CellInfo ci = d.cells[cell];
   RkCtl* ctl = ctls + cell;
   const int active = ctl->active, avgp = ctl->avg_pending;
@@ -1677,9 +2173,14 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
   a.pro = !active ? 0 : ctl->resume ? 2 : ctl->success ? 1 : 0;  // mpi_slave.c:150-226
   a.avg = avgp;
   double acc[17] = {};
-  (void)simfru4_body<R>(a, d, acc, blk);
   __shared__ double tot[17];
-  if (!reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot)) return;
+  if (WS) {
+    (void)simfru_ws<R>(a, d, acc, blk);
+    if (!reduce_ws<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, tot)) return;
+  } else {
+    (void)simfru4_body<R>(a, d, acc, blk);
+    if (!reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot)) return;
+  }
   __shared__ double st[kNglobal], F1[kNglobal], Jsh[4], cursh[kNcur];
   __shared__ FcnTrans fq;
   if (threadIdx.x == 0) ctl->pro = 0;  // this launch did the per-FRU work
@@ -1691,13 +2192,13 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
 // flux reduction, and in the cell's last block fcn for this stage and then
 // the next stage or the step decision (rk_pd54m.c:124-333) and the caf.c
 // output loop (caf.c:150-251).
-template <class R>
+template <class R, bool WS>
 __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctls) {
   // blocks [0, ncells): each cell's fcn block, dispatched first; then the
   // Monte Carlo blocks
   const bool fblock = (int)blockIdx.x < base.ncells;
   const int blk = fblock ? -1 : mc_block(base, d, base.ncells);
-  const int cell = fblock ? (int)blockIdx.x : base.ncells == 1 ? 0 : d.smap[blk];  // (one cell: no dependent load)
+  const int cell = fblock ? (int)blockIdx.x : (WS || base.ncells == 1) ? 0 : d.smap[blk];  // (one cell: no dependent load)
   const CellInfo ci = d.cells[cell];
   RkCtl* ctl = ctls + cell;
   if (!ctl->active) return;
@@ -1755,7 +2256,7 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
     }
   } else {
-    count_steps(d, simfru4_body<R>(a, d, acc, blk));
+    count_steps(d, WS ? simfru_ws<R>(a, d, acc, blk) : simfru4_body<R>(a, d, acc, blk));
   }
   // the control block's state vectors for the tail, loaded by every block
   // before the reduction so that the last block does not wait for them after it
@@ -1771,8 +2272,10 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   __shared__ double tot[17];
   // the eleven average sums (tot[6..16]) exist only in the attempt's last stage;
   // the fcn block is one more arrival
-  if (a.avg ? !reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1)
-            : !reduce_range<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1))
+  if (WS ? (a.avg ? !reduce_ws<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, tot, 1)
+                  : !reduce_ws<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, tot, 1))
+         : (a.avg ? !reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1)
+                  : !reduce_range<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1)))
     return;
   const unsigned long long tail0 = wall_clock64();
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
@@ -2320,6 +2823,7 @@ struct grw_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> evpool;  // per-launch timing of device-driven runs
   bool graphs = false;             // device-driven runs replay captured attempts
+  bool ws = false;                 // device integrator: wave-specialised Monte Carlo kernel (single cells)
   hipGraphExec_t graph = nullptr;  // kGraphAttempts RK attempts, captured on first use
   SimArgs graph_base{};            // the launch arguments the graph holds
   Dev graph_dev{};
@@ -2412,8 +2916,8 @@ int run_sim(grw_ctx* c, const SimArgs& a, bool reduce) {
 int alloc_fru_state(grw_ctx* c) {
   int rc = 0;
   const size_t ld = c->ld;
-  const size_t nslots = std::max<size_t>({(size_t)c->grid_sim, (size_t)c->grid_s, (size_t)c->grid_b,
-                                          (size_t)c->grid});
+  const size_t nslots = std::max<size_t>({(size_t)c->grid_sim, (size_t)4 * c->grid_s, (size_t)c->grid_b,
+                                          (size_t)c->grid});  // (4 x grid_s: the wave-specialised layout)
   if ((rc = dalloc(c, &c->d.conc, 5 * ld)) || (rc = dalloc(c, &c->d.chan, 4 * ld)) ||
       (rc = dalloc(c, &c->d.mtidx, ld)) || (rc = dalloc(c, &c->d.nreg, ld)) ||
       (rc = dalloc(c, &c->d.mt, ld * kMtStride)) || (rc = dalloc(c, &c->d.mtw_h, ld)) ||
@@ -2526,18 +3030,40 @@ int reset_order(grw_ctx* c) {
   return 0;
 }
 
+// Monte Carlo blocks of a device-integrator launch: 64 FRUs per block with the
+// four-lane kernel, 16 with the wave-specialised one (four blocks per 64).
+int mc_grid(const grw_ctx* c) { return c->ws ? 4 * c->grid_s : c->grid_s; }
+
+void launch_begin(grw_ctx* c, const SimArgs& base, bool mixed) {
+  const dim3 grid(mc_grid(c));
+  if (c->ws) {
+    if (mixed) hipLaunchKernelGGL((k_dev_begin<float, true>), grid, dim3(256), 0, c->stream, base, c->d, c->dctl);
+    else hipLaunchKernelGGL((k_dev_begin<double, true>), grid, dim3(256), 0, c->stream, base, c->d, c->dctl);
+  } else {
+    if (mixed) hipLaunchKernelGGL((k_dev_begin<float, false>), grid, dim3(256), 0, c->stream, base, c->d, c->dctl);
+    else hipLaunchKernelGGL((k_dev_begin<double, false>), grid, dim3(256), 0, c->stream, base, c->d, c->dctl);
+  }
+}
+
 // One RK attempt's launches: five k_dev_stage, then a batch's regrouping.
 // ek (may be null) holds ten events, the start and stop of each of the five
 // launches (hipExtLaunchKernelGGL: the dispatch's own begin / end timestamps).
 void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent_t* ek) {
+  const dim3 grid(c->ncells + mc_grid(c));
   for (int s5 = 0; s5 < 5; ++s5) {
     hipEvent_t e0 = ek ? ek[2 * s5] : nullptr, e1 = ek ? ek[2 * s5 + 1] : nullptr;
-    if (mixed)
-      hipExtLaunchKernelGGL(k_dev_stage<float>, dim3(c->ncells + c->grid_s), dim3(256), 0, c->stream, e0, e1, 0,
-                            base, c->d, c->dctl);
-    else
-      hipExtLaunchKernelGGL(k_dev_stage<double>, dim3(c->ncells + c->grid_s), dim3(256), 0, c->stream, e0, e1, 0,
-                            base, c->d, c->dctl);
+    if (c->ws) {
+      if (mixed)
+        hipExtLaunchKernelGGL(k_dev_stage<float, true>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d, c->dctl);
+      else
+        hipExtLaunchKernelGGL(k_dev_stage<double, true>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d, c->dctl);
+    } else {
+      if (mixed)
+        hipExtLaunchKernelGGL(k_dev_stage<float, false>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d, c->dctl);
+      else
+        hipExtLaunchKernelGGL(k_dev_stage<double, false>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d,
+                              c->dctl);
+    }
   }
   if (base.sorted) sort_and_order(c);
 }
@@ -2620,9 +3146,9 @@ int run_device(grw_ctx* c) {
     for (int k = 0; k < nc; ++k) was_active[k] = c->hctl[k].active != 0;
     if (first) {  // the run's first attempt; later ones start in the last stage's tail
       if (mixed)
-        hipLaunchKernelGGL(k_dev_begin<float>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+        launch_begin(c, base, true);
       else
-        hipLaunchKernelGGL(k_dev_begin<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+        launch_begin(c, base, false);
       if (base.sorted) sort_and_order(c);
     }
     first = false;
@@ -2682,9 +3208,9 @@ int run_device(grw_ctx* c) {
   for (const RkCtl& h : c->hctl) avg = avg || h.avg_pending;
   if (avg) {  // averages of the last output sample
     if (mixed)
-      hipLaunchKernelGGL(k_dev_begin<float>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+      launch_begin(c, base, true);
     else
-      hipLaunchKernelGGL(k_dev_begin<double>, dim3(c->grid_s), dim3(256), 0, c->stream, base, c->d, c->dctl);
+      launch_begin(c, base, false);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
       e = hipMemcpyAsync(c->hctl.data(), c->dctl, sizeof(RkCtl) * nc, hipMemcpyDeviceToHost, c->stream);
@@ -3055,6 +3581,21 @@ int grw_reset_stats(grw_ctx* c) {
 int grw_set_timing(grw_ctx* c, int on) {
   if (!c) return fail(GRW_EINVAL, "null context");
   c->timing = on != 0;
+  return 0;
+}
+
+int grw_set_layout(grw_ctx* c, int layout) {
+  if (!c) return fail(GRW_EINVAL, "null context");
+  if (layout != GRW_LAYOUT_QUAD && layout != GRW_LAYOUT_WAVES)
+    return fail(GRW_EINVAL, "layout must be GRW_LAYOUT_QUAD or GRW_LAYOUT_WAVES");
+  if (layout == GRW_LAYOUT_WAVES && (c->batch || c->ncells != 1))
+    return fail(GRW_EINVAL, "the wave-specialised layout runs single-cell contexts only");
+  const bool ws = layout == GRW_LAYOUT_WAVES;
+  if (ws != c->ws && c->graph) {  // the captured launches are the other kernel's
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  c->ws = ws;
   return 0;
 }
 

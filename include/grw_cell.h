@@ -85,6 +85,17 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
                         double *currents, double *other, double *log, int log_cap, int *log_n,
                         GrwRkStats *st);
 
+/* The Monte Carlo kernel of the device integrator (grw_run_cell_device):
+ * GRW_LAYOUT_QUAD, four lanes per FRU (one per cleft), 64 FRUs per block --
+ * the default, and the only one for batches; or GRW_LAYOUT_WAVES (single-cell
+ * contexts), 16 FRUs per block whose four wavefronts take the LCC, the two
+ * halves of the RyRs and the Heun step of every micro-step, exchanging through
+ * LDS.  The results are identical bit for bit; the speed differs with the
+ * workload (grw_autotune measures both).  No counterpart in the reference. */
+#define GRW_LAYOUT_QUAD 1
+#define GRW_LAYOUT_WAVES 2
+int grw_set_layout(grw_ctx *ctx, int layout);
+
 /* Per-beat statistics of the device-driven runs of ctx since its state was
  * last loaded or seeded (the reference writes only 1 ms samples,
  * data_io.c:22-140; these come from every accepted RK step).  Beat b starts at
