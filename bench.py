@@ -80,8 +80,10 @@ def parse():
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
-    ap.add_argument("--layout", choices=["quad", "waves"], default="quad",
-                    help="Monte Carlo kernel of the device integrator (grw_set_layout)")
+    ap.add_argument("--layout", choices=["auto", "quad", "waves", "quad3", "quad4"], default="auto",
+                    help="Monte Carlo kernel of the device integrator (grw_set_layout); auto: the autotuner "
+                         "picks it and the graph replays from timed trials (grw_autotune_cell / _batch)")
+    ap.add_argument("--tune-steps", type=int, default=10, help="simulated ms of each single-cell autotuner trial")
     ap.add_argument("--graphs", type=int, default=0, choices=[0, 1],
                     help="1: the device integrator replays RK attempts from a captured HIP graph")
     ap.add_argument("--no-traffic", action="store_true",
@@ -170,8 +172,8 @@ def _cell_cmd(args, steps, warmup):
     HIP events (the child a profiler watches)."""
     return [sys.executable, os.path.abspath(__file__), "--steps", str(steps), "--warmup", str(warmup),
             "--nfru", str(args.nfru), "--seed", str(args.seed), "--no-mixed", "--batch-cells", "0",
-            "--cpu-ms", "0", "--no-timing", "--no-traffic", "--no-trace", "--graphs", str(args.graphs),
-            "--layout", args.layout]
+            "--cpu-ms", "0", "--no-timing", "--no-traffic", "--no-trace", "--graphs", str(int(TUNED["fp64"]["graphs"]) if "fp64" in TUNED else args.graphs),
+            "--layout", TUNED["fp64"]["layout"] if "fp64" in TUNED else args.layout]
 
 
 def _run_profiled(prof_args, cmd, folder, timeout=300):
@@ -467,6 +469,10 @@ def host_cores() -> int:
     return max(1, min(16, n))
 
 
+# autotuner results of this run, by leg ("fp64", "mixed", "batch")
+TUNED: dict = {}
+
+
 def run_cell(args, precision, timing):
     """The single-cell beat: warmup, then args.steps ms timed in 100 ms chunks."""
     from greenstein2002_amd.cell import Cell, Protocol
@@ -475,10 +481,15 @@ def run_cell(args, precision, timing):
     cell = Cell(args.nfru, seed=args.seed, precision=precision, protocol=prot, block=args.block,
                 lanes_per_fru=args.lanes)
     cell.fru.set_graphs(bool(args.graphs))
-    if args.driver == "device":
+    if args.driver == "device" and args.layout != "auto":
         cell.set_layout(args.layout)
     if args.warmup:
         cell.run(args.warmup, outputs=False, log_cap=0, driver=args.driver)
+    if args.driver == "device" and args.layout == "auto":
+        # timed trials from the current state, which the tuner restores
+        TUNED[precision] = cell.autotune(args.tune_steps)
+        print(f"[bench] {precision}: autotuner chose {TUNED[precision]['layout']}, graphs "
+              f"{TUNED[precision]['graphs']}", file=sys.stderr, flush=True)
     cell.fru.set_timing(timing)
     cell.fru.reset_stats()
     chunks = []
@@ -511,6 +522,12 @@ def batched(args):
     specs = [CellSpec(int(nf[k]), seed=1234 + k, protocol=Protocol(period=float(cl[k]))) for k in range(n)]
     b = Batch(specs, precision=args.precision)
     b.set_graphs(bool(args.graphs))
+    if args.layout == "auto":
+        TUNED["batch"] = b.autotune(2)
+        print(f"[bench] batch: autotuner chose {TUNED['batch']['layout']}, graphs {TUNED['batch']['graphs']}",
+              file=sys.stderr, flush=True)
+    elif args.layout != "waves":
+        b.set_layout(args.layout)
     b.set_timing(True)
     b.reset_stats()
     t0 = time.perf_counter()
@@ -535,6 +552,11 @@ def batched(args):
     seq = []
     for k in pick:
         c = Cell(int(nf[k]), seed=1234 + int(k), precision=args.precision, protocol=Protocol(period=float(cl[k])))
+        if args.precision in TUNED:  # the single cell's tuned settings
+            c.set_layout(TUNED[args.precision]["layout"])
+            c.fru.set_graphs(TUNED[args.precision]["graphs"])
+        else:
+            c.fru.set_graphs(bool(args.graphs))
         ts = time.perf_counter()
         c.run(args.batch_ms, outputs=False, log_cap=0)
         seq.append(time.perf_counter() - ts)
@@ -545,7 +567,15 @@ def batched(args):
         workload=f"{n} cells, FRUs uniform in [1250, 12500] (total {int(nf.sum())}), cycle lengths uniform "
                  f"in [300, 2000] ms, t = 0..{args.batch_ms} ms from ic/stable, {args.precision}",
         cells=n, frus=int(nf.sum()), window_ms=args.batch_ms, wall_s=round(wall, 3),
-        cell_ms_per_s=round(n * args.batch_ms / wall, 2), beats_per_s=round(beats / wall, 4),
+        cell_ms_per_s=round(n * args.batch_ms / wall, 2),
+        beats_per_s_est=round(beats / wall, 4),
+        beats_per_s_basis=("estimate, not a count of finished beats: each cell's simulated ms over the window "
+                           "divided by its cycle length, over the wall time; the window holds the stimulus, "
+                           "upstroke and start of release, which cost more per simulated ms than the rest of "
+                           "a beat, so whole beats run faster than this"),
+        layout=TUNED["batch"]["layout"] if "batch" in TUNED else args.layout,
+        hip_graphs=bool(TUNED["batch"]["graphs"]) if "batch" in TUNED else bool(args.graphs),
+        autotune=TUNED.get("batch"),
         rk_attempts=att, fru_steps=st["fru_steps"],
         fru_steps_per_s=round(st["fru_steps"] / wall, 1),
         roofline=dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
@@ -554,7 +584,7 @@ def batched(args):
                       avg_launch_us=round(avg_s * 1e6, 3),
                       launches=st["launches"], timed_launches=st["timed_launches"]),
         sequential=dict(sample_cells=[int(k) for k in pick], sample_walls_s=[round(x, 3) for x in seq],
-                        est_wall_s=round(seq_wall, 2), beats_per_s=round(beats / seq_wall, 4)),
+                        est_wall_s=round(seq_wall, 2), beats_per_s_est=round(beats / seq_wall, 4)),
         speedup_vs_sequential=round(seq_wall / wall, 2))
 
 
@@ -817,7 +847,9 @@ def main():
                                                       else "mixed FP32/FP64", args.warmup,
                                                       args.warmup + args.steps),
                    "nfru": args.nfru, "precision": args.precision, "lanes_per_fru": args.lanes,
-                   "hip_graphs": bool(args.graphs), "layout": args.layout,
+                   "hip_graphs": bool(TUNED["fp64"]["graphs"]) if "fp64" in TUNED else bool(args.graphs),
+                   "layout": TUNED["fp64"]["layout"] if "fp64" in TUNED else args.layout,
+                   "autotune": TUNED.get("fp64"),
                    "driver": args.driver,
                    "rk_attempts": attempts, "rk_rejects": rejects,
                    "fru_steps": st["fru_steps"], "launches": st["launches"],
@@ -836,6 +868,8 @@ def main():
         line["mixed"] = dict(value=round(args.steps / mw, 4), unit="simulated ms / s",
                              speedup_vs_fp64=round(wall / mw, 4),
                              fru_steps=mst["fru_steps"],
+                             layout=TUNED["mixed"]["layout"] if "mixed" in TUNED else args.layout,
+                             hip_graphs=bool(TUNED["mixed"]["graphs"]) if "mixed" in TUNED else bool(args.graphs),
                              beat_vs_fp64={k: (round(mbeat[k] - fbeat[k], 6) if mbeat[k] == mbeat[k]
                                                and fbeat[k] == fbeat[k] else None)
                                            for k in ("APD90", "V_peak", "Cai_peak", "ICa_peak")}

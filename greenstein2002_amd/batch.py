@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 from ._lib import GRW_FP64, GRW_MIXED, GrwStats, check
 from .beats import NSTAT
-from .cell import NGLOBAL, GrwProtocol, GrwRkStats, Protocol, _declare as _declare_cell
+from .cell import LAYOUT_NAMES, NGLOBAL, GrwProtocol, GrwRkStats, GrwTune, Protocol, _declare as _declare_cell
 from .ic import REWIND, errweight, load_stable
 
 
@@ -58,6 +58,8 @@ def _declare(L):
     L.grw_batch_get_beats.argtypes = [vp, C.c_int, vp, C.c_int, C.POINTER(C.c_int), C.c_int]
     for n in ("grw_batch_create", "grw_batch_load_ic", "grw_batch_run", "grw_batch_get_beats"):
         getattr(L, n).restype = C.c_int
+    L.grw_autotune_batch.argtypes = [vp, C.c_int, C.POINTER(GrwTune)]
+    L.grw_autotune_batch.restype = C.c_int
     L._grw_batch_declared = True
 
 
@@ -160,6 +162,22 @@ class Batch:
 
     def set_timing(self, on: bool) -> None:
         check(self._L.grw_set_timing(self._h, int(bool(on))))
+
+    def set_layout(self, layout: str) -> None:
+        """Monte Carlo kernel of the batch ("quad", "quad3", "quad4"; see
+        Cell.set_layout)."""
+        codes = {v: k for k, v in LAYOUT_NAMES.items()}
+        if layout not in codes:
+            raise ValueError("layout must be one of %s" % sorted(codes))
+        check(self._L.grw_set_layout(self._h, codes[layout]))
+
+    def autotune(self, trial_steps: int = 2) -> dict:
+        """Time trial_steps output steps of every cell under each layout, with
+        and without graph replays, from a snapshot of the batch, and keep the
+        fastest (grw_autotune_batch).  The batch's state is as before."""
+        r = GrwTune()
+        check(self._L.grw_autotune_batch(self._h, int(trial_steps), C.byref(r)))
+        return r.as_dict()
 
     def set_graphs(self, on: bool) -> None:
         """Replay seven of every eight RK attempts of the device integrator

@@ -68,6 +68,21 @@ class GrwRkStats(C.Structure):
     _fields_ = [("attempts", C.c_int64), ("rejects", C.c_int64), ("forced", C.c_int64)]
 
 
+TUNE_MAX = 8  # GRW_TUNE_MAX
+LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4"}
+
+
+class GrwTune(C.Structure):
+    _fields_ = [("layout", C.c_int32), ("graphs", C.c_int32), ("ncand", C.c_int32), ("reserved", C.c_int32),
+                ("cand_layout", C.c_int32 * TUNE_MAX), ("cand_graphs", C.c_int32 * TUNE_MAX),
+                ("cand_ms", C.c_double * TUNE_MAX)]
+
+    def as_dict(self) -> dict:
+        return {"layout": LAYOUT_NAMES[self.layout], "graphs": bool(self.graphs),
+                "candidates": [{"layout": LAYOUT_NAMES[self.cand_layout[k]], "graphs": bool(self.cand_graphs[k]),
+                                "ms": round(self.cand_ms[k], 3)} for k in range(self.ncand)]}
+
+
 def _declare(L):
     if getattr(L, "_grw_cell_declared", False):
         return
@@ -90,6 +105,9 @@ def _declare(L):
     L.grw_get_beats.restype = C.c_int
     L.grw_set_layout.argtypes = [vp, C.c_int]
     L.grw_set_layout.restype = C.c_int
+    L.grw_autotune_cell.argtypes = [vp, C.POINTER(GrwProtocol), vp, C.c_double, vp, C.c_double, C.c_int,
+                                    C.POINTER(GrwTune)]
+    L.grw_autotune_cell.restype = C.c_int
     L._grw_cell_declared = True
 
 
@@ -229,14 +247,26 @@ class Cell:
 
     def set_layout(self, layout: str) -> None:
         """The Monte Carlo kernel of the device integrator (grw_set_layout):
-        "quad", four lanes per FRU (the default), or "waves", the
-        wave-specialised kernel (four wavefronts per 16 FRUs).  Both give
-        the same results bit for bit."""
-        codes = {"quad": LAYOUT_QUAD, "waves": LAYOUT_WAVES}
+        "quad", four lanes per FRU (the default); "waves", the
+        wave-specialised kernel (four wavefronts per 16 FRUs); "quad3" /
+        "quad4", the four-lane kernel with registers capped for three / four
+        wavefronts per SIMD.  All give the same results bit for bit."""
+        codes = {v: k for k, v in LAYOUT_NAMES.items()}
         if layout not in codes:
-            raise ValueError("layout must be 'quad' or 'waves'")
+            raise ValueError("layout must be one of %s" % sorted(codes))
         check(self._L.grw_set_layout(self.fru._h, codes[layout]))
         self.layout = layout
+
+    def autotune(self, trial_steps: int = 10) -> dict:
+        """Time trial_steps output steps of the device integrator from the
+        current state under every layout, with and without graph replays, and
+        keep the fastest (grw_autotune_cell).  The cell's state is as before."""
+        p = self.protocol.to_c()
+        r = GrwTune()
+        check(self._L.grw_autotune_cell(self.fru._h, C.byref(p), _vp(self.errw), float(self.t), _vp(self.state),
+                                        float(self.h), int(trial_steps), C.byref(r)))
+        self.layout = LAYOUT_NAMES[r.layout]
+        return r.as_dict()
 
     def beats(self, include_current: bool = True) -> np.ndarray:
         """Per-beat statistics of the device-driven runs so far (grw_get_beats;

@@ -18,9 +18,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -2193,7 +2195,7 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
 // the next stage or the step decision (rk_pd54m.c:124-333) and the caf.c
 // output loop (caf.c:150-251).
 template <class R, bool WS>
-__global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctls) {
+__device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkCtl* ctls) {
   // blocks [0, ncells): each cell's fcn block, dispatched first; then the
   // Monte Carlo blocks
   const bool fblock = (int)blockIdx.x < base.ncells;
@@ -2497,6 +2499,21 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   __shared__ double F1[kNglobal], cursh[kNcur];
   attempt_start(ctl, ci, tot, tot + 6, st, F1, Jsh, cursh, &fq);
   if (j == 0) atomicAdd(d.tailticks, wall_clock64() - tail0);
+}
+
+template <class R, bool WS>
+__global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctls) {
+  dev_stage<R, WS>(base, d, ctls);
+}
+
+// The four-lane stage with its registers capped so that W wavefronts fit on a
+// SIMD (512 / W VGPRs): more wavefronts to hide latency when a batch has more
+// of them than the chip holds at once, against spills.  Same arithmetic, same
+// results.
+template <class R, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) k_dev_stage_occ(const SimArgs base, const Dev d,
+                                                                                          RkCtl* ctls) {
+  dev_stage<R, false>(base, d, ctls);
 }
 
 // Regrouping of a batch cell's FRUs between RK attempts.  A wavefront runs as
@@ -2824,6 +2841,7 @@ struct grw_ctx {
   std::vector<hipEvent_t> evpool;  // per-launch timing of device-driven runs
   bool graphs = false;             // device-driven runs replay captured attempts
   bool ws = false;                 // device integrator: wave-specialised Monte Carlo kernel (single cells)
+  int occ = 0;                     // 0: the stage kernel as compiled; 3, 4: registers capped for 3 / 4 waves per SIMD
   hipGraphExec_t graph = nullptr;  // kGraphAttempts RK attempts, captured on first use
   SimArgs graph_base{};            // the launch arguments the graph holds
   Dev graph_dev{};
@@ -3052,7 +3070,19 @@ void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent
   const dim3 grid(c->ncells + mc_grid(c));
   for (int s5 = 0; s5 < 5; ++s5) {
     hipEvent_t e0 = ek ? ek[2 * s5] : nullptr, e1 = ek ? ek[2 * s5 + 1] : nullptr;
-    if (c->ws) {
+    if (c->occ == 3 || c->occ == 4) {
+      if (c->occ == 3) {
+        if (mixed)
+          hipExtLaunchKernelGGL(k_dev_stage_occ<float, 3>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d, c->dctl);
+        else
+          hipExtLaunchKernelGGL(k_dev_stage_occ<double, 3>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d, c->dctl);
+      } else {
+        if (mixed)
+          hipExtLaunchKernelGGL(k_dev_stage_occ<float, 4>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d, c->dctl);
+        else
+          hipExtLaunchKernelGGL(k_dev_stage_occ<double, 4>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d, c->dctl);
+      }
+    } else if (c->ws) {
       if (mixed)
         hipExtLaunchKernelGGL(k_dev_stage<float, true>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d, c->dctl);
       else
@@ -3586,16 +3616,18 @@ int grw_set_timing(grw_ctx* c, int on) {
 
 int grw_set_layout(grw_ctx* c, int layout) {
   if (!c) return fail(GRW_EINVAL, "null context");
-  if (layout != GRW_LAYOUT_QUAD && layout != GRW_LAYOUT_WAVES)
-    return fail(GRW_EINVAL, "layout must be GRW_LAYOUT_QUAD or GRW_LAYOUT_WAVES");
+  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_QUAD4)
+    return fail(GRW_EINVAL, "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4");
   if (layout == GRW_LAYOUT_WAVES && (c->batch || c->ncells != 1))
     return fail(GRW_EINVAL, "the wave-specialised layout runs single-cell contexts only");
   const bool ws = layout == GRW_LAYOUT_WAVES;
-  if (ws != c->ws && c->graph) {  // the captured launches are the other kernel's
+  const int occ = layout == GRW_LAYOUT_QUAD3 ? 3 : layout == GRW_LAYOUT_QUAD4 ? 4 : 0;
+  if ((ws != c->ws || occ != c->occ) && c->graph) {  // the captured launches are the other kernel's
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
   }
   c->ws = ws;
+  c->occ = occ;
   return 0;
 }
 
@@ -3861,3 +3893,151 @@ int grw_batch_get_beats(grw_ctx* c, int cell, double* out, int cap, int* n, int 
 
 }  // extern "C"
 
+
+// ---------------------------------------------------------------------------
+// Autotuner: the device integrator's kernel layouts (grw_set_layout) and graph
+// replays give the same results bit for bit, so the choice is a matter of speed
+// alone, and that depends on the workload (FRU counts, precision, number of
+// cells).  The tuner times a short run of the caller's own workload under each
+// candidate, every one from the same snapshot of the context's device state,
+// restores the snapshot and keeps the fastest.
+namespace {
+
+// Every device array a run of the device integrator reads or writes and keeps
+// (the per-launch scratch -- partials, tickets, out -- excepted), with the
+// host copies of the control blocks and the counters.
+struct CtxSnapshot {
+  std::vector<std::pair<void*, size_t>> regions;  // device address, bytes
+  char* mem = nullptr;
+  std::vector<RkCtl> hctl;
+  grw_stats stats{};
+  ~CtxSnapshot() {
+    if (mem) (void)hipFree(mem);
+  }
+};
+
+int snapshot_take(grw_ctx* c, CtxSnapshot& s) {
+  const size_t ld = (size_t)c->ld, nc = (size_t)c->ncells;
+  s.regions = {{c->d.conc, 5 * ld * sizeof(double)},
+               {c->d.chan, 4 * ld * sizeof(uint32_t)},
+               {c->d.mtidx, ld * sizeof(int32_t)},
+               {c->d.nreg, ld * sizeof(int32_t)},
+               {c->d.mt, ld * kMtStride * sizeof(uint32_t)},
+               {c->d.mtw_h, ld * sizeof(uint32_t)},
+               {c->d.conc_h, 5 * ld * sizeof(double)},
+               {c->d.chan_h, 4 * ld * sizeof(uint32_t)},
+               {c->d.mtidx_h, ld * sizeof(int32_t)},
+               {c->d.nsteps, sizeof(unsigned long long)},
+               {c->d.err, sizeof(int)},
+               {c->d.maxsteps, sizeof(unsigned int)},
+               {c->d.critsteps, sizeof(unsigned long long)},
+               {c->d.tailticks, sizeof(unsigned long long)},
+               {c->d.perm, ld * sizeof(int32_t)},
+               {c->d.cost, ld * sizeof(uint32_t)},
+               {c->d.bkey, (size_t)std::max(c->grid_s, 1) * sizeof(int32_t)},
+               {c->d.border, (size_t)std::max(c->grid_s, 1) * sizeof(int32_t)},
+               {c->dctl, nc * sizeof(RkCtl)},
+               {c->dbeats, nc * (size_t)c->beat_cap * kBeatStats * sizeof(double)}};
+  size_t total = 0;
+  for (auto& r : s.regions)
+    if (r.first) total += (r.second + 255) / 256 * 256;
+  HIPCHK(hipMalloc((void**)&s.mem, std::max<size_t>(total, 256)));
+  size_t off = 0;
+  for (auto& r : s.regions) {
+    if (!r.first || !r.second) continue;
+    HIPCHK(hipMemcpyAsync(s.mem + off, r.first, r.second, hipMemcpyDeviceToDevice, c->stream));
+    off += (r.second + 255) / 256 * 256;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  s.hctl = c->hctl;
+  s.stats = c->stats;
+  return 0;
+}
+
+int snapshot_restore(grw_ctx* c, const CtxSnapshot& s) {
+  size_t off = 0;
+  for (auto& r : s.regions) {
+    if (!r.first || !r.second) continue;
+    HIPCHK(hipMemcpyAsync(r.first, s.mem + off, r.second, hipMemcpyDeviceToDevice, c->stream));
+    off += (r.second + 255) / 256 * 256;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->hctl = s.hctl;
+  c->stats = s.stats;
+  return 0;
+}
+
+// Candidates in a fixed order; a warm-up run of one output step (graph capture,
+// code loading) precedes each timed run, both from the snapshot.
+int autotune(grw_ctx* c, const std::vector<int>& layouts, const std::function<int(int)>& run, int trial_steps,
+             GrwTune* out) {
+  const bool ws0 = c->ws, g0 = c->graphs;
+  const int occ0 = c->occ;
+  const int layout0 = ws0 ? GRW_LAYOUT_WAVES : occ0 == 3 ? GRW_LAYOUT_QUAD3 : occ0 == 4 ? GRW_LAYOUT_QUAD4 : GRW_LAYOUT_QUAD;
+  CtxSnapshot snap;
+  if (int rc = snapshot_take(c, snap)) return rc;
+  GrwTune r{};
+  double best = 0.0;
+  int rc = 0;
+  for (int lay : layouts) {
+    for (int g = 0; g < 2 && rc == 0; ++g) {
+      if (r.ncand >= GRW_TUNE_MAX) break;
+      if ((rc = grw_set_layout(c, lay)) || (rc = grw_set_graphs(c, g))) break;
+      if ((rc = snapshot_restore(c, snap)) || (rc = run(1)) || (rc = snapshot_restore(c, snap))) break;
+      const auto t0 = std::chrono::steady_clock::now();
+      if ((rc = run(trial_steps))) break;
+      const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      r.cand_layout[r.ncand] = lay;
+      r.cand_graphs[r.ncand] = g;
+      r.cand_ms[r.ncand] = ms;
+      if (r.ncand == 0 || ms < best) {
+        best = ms;
+        r.layout = lay;
+        r.graphs = g;
+      }
+      ++r.ncand;
+    }
+  }
+  const int rc2 = snapshot_restore(c, snap);
+  if (rc || rc2 || r.ncand == 0) {  // leave the context as it was
+    std::string msg = grw_last_error();
+    (void)grw_set_layout(c, layout0);
+    (void)grw_set_graphs(c, g0 ? 1 : 0);
+    return fail(rc ? rc : rc2 ? rc2 : GRW_EINVAL, "autotune: " + msg);
+  }
+  if ((rc = grw_set_layout(c, r.layout)) || (rc = grw_set_graphs(c, r.graphs))) return rc;
+  if (out) *out = r;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int grw_autotune_cell(grw_ctx* c, const GrwProtocol* p, const double errw[37], double t0, const double state[37],
+                      double h, int trial_steps, GrwTune* out) {
+  if (int rc = single(c)) return rc;
+  if (!p || !errw || !state || trial_steps < 1 || !(h > 0.0)) return fail(GRW_EINVAL, "bad argument to grw_autotune_cell");
+  auto run = [&](int nsteps) {
+    double st[kNglobal];
+    for (int q = 0; q < kNglobal; ++q) st[q] = state[q];
+    double hh = h;
+    int nlog = 0;
+    return grw_run_cell_device(c, p, errw, t0, nsteps, st, &hh, nullptr, nullptr, nullptr, nullptr, 0, &nlog,
+                               nullptr);
+  };
+  return autotune(c, {GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES, GRW_LAYOUT_QUAD3}, run, trial_steps, out);
+}
+
+int grw_autotune_batch(grw_ctx* c, int trial_steps, GrwTune* out) {
+  if (!c) return fail(GRW_EINVAL, "null context");
+  if (!c->batch) return fail(GRW_EINVAL, "grw_autotune_batch acts on a batch context (grw_batch_create)");
+  if (trial_steps < 1) return fail(GRW_EINVAL, "trial_steps must be positive");
+  auto run = [&](int nsteps) {
+    std::vector<int32_t> ns((size_t)c->ncells, nsteps);
+    return grw_batch_run(c, ns.data(), nullptr, nullptr);
+  };
+  return autotune(c, {GRW_LAYOUT_QUAD, GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4}, run, trial_steps, out);
+}
+
+}  // extern "C"

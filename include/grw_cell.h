@@ -85,16 +85,45 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
                         double *currents, double *other, double *log, int log_cap, int *log_n,
                         GrwRkStats *st);
 
-/* The Monte Carlo kernel of the device integrator (grw_run_cell_device):
- * GRW_LAYOUT_QUAD, four lanes per FRU (one per cleft), 64 FRUs per block --
- * the default, and the only one for batches; or GRW_LAYOUT_WAVES (single-cell
- * contexts), 16 FRUs per block whose four wavefronts take the LCC, the two
- * halves of the RyRs and the Heun step of every micro-step, exchanging through
- * LDS.  The results are identical bit for bit; the speed differs with the
- * workload (grw_autotune measures both).  No counterpart in the reference. */
+/* The Monte Carlo kernel of the device integrator (grw_run_cell_device,
+ * grw_batch_run):
+ *   GRW_LAYOUT_QUAD   four lanes per FRU (one per cleft), 64 FRUs per block,
+ *                     registers as the compiler allocates them (the default);
+ *   GRW_LAYOUT_WAVES  single-cell contexts only: 16 FRUs per block whose four
+ *                     wavefronts take the LCC, the two halves of the RyRs and
+ *                     the Heun step of every micro-step, exchanging through LDS;
+ *   GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4  the four-lane kernel with its registers
+ *                     capped for three or four wavefronts per SIMD (latency
+ *                     hiding for batches, against spills).
+ * Every layout gives the same results bit for bit; the speed differs with the
+ * workload (grw_autotune_cell / grw_autotune_batch time them).  No counterpart
+ * in the reference. */
 #define GRW_LAYOUT_QUAD 1
 #define GRW_LAYOUT_WAVES 2
+#define GRW_LAYOUT_QUAD3 3
+#define GRW_LAYOUT_QUAD4 4
 int grw_set_layout(grw_ctx *ctx, int layout);
+
+/* Result of an autotuning pass: the choice (now set on the context) and every
+ * candidate timed, with the wall time of its trial run. */
+#define GRW_TUNE_MAX 8
+typedef struct {
+  int32_t layout, graphs; /* the fastest candidate (grw_set_layout, grw_set_graphs) */
+  int32_t ncand, reserved;
+  int32_t cand_layout[GRW_TUNE_MAX], cand_graphs[GRW_TUNE_MAX];
+  double cand_ms[GRW_TUNE_MAX];
+} GrwTune;
+
+/* Autotuner of the device integrator for a single cell: times trial_steps
+ * output steps of grw_run_cell_device from (t0, state, h) under each of
+ * GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES and GRW_LAYOUT_QUAD3, with and without
+ * graph replays, every trial from a snapshot of the context's device state
+ * (a warm-up step before each); restores the snapshot, so the context's FRU
+ * state, beats and counters are as they were, and sets the fastest candidate.
+ * The layouts give identical results, so the choice changes only the speed.
+ * No counterpart in the reference. */
+int grw_autotune_cell(grw_ctx *ctx, const GrwProtocol *p, const double errw[37], double t0,
+                      const double state[37], double h, int trial_steps, GrwTune *out);
 
 /* Per-beat statistics of the device-driven runs of ctx since its state was
  * last loaded or seeded (the reference writes only 1 ms samples,

@@ -93,5 +93,76 @@ def test_wave_layout_is_refused_for_batches():
     with Batch([CellSpec(nfru=64, seed=1), CellSpec(nfru=64, seed=2)]) as bt:
         assert L.grw_set_layout(bt._h, 2) != 0
         assert L.grw_set_layout(bt._h, 1) == 0
+        assert L.grw_set_layout(bt._h, 4) == 0
     with Cell(64) as cell:
-        assert L.grw_set_layout(cell.fru._h, 3) != 0
+        assert L.grw_set_layout(cell.fru._h, 5) != 0
+        assert L.grw_set_layout(cell.fru._h, 0) != 0
+
+
+@pytest.mark.parametrize("layout", ["quad3", "quad4"])
+def test_register_capped_layouts_equal_quad(layout):
+    n, seed, ms = 384, 31, 20
+    print(f"layout parity {layout} n={n} seed={seed} ms={ms}", flush=True)
+    q = _run(n, seed, "fp64", "quad", ms, graphs=True)
+    w = _run(n, seed, "fp64", layout, ms, graphs=True)
+    _assert_same(q, w)
+
+
+def _batch_specs():
+    return [CellSpec(nfru=n, seed=100 + k, protocol=Protocol(period=cl))
+            for k, (n, cl) in enumerate([(300, 400.0), (130, 1000.0), (517, 700.0), (64, 300.0)])]
+
+
+def _batch_run(layout=None, tune=False, ms=14):
+    with Batch(_batch_specs()) as bt:
+        bt.set_graphs(True)
+        info = None
+        if layout:
+            bt.set_layout(layout)
+        if tune:
+            info = bt.autotune(2)
+        st = bt.run([ms] * 4)
+        beats = [bt.beats(k) for k in range(4)]
+    return st, beats, info
+
+
+def test_batch_layouts_and_autotune_give_the_same_run():
+    """A batch of four uneven cells: the register-capped layouts and a run
+    after the autotuner (which times every candidate from a snapshot and
+    restores it) end in the same states and beats as the default."""
+    print("batch layouts + autotune", flush=True)
+    ref, rb, _ = _batch_run()
+    for lay in ("quad3", "quad4"):
+        st, b, _ = _batch_run(layout=lay)
+        np.testing.assert_array_equal(st, ref, err_msg=lay)
+        for x, y in zip(b, rb):
+            np.testing.assert_array_equal(x, y)
+    st, b, info = _batch_run(tune=True)
+    assert info["layout"] in ("quad", "quad3", "quad4")
+    assert len(info["candidates"]) == 6 and all(c["ms"] > 0 for c in info["candidates"])
+    np.testing.assert_array_equal(st, ref)
+    for x, y in zip(b, rb):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_cell_autotune_leaves_the_cell_as_it_was():
+    n, seed = 512, 77
+    print(f"cell autotune n={n} seed={seed}", flush=True)
+    with Cell(n, seed=seed) as a, Cell(n, seed=seed) as b:
+        ra0 = a.run(6)
+        rb0 = b.run(6)
+        info = b.autotune(trial_steps=4)
+        assert info["layout"] in ("quad", "waves", "quad3")
+        assert len(info["candidates"]) == 6 and all(c["ms"] > 0 for c in info["candidates"])
+        assert b.layout == info["layout"]
+        ra = a.run(14)
+        rb = b.run(14)
+        fa, fb = a.fru.parallel_get_FRUs(), b.fru.parallel_get_FRUs()
+        ba, bb = a.beats(), b.beats()
+    np.testing.assert_array_equal(ra0.states, rb0.states)
+    for k in ("states", "currents", "other", "log"):
+        np.testing.assert_array_equal(getattr(ra, k), getattr(rb, k), err_msg=k)
+    assert ra.stats == rb.stats
+    for k in fa:
+        np.testing.assert_array_equal(np.asarray(fa[k]), np.asarray(fb[k]), err_msg=k)
+    np.testing.assert_array_equal(ba, bb)
