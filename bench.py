@@ -74,9 +74,10 @@ def parse():
     ap.add_argument("--lanes", type=int, default=4, choices=[1, 4], help="lanes per FRU")
     ap.add_argument("--driver", choices=["device", "host"], default="device",
                     help="integrator on the GPU (default) or on the host calling the FRU ABI")
-    ap.add_argument("--cpu-ms", type=float, default=1.0,
-                    help="0 disables the CPU baselines (the threaded oracle over the timed window, and "
-                         "the reference's MPI program where it is built)")
+    ap.add_argument("--cpu-ms", type=int, default=20,
+                    help="simulated ms of the CPU baselines' sample, t = warmup .. warmup + this (the threaded "
+                         "oracle, and the reference's MPI program where it is built); the GPU run times the "
+                         "same window as its first chunk and reports its rate beside them; 0 disables them")
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
@@ -168,11 +169,13 @@ def _profiler():
 
 
 def _cell_cmd(args, steps, warmup):
-    """This script on the same cell, FP64, without the other legs and without
-    HIP events (the child a profiler watches)."""
+    """This script on the same cell, FP64, with the layout the run used,
+    without the other legs and without HIP events (the child a profiler
+    watches).  Always direct launches: a graph replay submits the same
+    kernels, and the profiler traces directly launched ones."""
     return [sys.executable, os.path.abspath(__file__), "--steps", str(steps), "--warmup", str(warmup),
             "--nfru", str(args.nfru), "--seed", str(args.seed), "--no-mixed", "--batch-cells", "0",
-            "--cpu-ms", "0", "--no-timing", "--no-traffic", "--no-trace", "--graphs", str(int(TUNED["fp64"]["graphs"]) if "fp64" in TUNED else args.graphs),
+            "--cpu-ms", "0", "--no-timing", "--no-traffic", "--no-trace", "--graphs", "0",
             "--layout", TUNED["fp64"]["layout"] if "fp64" in TUNED else args.layout]
 
 
@@ -323,15 +326,22 @@ def summarize(folder: str) -> None:
 
     for path in sorted(glob.glob(os.path.join(folder, "**", "*kernel_trace.csv"), recursive=True)):
         acc = defaultdict(list)
+        ends = defaultdict(list)
         with open(path) as f:
             for r in csv.DictReader(f):
                 grid = int(r["Grid_Size_X"]) // max(int(r["Workgroup_Size_X"]), 1)
-                acc[(short(r["Kernel_Name"]), grid)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+                a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+                acc[(short(r["Kernel_Name"]), grid)].append((b - a) / 1e3)
+                ends[(short(r["Kernel_Name"]), grid)].append((a, b))
         print(f"\n{os.path.relpath(path, folder)}\n")
-        print("| kernel | workgroups | dispatches | mean us | min us | max us | total ms |")
-        print("|---|---|---|---|---|---|---|")
+        # span: first start to last end of the group's dispatches
+        print("| kernel | workgroups | dispatches | mean us | min us | max us | total ms | span ms |")
+        print("|---|---|---|---|---|---|---|---|")
         for (k, g), v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
-            print(f"| {k} | {g} | {len(v)} | {sum(v) / len(v):.2f} | {min(v):.2f} | {max(v):.2f} | {sum(v) / 1e3:.2f} |")
+            e = ends[(k, g)]
+            span = (max(b for _, b in e) - min(a for a, _ in e)) / 1e6
+            print(f"| {k} | {g} | {len(v)} | {sum(v) / len(v):.2f} | {min(v):.2f} | {max(v):.2f} | "
+                  f"{sum(v) / 1e3:.2f} | {span:.2f} |")
     for path in sorted(glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True)):
         acc = defaultdict(lambda: defaultdict(list))
         with open(path) as f:
@@ -383,6 +393,8 @@ def reference_cpu_baseline(t0: float, t1: float, ranks: int) -> dict | None:
                     except ValueError:
                         continue
                     marks[t] = time.perf_counter()
+                    if t % 5.0 == 0.0:
+                        print(f"[bench] reference MPI program: t = {t:g} ms simulated", file=sys.stderr, flush=True)
                     if t >= t1 or time.perf_counter() > deadline:
                         break
         finally:
@@ -404,7 +416,7 @@ def reference_cpu_baseline(t0: float, t1: float, ranks: int) -> dict | None:
                 sample=f"the reference's MPI program (oracle/_ref/stoch: caf.c, rk_pd54m.c, mpi_master.c, "
                        f"mpi_slave.c and the hot path, unmodified, gcc -O3, MPICH) on {ranks} ranks of this "
                        f"host, its compiled 1,000-FRU cell, 1 Hz current clamp from ic/stable, timed over "
-                       f"t = {t0:g}..{t1:g} ms (the GPU line's window): {rate:.3f} simulated ms/s at 1,000 FRUs "
+                       f"t = {t0:g}..{t1:g} ms (the CPU oracle's window): {rate:.3f} simulated ms/s at 1,000 FRUs "
                        f"in {wall:.2f} s, scaled by 1,000/12,500 to the 12,500-FRU cell (its run time is "
                        f"per-FRU work: >= 95% in the hot path)",
                 value_1000fru=round(rate, 4), wall_s=round(wall, 3))
@@ -414,7 +426,7 @@ def cpu_baseline(nfru: int, t0: int, t1: int, seed: int, threads: int) -> dict:
     """The CPU oracle (oracle/libgrw_oracle.so, the reference's algorithm,
     bit-exact with it; its FRUs within a sub-interval on `threads` OpenMP
     threads) on the same cell, run from ic/stable and timed over simulated
-    t0..t1 ms, the GPU line's window."""
+    t0..t1 ms (a bounded sample of the GPU line's workload)."""
     import ctypes as C
 
     import numpy as np
@@ -448,6 +460,9 @@ def cpu_baseline(nfru: int, t0: int, t1: int, seed: int, threads: int) -> dict:
     ts = time.perf_counter()
     for k in range(t0, t1):
         h, _ = cell.rk54pd(float(k), float(k + 1), h, state, cur)
+        if (k + 1 - t0) % 5 == 0:
+            print(f"[bench] CPU oracle: t = {k + 1} ms simulated, {time.perf_counter() - ts:.1f} s wall",
+                  file=sys.stderr, flush=True)
     dt = time.perf_counter() - ts
     steps = int(cell.c.n_fru_steps) - steps0
     cell.close()
@@ -455,8 +470,9 @@ def cpu_baseline(nfru: int, t0: int, t1: int, seed: int, threads: int) -> dict:
                 kind="port",
                 sample=f"the CPU oracle (oracle/grw_oracle.c, the reference's algorithm, bit-exact with its "
                        f"rk54pd and MPI layer) on the same {nfru}-FRU cell and seed, FRUs of each "
-                       f"sub-interval on {threads} OpenMP threads, timed over t = {t0}..{t1} ms (the GPU "
-                       f"line's window): {steps} FRU-steps in {dt:.2f} s",
+                       f"sub-interval on {threads} OpenMP threads, timed over t = {t0}..{t1} ms (the stimulus "
+                       f"at 10 ms, upstroke and start of release; the GPU's rate over the same window is "
+                       f"gpu_value_same_window): {steps} FRU-steps in {dt:.2f} s",
                 fru_steps=steps, wall_s=round(dt, 3))
 
 
@@ -471,6 +487,8 @@ def host_cores() -> int:
 
 # autotuner results of this run, by leg ("fp64", "mixed", "batch")
 TUNED: dict = {}
+# (simulated ms, wall s) of each timed chunk of the single-cell runs, by precision
+RUN_WALLS: dict = {}
 
 
 def run_cell(args, precision, timing):
@@ -493,11 +511,16 @@ def run_cell(args, precision, timing):
     cell.fru.set_timing(timing)
     cell.fru.reset_stats()
     chunks = []
+    walls = []
     t0 = time.perf_counter()
     done = 0
+    # the first chunk is the CPU baselines' window (t = warmup .. warmup + cpu_ms)
+    first = args.cpu_ms if 0 < args.cpu_ms < args.steps else 0
     while done < args.steps:
-        n = min(100, args.steps - done)
+        n = first if (done == 0 and first) else min(100, args.steps - done)
+        tc = time.perf_counter()
         chunks.append(cell.run(n, log_cap=0, driver=args.driver))
+        walls.append((n, time.perf_counter() - tc))
         done += n
         print(f"[bench] {precision}: t = {args.warmup + done} ms simulated, {time.perf_counter() - t0:.1f} s wall",
               file=sys.stderr, flush=True)
@@ -505,6 +528,7 @@ def run_cell(args, precision, timing):
     st = cell.fru.stats()
     beats = cell.beats() if args.driver == "device" else None
     cell.close()
+    RUN_WALLS[precision] = walls
     return wall, chunks, st, beats
 
 
@@ -561,6 +585,7 @@ def batched(args):
         c.run(args.batch_ms, outputs=False, log_cap=0)
         seq.append(time.perf_counter() - ts)
         c.close()
+        print(f"[bench] batch cell {k} alone: {seq[-1]:.2f} s wall", file=sys.stderr, flush=True)
     seq_wall = float(np.mean(seq)) * n
     beats = float(np.sum(args.batch_ms / cl))
     return dict(
@@ -883,17 +908,22 @@ def main():
             roof["traffic_detail"] = tr
     if args.cpu_ms > 0:
         cores = host_cores()
-        line["cpu_baseline"] = cpu_baseline(args.nfru, args.warmup, args.warmup + args.steps, args.seed, cores)
-        line["vs_cpu_baseline"] = round(value / line["cpu_baseline"]["value"], 2)
+        w0, w1 = args.warmup, args.warmup + min(args.cpu_ms, args.steps)
+        line["cpu_baseline"] = cpu_baseline(args.nfru, w0, w1, args.seed, cores)
+        # the GPU's rate over the same window: the timed run's first chunk
+        n0, s0 = RUN_WALLS[args.precision][0]
+        gpu_same = n0 / s0
+        line["cpu_baseline"]["gpu_value_same_window"] = round(gpu_same, 4)
+        line["vs_cpu_baseline"] = round(gpu_same / line["cpu_baseline"]["value"], 2)
         if args.nfru == 12500 and args.precision == "fp64":
-            runs = [reference_cpu_baseline(float(args.warmup), float(args.warmup + args.steps), cores)
-                    for _ in range(3)]
+            runs = [reference_cpu_baseline(float(w0), float(w1), cores) for _ in range(3)]
             runs = [r for r in runs if r]
             if runs:
                 ref = sorted(runs, key=lambda r: r["value"])[len(runs) // 2]
                 ref["repeats"] = [r["value_1000fru"] for r in runs]
+                ref["gpu_value_same_window"] = round(gpu_same, 4)
                 line["cpu_reference_mpi"] = ref
-                line["vs_cpu_reference_mpi"] = round(value / ref["value"], 2)
+                line["vs_cpu_reference_mpi"] = round(gpu_same / ref["value"], 2)
     print(json.dumps(line), flush=True)
 
 

@@ -3967,8 +3967,10 @@ int snapshot_restore(grw_ctx* c, const CtxSnapshot& s) {
   return 0;
 }
 
-// Candidates in a fixed order; a warm-up run of one output step (graph capture,
-// code loading) precedes each timed run, both from the snapshot.
+// Candidates in a fixed order, the default first; a warm-up run of one output
+// step (graph capture, code loading) precedes each timed run, both from the
+// snapshot.
+constexpr double kTuneMargin = 0.01;
 int autotune(grw_ctx* c, const std::vector<int>& layouts, const std::function<int(int)>& run, int trial_steps,
              GrwTune* out) {
   const bool ws0 = c->ws, g0 = c->graphs;
@@ -3990,7 +3992,10 @@ int autotune(grw_ctx* c, const std::vector<int>& layouts, const std::function<in
       r.cand_layout[r.ncand] = lay;
       r.cand_graphs[r.ncand] = g;
       r.cand_ms[r.ncand] = ms;
-      if (r.ncand == 0 || ms < best) {
+      // a candidate replaces the one kept so far only if it is faster by more
+      // than the trials' run-to-run spread, so that ties keep the earlier
+      // (default) setting
+      if (r.ncand == 0 || ms < best * (1.0 - kTuneMargin)) {
         best = ms;
         r.layout = lay;
         r.graphs = g;

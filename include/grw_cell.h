@@ -119,7 +119,9 @@ typedef struct {
  * GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES and GRW_LAYOUT_QUAD3, with and without
  * graph replays, every trial from a snapshot of the context's device state
  * (a warm-up step before each); restores the snapshot, so the context's FRU
- * state, beats and counters are as they were, and sets the fastest candidate.
+ * state, beats and counters are as they were, and sets the fastest candidate
+ * (a later candidate must beat the one kept by more than 1%: ties keep the
+ * default, GRW_LAYOUT_QUAD without graphs).
  * The layouts give identical results, so the choice changes only the speed.
  * No counterpart in the reference. */
 int grw_autotune_cell(grw_ctx *ctx, const GrwProtocol *p, const double errw[37], double t0,
