@@ -106,6 +106,10 @@ def parse():
                          "Ca entry, Ca release and EC-coupling gain of the step")
     ap.add_argument("--clamp-shift", type=float, default=100.0, help="ms held before the step")
     ap.add_argument("--clamp-duration", type=float, default=200.0, help="ms of the step")
+    ap.add_argument("--tune-table", default="",
+                    help="only run the autotuner on single cells of these comma-separated FRU counts, both "
+                         "precisions, from t = --warmup (stimulus at 10 ms), trials of --tune-steps ms, and "
+                         "print every candidate's time (one JSON line)")
     ap.add_argument("--sweep", type=int, default=0,
                     help="only run the parameter sweep: this many cells of --nfru FRUs (1,024 x 12,500 "
                          "= 12.8 M FRUs for the full sweep) with RyR-opening and LCC mode-switch scales "
@@ -787,8 +791,35 @@ def sweep(args) -> dict:
                     serca=[float(serca.min()), float(serca.max())]))
 
 
+def tune_table(args) -> dict:
+    """The autotuner's trials (grw_autotune_cell) on single cells of several
+    FRU counts: every layout / graph candidate's wall time for the same
+    --tune-steps simulated ms from t = --warmup, and the choice."""
+    from greenstein2002_amd.cell import Cell
+
+    rows = []
+    for n in [int(x) for x in args.tune_table.split(",") if x]:
+        for prec in ("fp64", "mixed"):
+            with Cell(n, seed=args.seed, precision=prec) as c:
+                if args.warmup:
+                    c.run(args.warmup, outputs=False, log_cap=0)
+                info = c.autotune(args.tune_steps)
+            by = {(k["layout"], k["graphs"]): k["ms"] for k in info["candidates"]}
+            quad = by[("quad", False)]
+            best = min(by.values())
+            rows.append(dict(nfru=n, precision=prec, chosen=info["layout"], graphs=info["graphs"],
+                             candidates_ms={f"{a}{'+graphs' if g else ''}": v for (a, g), v in by.items()},
+                             fastest_vs_quad=round(quad / best, 3)))
+            print(f"[bench] tune table: {n} FRUs {prec}: {info['layout']} (x{quad / best:.3f} over quad)",
+                  file=sys.stderr, flush=True)
+    return dict(table="autotuner trials", window_ms=[args.warmup, args.warmup + args.tune_steps], rows=rows)
+
+
 def main():
     args = parse()
+    if args.tune_table:
+        print(json.dumps(tune_table(args)), flush=True)
+        return
     if args.sweep:
         print(json.dumps(sweep(args)), flush=True)
         return
