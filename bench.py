@@ -602,7 +602,7 @@ def batched(args):
                            "divided by its cycle length, over the wall time; the window holds the stimulus, "
                            "upstroke and start of release, which cost more per simulated ms than the rest of "
                            "a beat, so whole beats run faster than this"),
-        layout=TUNED["batch"]["layout"] if "batch" in TUNED else args.layout,
+        layout=TUNED["batch"]["layout"] if "batch" in TUNED else ("quad" if args.layout == "waves" else args.layout),
         hip_graphs=bool(TUNED["batch"]["graphs"]) if "batch" in TUNED else bool(args.graphs),
         autotune=TUNED.get("batch"),
         rk_attempts=att, fru_steps=st["fru_steps"],
