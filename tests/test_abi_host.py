@@ -12,9 +12,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
+    """grw_* functions of every header in include/ (grw_mpi.h declares the
+    reference's own names, checked against libgrw_mpi.so below)."""
+    import glob
+
     names = set()
-    for h in ("grw_fru.h", "grw_cell.h"):
-        text = open(os.path.join(ROOT, "include", h)).read()
+    for path in sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))):
+        text = open(path).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names |= set(re.findall(r"\b(grw_[a-z0-9_]+)\s*\(", text))
     return names

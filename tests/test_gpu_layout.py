@@ -99,12 +99,12 @@ def test_wave_layout_is_refused_for_batches():
         assert L.grw_set_layout(cell.fru._h, 0) != 0
 
 
-@pytest.mark.parametrize("layout", ["quad3", "quad4"])
-def test_register_capped_layouts_equal_quad(layout):
+@pytest.mark.parametrize("layout,precision", [("quad3", "fp64"), ("quad4", "fp64"), ("quad3", "mixed")])
+def test_register_capped_layouts_equal_quad(layout, precision):
     n, seed, ms = 384, 31, 20
-    print(f"layout parity {layout} n={n} seed={seed} ms={ms}", flush=True)
-    q = _run(n, seed, "fp64", "quad", ms, graphs=True)
-    w = _run(n, seed, "fp64", layout, ms, graphs=True)
+    print(f"layout parity {layout} {precision} n={n} seed={seed} ms={ms}", flush=True)
+    q = _run(n, seed, precision, "quad", ms, graphs=True)
+    w = _run(n, seed, precision, layout, ms, graphs=True)
     _assert_same(q, w)
 
 
