@@ -165,8 +165,8 @@ class Batch:
         check(self._L.grw_set_timing(self._h, int(bool(on))))
 
     def set_layout(self, layout: str) -> None:
-        """Monte Carlo kernel of the batch ("quad", "quad3", "quad4"; see
-        Cell.set_layout)."""
+        """Monte Carlo kernel of the batch ("quad", "quad3", "quad4",
+        "queue"; see Cell.set_layout)."""
         codes = {v: k for k, v in LAYOUT_NAMES.items()}
         if layout not in codes:
             raise ValueError("layout must be one of %s" % sorted(codes))

@@ -183,6 +183,7 @@ struct Dev {
   uint32_t* mtw_h;
   double* part;              // [grid][kSlot] block partial sums
   unsigned int* ticket;      // last-block election counters, one per cell (self-resetting)
+  unsigned int* qhead;       // next work item of a queue-layout Monte Carlo launch
   double* out;               // [32]: 0-3 fluxes, 4-8 flux sums, 16-26 avg sums
   unsigned long long* nsteps;
   int* err;
@@ -929,11 +930,12 @@ __device__ __forceinline__ T in_vgpr(T x) {
   return x;
 }
 
-// One sub-interval of every FRU with four lanes per FRU; returns this lane's
-// FRU-step count (nonzero only in lane 0 of a quad) and its flux terms in acc.
+// One sub-interval of every FRU with four lanes per FRU; gid is the lane's
+// index in its cell's four-lane grid (quad slot x 4 + cleft).  Returns this
+// lane's FRU-step count (nonzero only in lane 0 of a quad) and its flux terms
+// in acc.
 template <class R>
-__device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[17], int blk) {
-  const int gid = (blk - a.b0) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, double acc[17], int gid) {
   const int slot = gid >> 2;
   const int c = gid & 3;
   unsigned steps = 0;
@@ -1220,6 +1222,11 @@ __device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d,
     }
   }
   return steps;
+}
+
+template <class R>
+__device__ __forceinline__ unsigned simfru4_body(const SimArgs& a, const Dev& d, double acc[17], int blk) {
+  return simfru4_gid<R>(a, d, acc, (blk - a.b0) * (int)blockDim.x + (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1655,36 +1662,15 @@ __device__ __forceinline__ unsigned simfru_ws(const SimArgs& a, const Dev& d, do
   return steps;
 }
 
-// The flux sums of a wave-specialised launch: the H wavefront's butterfly of
-// each block's 16 FRUs into the block's slot, the last block (ticket) adding
-// four consecutive slots in the order the four-lane kernel's block adds its
-// four wavefronts, then those in that kernel's order -- the same numbers.
+// The flux sums of a cell from the partials of its four-lane blocks' wavefronts
+// (slot 4 b + w for wavefront w of block b): each block's four added in
+// wavefront order, then the blocks in the order of reduce_range's last block
+// -- the same numbers as the four-lane kernel's block partials give.  The
+// whole block calls it; the totals are in tot_sh afterwards.
 template <int NV>
-__device__ __forceinline__ bool reduce_ws(double v[NV], const Dev& d, unsigned int* ticket, int blk, int b0, int nb,
-                                          double* tot_sh, int extra = 0) {
-  static_assert(NV <= kSlot, "block partials overflow their slot row");
+__device__ __forceinline__ void sum_wave_slots(const Dev& d, int b0, int nb, double* tot_sh) {
   __shared__ double wsum[4][NV];
-  __shared__ bool last;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  // A block's arrival at the ticket follows the completed stores of all its
-  // wavefronts (the fcn block's stage terms come from wavefronts 0 and 1).
-  __syncthreads();
-  if (ws_role() == kWsH) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = wave_sum_xor(v[k]);
-    if (lane == 0) {
-      if (blk >= 0)
-        for (int k = 0; k < NV; ++k)
-          __hip_atomic_store(&d.part[(size_t)blk * kSlot + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      __builtin_amdgcn_s_waitcnt(0);
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      last = (t == (unsigned)(4 * nb + extra) - 1u);
-    }
-  }
-  __syncthreads();
-  if (!last) return false;
   double s[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) s[k] = 0.0;
@@ -1712,8 +1698,41 @@ __device__ __forceinline__ bool reduce_ws(double v[NV], const Dev& d, unsigned i
       for (int w = 1; w < nw; ++w) a += wsum[w][k];
       tot_sh[k] = a;
     }
-    *ticket = 0u;
   }
+  __syncthreads();
+}
+
+// The flux sums of a wave-specialised launch: the H wavefront's butterfly of
+// each block's 16 FRUs into the block's slot, the last block (ticket) adding
+// four consecutive slots in the order the four-lane kernel's block adds its
+// four wavefronts, then those in that kernel's order -- the same numbers.
+template <int NV>
+__device__ __forceinline__ bool reduce_ws(double v[NV], const Dev& d, unsigned int* ticket, int blk, int b0, int nb,
+                                          double* tot_sh, int extra = 0) {
+  static_assert(NV <= kSlot, "block partials overflow their slot row");
+  __shared__ bool last;
+  const int lane = threadIdx.x & 63;
+  // A block's arrival at the ticket follows the completed stores of all its
+  // wavefronts (the fcn block's stage terms come from wavefronts 0 and 1).
+  __syncthreads();
+  if (ws_role() == kWsH) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum_xor(v[k]);
+    if (lane == 0) {
+      if (blk >= 0)
+        for (int k = 0; k < NV; ++k)
+          __hip_atomic_store(&d.part[(size_t)blk * kSlot + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = (t == (unsigned)(4 * nb + extra) - 1u);
+    }
+  }
+  __syncthreads();
+  if (!last) return false;
+  sum_wave_slots<NV>(d, b0, nb, tot_sh);
+  if (threadIdx.x == 0) *ticket = 0u;
   __syncthreads();
   return true;
 }
@@ -2194,11 +2213,11 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
 // flux reduction, and in the cell's last block fcn for this stage and then
 // the next stage or the step decision (rk_pd54m.c:124-333) and the caf.c
 // output loop (caf.c:150-251).
-template <class R, bool WS>
+template <class R, bool WS, bool TAIL = false>
 __device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkCtl* ctls) {
   // blocks [0, ncells): each cell's fcn block, dispatched first; then the
   // Monte Carlo blocks
-  const bool fblock = (int)blockIdx.x < base.ncells;
+  const bool fblock = TAIL || (int)blockIdx.x < base.ncells;
   const int blk = fblock ? -1 : mc_block(base, d, base.ncells);
   const int cell = fblock ? (int)blockIdx.x : (WS || base.ncells == 1) ? 0 : d.smap[blk];  // (one cell: no dependent load)
   const CellInfo ci = d.cells[cell];
@@ -2274,11 +2293,18 @@ __device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkC
   __shared__ double tot[17];
   // the eleven average sums (tot[6..16]) exist only in the attempt's last stage;
   // the fcn block is one more arrival
-  if (WS ? (a.avg ? !reduce_ws<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, tot, 1)
-                  : !reduce_ws<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, tot, 1))
-         : (a.avg ? !reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1)
-                  : !reduce_range<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1)))
+  if (TAIL) {  // the Monte Carlo work ran in the queue kernel before this launch
+    __syncthreads();  // the fcn terms' stores are complete in every wavefront
+    if (a.avg)
+      sum_wave_slots<17>(d, ci.sb0, ci.snb, tot);
+    else
+      sum_wave_slots<6>(d, ci.sb0, ci.snb, tot);
+  } else if (WS ? (a.avg ? !reduce_ws<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, tot, 1)
+                         : !reduce_ws<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, tot, 1))
+                : (a.avg ? !reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1)
+                         : !reduce_range<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1))) {
     return;
+  }
   const unsigned long long tail0 = wall_clock64();
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
   __shared__ double Jsh[4], pre[kPre];
@@ -2504,6 +2530,72 @@ __device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkC
 template <class R, bool WS>
 __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctls) {
   dev_stage<R, WS>(base, d, ctls);
+}
+
+// Queue layout, second kernel of a stage: one block per cell runs the cell's
+// fcn terms, adds the wavefront partials the queue kernel left and runs the
+// integrator stage (the tail of k_dev_stage, unchanged).  Resets the queue for
+// the next launch.
+template <class R>
+__global__ void __launch_bounds__(256) k_dev_tail(const SimArgs base, const Dev d, RkCtl* ctls) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d.qhead = 0u;
+  dev_stage<R, false, true>(base, d, ctls);
+}
+
+// Queue layout, first kernel of a stage: the Monte Carlo sub-interval of every
+// active cell, one work item per wavefront's worth (16 FRUs, wavefront w of a
+// four-lane block), taken from a counter in dispatch order (the batch's
+// costliest blocks first).  A wavefront that finishes its item takes the next
+// one instead of waiting for the other wavefronts of its block; the grid is
+// what the GPU holds at once.  Each item's partials go to slot 4 b + w, which
+// k_dev_tail adds in the four-lane kernel's order: the same numbers.  No
+// wavefront waits for another: the launch ends when the counter runs past the
+// last item.
+template <class R>
+__global__ void __launch_bounds__(256) k_mc_queue(const SimArgs base, const Dev d, RkCtl* ctls, int nitems) {
+  const int lane = threadIdx.x & 63;
+  while (true) {
+    unsigned item = 0;
+    if (lane == 0) item = atomicAdd(d.qhead, 1u);
+    item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+    if (item >= (unsigned)nitems) break;
+    const int i = (int)(item >> 2), w = (int)(item & 3u);
+    const int blk = base.sorted ? d.border[i] : i;
+    const int cell = base.ncells == 1 ? 0 : d.smap[blk];
+    const CellInfo ci = d.cells[cell];
+    const RkCtl* ctl = ctls + cell;
+    if (!ctl->active) continue;
+    SimArgs a = base;
+    a.t0 = ctl->ta;
+    a.t1 = ctl->tb;
+    for (int q = 0; q < 3; ++q) {
+      a.dep0[q] = ctl->dep0[q];
+      a.depf[q] = ctl->depf[q];
+    }
+    a.nfru = ci.nfru;
+    a.f0 = ci.f0;
+    a.b0 = ci.sb0;
+    a.k23c = ci.k23c;
+    a.gcf = ci.gcf;
+    const int s0 = ctl->stage;
+    a.pro = (s0 == 1) ? ctl->pro : 0;
+    a.avg = (s0 == 5);
+    double acc[17] = {};
+    count_steps(d, simfru4_gid<R>(a, d, acc, (blk - a.b0) * 256 + w * 64 + lane));
+    double* const slot = d.part + (size_t)(4 * blk + w) * kSlot;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double v = wave_sum_xor(acc[k]);
+      if (lane == 0) slot[k] = v;
+    }
+    if (a.avg) {
+#pragma unroll
+      for (int k = 6; k < 17; ++k) {
+        const double v = wave_sum_xor(acc[k]);
+        if (lane == 0) slot[k] = v;
+      }
+    }
+  }
 }
 
 // The four-lane stage with its registers capped so that W wavefronts fit on a
@@ -2842,6 +2934,8 @@ struct grw_ctx {
   bool graphs = false;             // device-driven runs replay captured attempts
   bool ws = false;                 // device integrator: wave-specialised Monte Carlo kernel (single cells)
   int occ = 0;                     // 0: the stage kernel as compiled; 3, 4: registers capped for 3 / 4 waves per SIMD
+  bool queue = false;              // device integrator: Monte Carlo work queue + per-cell tail kernel
+  int qgrid = 0;                   // blocks of the queue kernel: as many as the GPU holds at once
   hipGraphExec_t graph = nullptr;  // kGraphAttempts RK attempts, captured on first use
   SimArgs graph_base{};            // the launch arguments the graph holds
   Dev graph_dev{};
@@ -2941,7 +3035,7 @@ int alloc_fru_state(grw_ctx* c) {
       (rc = dalloc(c, &c->d.mt, ld * kMtStride)) || (rc = dalloc(c, &c->d.mtw_h, ld)) ||
       (rc = dalloc(c, &c->d.conc_h, 5 * ld)) || (rc = dalloc(c, &c->d.chan_h, 4 * ld)) ||
       (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, nslots * kSlot)) ||
-      (rc = dalloc(c, &c->d.ticket, (size_t)c->ncells)) || (rc = dalloc(c, &c->d.out, 32)) ||
+      (rc = dalloc(c, &c->d.ticket, (size_t)c->ncells)) || (rc = dalloc(c, &c->d.qhead, 1)) || (rc = dalloc(c, &c->d.out, 32)) ||
       (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
       (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
       (rc = dalloc(c, &c->d.tailticks, 1)) || (rc = dalloc(c, &c->d.perm, ld)) ||
@@ -3070,7 +3164,20 @@ void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent
   const dim3 grid(c->ncells + mc_grid(c));
   for (int s5 = 0; s5 < 5; ++s5) {
     hipEvent_t e0 = ek ? ek[2 * s5] : nullptr, e1 = ek ? ek[2 * s5 + 1] : nullptr;
-    if (c->occ == 3 || c->occ == 4) {
+    if (c->queue) {
+      const int nitems = 4 * c->grid_s;
+      if (mixed) {
+        hipExtLaunchKernelGGL(k_mc_queue<float>, dim3(c->qgrid), dim3(256), 0, c->stream, e0, nullptr, 0, base, c->d,
+                              c->dctl, nitems);
+        hipExtLaunchKernelGGL(k_dev_tail<float>, dim3(c->ncells), dim3(256), 0, c->stream, nullptr, e1, 0, base, c->d,
+                              c->dctl);
+      } else {
+        hipExtLaunchKernelGGL(k_mc_queue<double>, dim3(c->qgrid), dim3(256), 0, c->stream, e0, nullptr, 0, base,
+                              c->d, c->dctl, nitems);
+        hipExtLaunchKernelGGL(k_dev_tail<double>, dim3(c->ncells), dim3(256), 0, c->stream, nullptr, e1, 0, base,
+                              c->d, c->dctl);
+      }
+    } else if (c->occ == 3 || c->occ == 4) {
       if (c->occ == 3) {
         if (mixed)
           hipExtLaunchKernelGGL(k_dev_stage_occ<float, 3>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d, c->dctl);
@@ -3616,18 +3723,35 @@ int grw_set_timing(grw_ctx* c, int on) {
 
 int grw_set_layout(grw_ctx* c, int layout) {
   if (!c) return fail(GRW_EINVAL, "null context");
-  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_QUAD4)
-    return fail(GRW_EINVAL, "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4");
+  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_QUEUE)
+    return fail(GRW_EINVAL, "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4, _QUEUE");
   if (layout == GRW_LAYOUT_WAVES && (c->batch || c->ncells != 1))
     return fail(GRW_EINVAL, "the wave-specialised layout runs single-cell contexts only");
+  // The queue layout is switched off: the one run of it on the device (a
+  // 384-FRU cell) did not finish, and no cause has been found in its code.
+  if (layout == GRW_LAYOUT_QUEUE)
+    return fail(GRW_EINVAL, "the queue layout is switched off (a device run of it did not finish)");
   const bool ws = layout == GRW_LAYOUT_WAVES;
+  const bool queue = layout == GRW_LAYOUT_QUEUE;
   const int occ = layout == GRW_LAYOUT_QUAD3 ? 3 : layout == GRW_LAYOUT_QUAD4 ? 4 : 0;
-  if ((ws != c->ws || occ != c->occ) && c->graph) {  // the captured launches are the other kernel's
+  if (queue && c->qgrid == 0) {  // as many queue blocks as the GPU holds at once
+    int dev = 0, ncu = 0, per = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const bool mixed = c->cfg.precision == GRW_MIXED;
+    if (mixed)
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc_queue<float>, 256, 0));
+    else
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc_queue<double>, 256, 0));
+    c->qgrid = std::max(1, std::min(c->grid_s, std::max(per, 1) * std::max(ncu, 1)));
+  }
+  if ((ws != c->ws || occ != c->occ || queue != c->queue) && c->graph) {  // the captured launches are another kernel's
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
   }
   c->ws = ws;
   c->occ = occ;
+  c->queue = queue;
   return 0;
 }
 
@@ -3975,7 +4099,11 @@ int autotune(grw_ctx* c, const std::vector<int>& layouts, const std::function<in
              GrwTune* out) {
   const bool ws0 = c->ws, g0 = c->graphs;
   const int occ0 = c->occ;
-  const int layout0 = ws0 ? GRW_LAYOUT_WAVES : occ0 == 3 ? GRW_LAYOUT_QUAD3 : occ0 == 4 ? GRW_LAYOUT_QUAD4 : GRW_LAYOUT_QUAD;
+  const int layout0 = c->queue ? GRW_LAYOUT_QUEUE
+                      : ws0       ? GRW_LAYOUT_WAVES
+                      : occ0 == 3 ? GRW_LAYOUT_QUAD3
+                      : occ0 == 4 ? GRW_LAYOUT_QUAD4
+                                  : GRW_LAYOUT_QUAD;
   CtxSnapshot snap;
   if (int rc = snapshot_take(c, snap)) return rc;
   GrwTune r{};

@@ -94,7 +94,12 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
  *                     the Heun step of every micro-step, exchanging through LDS;
  *   GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4  the four-lane kernel with its registers
  *                     capped for three or four wavefronts per SIMD (latency
- *                     hiding for batches, against spills).
+ *                     hiding for batches, against spills);
+ *   GRW_LAYOUT_QUEUE  the four-lane kernel's work as a queue of 16-FRU items
+ *                     that each wavefront takes one after another, then one
+ *                     block per cell for the cell's fcn and integrator stage;
+ *                     switched off: grw_set_layout refuses it (a device run of
+ *                     it did not finish and the cause is not known).
  * Every layout gives the same results bit for bit; the speed differs with the
  * workload (grw_autotune_cell / grw_autotune_batch time them).  No counterpart
  * in the reference. */
@@ -102,6 +107,7 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
 #define GRW_LAYOUT_WAVES 2
 #define GRW_LAYOUT_QUAD3 3
 #define GRW_LAYOUT_QUAD4 4
+#define GRW_LAYOUT_QUEUE 5
 int grw_set_layout(grw_ctx *ctx, int layout);
 
 /* Result of an autotuning pass: the choice (now set on the context) and every

@@ -94,13 +94,20 @@ def test_wave_layout_is_refused_for_batches():
         assert L.grw_set_layout(bt._h, 2) != 0
         assert L.grw_set_layout(bt._h, 1) == 0
         assert L.grw_set_layout(bt._h, 4) == 0
+        assert L.grw_set_layout(bt._h, 5) != 0  # the queue layout is switched off
     with Cell(64) as cell:
-        assert L.grw_set_layout(cell.fru._h, 5) != 0
+        assert L.grw_set_layout(cell.fru._h, 6) != 0
         assert L.grw_set_layout(cell.fru._h, 0) != 0
 
 
-@pytest.mark.parametrize("layout,precision", [("quad3", "fp64"), ("quad4", "fp64"), ("quad3", "mixed")])
-def test_register_capped_layouts_equal_quad(layout, precision):
+QUEUE_OFF = pytest.mark.skip(reason="the queue layout is switched off: its one device run (384 FRUs, FP64, "
+                                     "20 ms) did not finish and the cause is not known")
+
+
+@pytest.mark.parametrize("layout,precision", [("quad3", "fp64"), ("quad4", "fp64"), ("quad3", "mixed"),
+                                              pytest.param("queue", "fp64", marks=QUEUE_OFF),
+                                              pytest.param("queue", "mixed", marks=QUEUE_OFF)])
+def test_other_layouts_equal_quad(layout, precision):
     n, seed, ms = 384, 31, 20
     print(f"layout parity {layout} {precision} n={n} seed={seed} ms={ms}", flush=True)
     q = _run(n, seed, precision, "quad", ms, graphs=True)
