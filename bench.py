@@ -941,9 +941,13 @@ def main():
         cores = host_cores()
         w0, w1 = args.warmup, args.warmup + min(args.cpu_ms, args.steps)
         line["cpu_baseline"] = cpu_baseline(args.nfru, w0, w1, args.seed, cores)
-        # the GPU's rate over the same window: the timed run's first chunk
-        n0, s0 = RUN_WALLS[args.precision][0]
-        gpu_same = n0 / s0
+        # the GPU's rate over the same window: the timed run's first chunk, or
+        # the whole timed run when the window covers it
+        if args.cpu_ms < args.steps:
+            n0, s0 = RUN_WALLS[args.precision][0]
+            gpu_same = n0 / s0
+        else:
+            gpu_same = value
         line["cpu_baseline"]["gpu_value_same_window"] = round(gpu_same, 4)
         line["vs_cpu_baseline"] = round(gpu_same / line["cpu_baseline"]["value"], 2)
         if args.nfru == 12500 and args.precision == "fp64":
