@@ -1036,7 +1036,7 @@ __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, 
       lcc_arith<R>(ls, alpha, beta, bp, gcf * cc, lom[(lk >= 0 && lk <= 4) ? lk : 0][tl], lb, lf, lm, lt);
       const R vr = rdiv(ch_y(w) == 2 ? (R(1.0) - yinf) : yinf, tau);
       const R ir = (ch_ito2(w) == 1) ? R(kfIto2) * cc : R(kbIto2);
-      R maxr = rmax(rmax(rmax(R(0.0), lt), vr), ir);
+      R maxr = vmax(vmax(vmax(R(0.0), lt), vr), ir);
       const RyrK<R> k = ryr_k(cc, k23c);
       int st[kNryr];
       int nd = 0;
@@ -1088,20 +1088,20 @@ __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, 
       for (int r = 0; r < kNryr; ++r) {
         w = ch_set_ryr(w, r, st[r]);
         tr[r] = rtot[st[r] - 1][tl];  // total of the resolved state (fru_rates_local.c:390-727)
-        maxr = rmax(maxr, tr[r]);
+        maxr = vmax(maxr, tr[r]);
       }
-      maxr = rmax(maxr, qperm<QX1>(maxr));
-      maxr = rmax(maxr, qperm<QX2>(maxr));
+      maxr = vmax(maxr, qperm<QX1>(maxr));
+      maxr = vmax(maxr, qperm<QX2>(maxr));
       bad |= qperm<QX1>(bad);
       bad |= qperm<QX2>(bad);
       // ---- step length (simfru_local.c:135-146)
       double dt = 0.1 / (double)maxr;
-      dt = rmin(dt, 10.001e-3);
-      double cm = rmax(css, qperm<QX1>(css));
-      cm = rmax(cm, qperm<QX2>(cm));
-      const double cmax = rmax(0.e-3, cm);
-      if (cmax > 1.5e-3) dt = rmin(dt, 5.001e-3);
-      dt = rmin(dt, t1 - t);
+      dt = vmin(dt, 10.001e-3);
+      double cm = vmax(css, qperm<QX1>(css));
+      cm = vmax(cm, qperm<QX2>(cm));
+      const double cmax = vmax(0.e-3, cm);
+      if (cmax > 1.5e-3) dt = vmin(dt, 5.001e-3);
+      dt = vmin(dt, t1 - t);
       // an unknown channel state, non-finite rates or a runaway (quad-uniform)
       if (bad || !(t + dt > t) || ++nguard > kStepCap) {
         bad = 1;
@@ -1504,18 +1504,18 @@ __device__ __forceinline__ unsigned simfru_ws(const SimArgs& a, const Dev& d, do
       lcc_arith<R>(ls, alpha, beta, bp, gcf * cc, lom[(lk >= 0 && lk <= 4) ? lk : 0][tl], lb, lf, lm, lt);
       vr = rdiv(ch_y(w) == 2 ? (R(1.0) - yinf) : yinf, tau);
       ir = (ch_ito2(w) == 1) ? R(kfIto2) * cc : R(kbIto2);
-      mpart = rmax(rmax(rmax(R(0.0), lt), vr), ir);
+      mpart = vmax(vmax(vmax(R(0.0), lt), vr), ir);
       mt_twist_part<0, 1, 7>(idx + E + 8 * c, winL, dw);
     } else if (role == kWsRA) {
       tr[0] = rtot[st[0] - 1][tl];
       tr[1] = rtot[st[1] - 1][tl];
-      mpart = rmax(tr[0], tr[1]);
+      mpart = vmax(tr[0], tr[1]);
       mt_twist_part<2, 3>(idx + E + 8 * c, winA, dw);
     } else if (role == kWsRB) {
       tr[2] = rtot[st[2] - 1][tl];
       tr[3] = rtot[st[3] - 1][tl];
       tr[4] = rtot[st[4] - 1][tl];
-      mpart = rmax(rmax(tr[2], tr[3]), tr[4]);
+      mpart = vmax(vmax(tr[2], tr[3]), tr[4]);
       mt_twist_part<4, 5, 6>(idx + E + 8 * c, winB, dw);
     } else {  // H: the first evaluation of the Heun step needs no step length
       fcn_fru4<R>((R)cj, cc, gv, (R)dCai, (R)dNSR, lopen, nopen, djs1, dss1);
@@ -1526,19 +1526,19 @@ __device__ __forceinline__ unsigned simfru_ws(const SimArgs& a, const Dev& d, do
     }
     __syncthreads();  // barrier 1: the rates of every role
     // ---- step length (simfru_local.c:135-146), the same in every wavefront
-    R maxr = rmax(rmax((R)x_max[kWsL][lane], (R)x_max[kWsRA][lane]), (R)x_max[kWsRB][lane]);
+    R maxr = vmax(vmax((R)x_max[kWsL][lane], (R)x_max[kWsRA][lane]), (R)x_max[kWsRB][lane]);
     bad = x_bad[kWsL][lane] | x_bad[kWsRA][lane] | x_bad[kWsRB][lane];
-    maxr = rmax(maxr, qperm<QX1>(maxr));
-    maxr = rmax(maxr, qperm<QX2>(maxr));
+    maxr = vmax(maxr, qperm<QX1>(maxr));
+    maxr = vmax(maxr, qperm<QX2>(maxr));
     bad |= qperm<QX1>(bad);
     bad |= qperm<QX2>(bad);
     double dt = 0.1 / (double)maxr;
-    dt = rmin(dt, 10.001e-3);
-    double cm = rmax(css, qperm<QX1>(css));
-    cm = rmax(cm, qperm<QX2>(cm));
-    const double cmax = rmax(0.e-3, cm);
-    if (cmax > 1.5e-3) dt = rmin(dt, 5.001e-3);
-    dt = rmin(dt, t1 - t);
+    dt = vmin(dt, 10.001e-3);
+    double cm = vmax(css, qperm<QX1>(css));
+    cm = vmax(cm, qperm<QX2>(cm));
+    const double cmax = vmax(0.e-3, cm);
+    if (cmax > 1.5e-3) dt = vmin(dt, 5.001e-3);
+    dt = vmin(dt, t1 - t);
     if (bad || !(t + dt > t) || ++nguard > kStepCap) {
       bad = 1;
       break;

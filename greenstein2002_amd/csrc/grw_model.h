@@ -72,6 +72,19 @@ constexpr double threshMAXCa = 0.0504410547074504;
 template <class T> GRW_HD T rmax(T a, T b) { return (a < b) ? b : a; }
 template <class T> GRW_HD T rmin(T a, T b) { return (a < b) ? a : b; }
 
+#if defined(__HIPCC__)
+// IEEE maxNum / minNum, one v_max / v_min instruction instead of a compare and
+// two selects.  They equal rmax / rmin whenever the first operand is not NaN
+// (a NaN second operand is dropped by both) and the operands are not zeros of
+// opposite sign.  The Monte Carlo step uses them where that holds: the largest
+// exit rate is an accumulator that starts at +0 and takes only larger rates,
+// and step lengths and subspace Ca are positive.
+__device__ __forceinline__ double vmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
+#endif
+
 // Division and exp in the rate and local-Ca arithmetic: IEEE in FP64 (the
 // drop-in keeps the reference's operations bit for bit); in the FP32 mixed form
 // (device only) the hardware reciprocal and exp2 (v_rcp_f32, v_exp_f32, about
