@@ -2532,7 +2532,8 @@ __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev
   dev_stage<R, WS>(base, d, ctls);
 }
 
-// Queue layout, second kernel of a stage: one block per cell runs the cell's
+// Queue layout (switched off: grw_set_layout refuses GRW_LAYOUT_QUEUE, since
+// its one device run did not finish), second kernel of a stage: one block per cell runs the cell's
 // fcn terms, adds the wavefront partials the queue kernel left and runs the
 // integrator stage (the tail of k_dev_stage, unchanged).  Resets the queue for
 // the next launch.
@@ -2542,7 +2543,7 @@ __global__ void __launch_bounds__(256) k_dev_tail(const SimArgs base, const Dev 
   dev_stage<R, false, true>(base, d, ctls);
 }
 
-// Queue layout, first kernel of a stage: the Monte Carlo sub-interval of every
+// Queue layout (switched off, see k_dev_tail), first kernel of a stage: the Monte Carlo sub-interval of every
 // active cell, one work item per wavefront's worth (16 FRUs, wavefront w of a
 // four-lane block), taken from a counter in dispatch order (the batch's
 // costliest blocks first).  A wavefront that finishes its item takes the next
