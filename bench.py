@@ -536,8 +536,9 @@ def run_cell(args, precision, timing):
     return wall, chunks, st, beats
 
 
-def batched(args):
-    """256 uneven cells in one batch vs the same cells one at a time."""
+def batched(args, precision: str | None = None, sequential: bool = True):
+    """256 uneven cells in one batch vs the same cells one at a time (the
+    sequential sample only when `sequential`)."""
     import numpy as np
 
     from greenstein2002_amd.batch import Batch, CellSpec
@@ -548,11 +549,13 @@ def batched(args):
     nf = rng.integers(1250, 12501, n)
     cl = rng.uniform(300.0, 2000.0, n)
     specs = [CellSpec(int(nf[k]), seed=1234 + k, protocol=Protocol(period=float(cl[k]))) for k in range(n)]
-    b = Batch(specs, precision=args.precision)
+    prec = precision or args.precision
+    key = "batch" if prec == args.precision else "batch_" + prec
+    b = Batch(specs, precision=prec)
     b.set_graphs(bool(args.graphs))
     if args.layout == "auto":
-        TUNED["batch"] = b.autotune(2)
-        print(f"[bench] batch: autotuner chose {TUNED['batch']['layout']}, graphs {TUNED['batch']['graphs']}",
+        TUNED[key] = b.autotune(2)
+        print(f"[bench] batch ({prec}): autotuner chose {TUNED[key]['layout']}, graphs {TUNED[key]['graphs']}",
               file=sys.stderr, flush=True)
     elif args.layout != "waves":
         b.set_layout(args.layout)
@@ -576,13 +579,13 @@ def batched(args):
     avg_s = st["kernel_ms"] / 1e3 / max(st["timed_launches"], 1)
     achieved = alg / max(st["launches"], 1) / avg_s / 1e9 if st["kernel_ms"] else 0.0
     # the same cells one at a time (an evenly spaced sample of them)
-    pick = np.unique(np.linspace(0, n - 1, max(1, min(args.batch_seq_sample, n))).astype(int))
+    pick = np.unique(np.linspace(0, n - 1, max(1, min(args.batch_seq_sample, n))).astype(int)) if sequential else []
     seq = []
     for k in pick:
-        c = Cell(int(nf[k]), seed=1234 + int(k), precision=args.precision, protocol=Protocol(period=float(cl[k])))
-        if args.precision in TUNED:  # the single cell's tuned settings
-            c.set_layout(TUNED[args.precision]["layout"])
-            c.fru.set_graphs(TUNED[args.precision]["graphs"])
+        c = Cell(int(nf[k]), seed=1234 + int(k), precision=prec, protocol=Protocol(period=float(cl[k])))
+        if prec in TUNED:  # the single cell's tuned settings
+            c.set_layout(TUNED[prec]["layout"])
+            c.fru.set_graphs(TUNED[prec]["graphs"])
         else:
             c.fru.set_graphs(bool(args.graphs))
         ts = time.perf_counter()
@@ -590,11 +593,11 @@ def batched(args):
         seq.append(time.perf_counter() - ts)
         c.close()
         print(f"[bench] batch cell {k} alone: {seq[-1]:.2f} s wall", file=sys.stderr, flush=True)
-    seq_wall = float(np.mean(seq)) * n
+    seq_wall = float(np.mean(seq)) * n if seq else None
     beats = float(np.sum(args.batch_ms / cl))
     return dict(
         workload=f"{n} cells, FRUs uniform in [1250, 12500] (total {int(nf.sum())}), cycle lengths uniform "
-                 f"in [300, 2000] ms, t = 0..{args.batch_ms} ms from ic/stable, {args.precision}",
+                 f"in [300, 2000] ms, t = 0..{args.batch_ms} ms from ic/stable, {prec}",
         cells=n, frus=int(nf.sum()), window_ms=args.batch_ms, wall_s=round(wall, 3),
         cell_ms_per_s=round(n * args.batch_ms / wall, 2),
         beats_per_s_est=round(beats / wall, 4),
@@ -602,19 +605,19 @@ def batched(args):
                            "divided by its cycle length, over the wall time; the window holds the stimulus, "
                            "upstroke and start of release, which cost more per simulated ms than the rest of "
                            "a beat, so whole beats run faster than this"),
-        layout=TUNED["batch"]["layout"] if "batch" in TUNED else ("quad" if args.layout == "waves" else args.layout),
-        hip_graphs=bool(TUNED["batch"]["graphs"]) if "batch" in TUNED else bool(args.graphs),
-        autotune=TUNED.get("batch"),
+        layout=TUNED[key]["layout"] if key in TUNED else ("quad" if args.layout == "waves" else args.layout),
+        hip_graphs=bool(TUNED[key]["graphs"]) if key in TUNED else bool(args.graphs),
+        autotune=TUNED.get(key),
         rk_attempts=att, fru_steps=st["fru_steps"],
         fru_steps_per_s=round(st["fru_steps"] / wall, 1),
         roofline=dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=round(achieved / HBM_PEAK_GBS, 6),
-                      kernel="k_dev_stage<%s> over all cells" % ("double" if args.precision == "fp64" else "float"),
+                      kernel="k_dev_stage<%s> over all cells" % ("double" if prec == "fp64" else "float"),
                       avg_launch_us=round(avg_s * 1e6, 3),
                       launches=st["launches"], timed_launches=st["timed_launches"]),
         sequential=dict(sample_cells=[int(k) for k in pick], sample_walls_s=[round(x, 3) for x in seq],
-                        est_wall_s=round(seq_wall, 2), beats_per_s_est=round(beats / seq_wall, 4)),
-        speedup_vs_sequential=round(seq_wall / wall, 2))
+                        est_wall_s=round(seq_wall, 2), beats_per_s_est=round(beats / seq_wall, 4)) if seq else None,
+        speedup_vs_sequential=round(seq_wall / wall, 2) if seq else None)
 
 
 def ensemble(args) -> dict:
@@ -932,6 +935,13 @@ def main():
                              if mbeat and fbeat else None)
     if args.batch_cells > 0:
         line["batched"] = batched(args)
+        if args.precision == "fp64" and not args.no_mixed:
+            bm = batched(args, "mixed", sequential=False)
+            line["batched"]["mixed"] = dict(cell_ms_per_s=bm["cell_ms_per_s"], beats_per_s_est=bm["beats_per_s_est"],
+                                            fru_steps_per_s=bm["fru_steps_per_s"], wall_s=bm["wall_s"],
+                                            layout=bm["layout"], hip_graphs=bm["hip_graphs"],
+                                            speedup_vs_fp64=round(bm["cell_ms_per_s"] /
+                                                                  line["batched"]["cell_ms_per_s"], 4))
     if roof is not None and not args.no_traffic and args.driver == "device" and args.precision == "fp64":
         tr = measure_traffic(args)
         if tr is not None:
