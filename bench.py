@@ -81,7 +81,7 @@ def parse():
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
-    ap.add_argument("--layout", choices=["auto", "quad", "waves", "quad3", "quad4"], default="auto",
+    ap.add_argument("--layout", choices=["auto", "quad", "waves", "quad3", "quad4", "wave1"], default="auto",
                     help="Monte Carlo kernel of the device integrator (grw_set_layout); auto: the autotuner "
                          "picks it and the graph replays from timed trials (grw_autotune_cell / _batch)")
     ap.add_argument("--tune-steps", type=int, default=10, help="simulated ms of each single-cell autotuner trial")
@@ -536,6 +536,14 @@ def run_cell(args, precision, timing):
     return wall, chunks, st, beats
 
 
+def stage_kernels(layout: str, prec: str) -> str:
+    """The kernels of one RK stage under a layout (what a stage's timed span covers)."""
+    t = "double" if prec == "fp64" else "float"
+    if layout == "wave1":
+        return "k_mc_wave<%s> + k_dev_tail<%s> (one RK stage)" % (t, t)
+    return "k_dev_stage<%s>" % t
+
+
 def batched(args, precision: str | None = None, sequential: bool = True):
     """256 uneven cells in one batch vs the same cells one at a time (the
     sequential sample only when `sequential`)."""
@@ -612,7 +620,8 @@ def batched(args, precision: str | None = None, sequential: bool = True):
         fru_steps_per_s=round(st["fru_steps"] / wall, 1),
         roofline=dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=round(achieved / HBM_PEAK_GBS, 6),
-                      kernel="k_dev_stage<%s> over all cells" % ("double" if prec == "fp64" else "float"),
+                      kernel=stage_kernels(TUNED[key]["layout"] if key in TUNED else args.layout, prec) +
+                      " over all cells",
                       avg_launch_us=round(avg_s * 1e6, 3),
                       launches=st["launches"], timed_launches=st["timed_launches"]),
         sequential=dict(sample_cells=[int(k) for k in pick], sample_walls_s=[round(x, 3) for x in seq],

@@ -69,7 +69,7 @@ class GrwRkStats(C.Structure):
 
 
 TUNE_MAX = 8  # GRW_TUNE_MAX
-LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4", 5: "queue"}
+LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4", 5: "queue", 6: "wave1"}
 
 
 class GrwTune(C.Structure):
@@ -251,8 +251,10 @@ class Cell:
         wave-specialised kernel (four wavefronts per 16 FRUs); "quad3" /
         "quad4", the four-lane kernel with registers capped for three / four
         wavefronts per SIMD; "queue", the four-lane kernel's work as a queue
-        of 16-FRU items per wavefront, then a per-cell stage kernel.  All give
-        the same results bit for bit."""
+        of 16-FRU items per wavefront, then a per-cell stage kernel (switched
+        off); "wave1", the four-lane kernel's work with one wavefront per
+        block, then a per-cell stage kernel.  All give the same results bit
+        for bit."""
         codes = {v: k for k, v in LAYOUT_NAMES.items()}
         if layout not in codes:
             raise ValueError("layout must be one of %s" % sorted(codes))

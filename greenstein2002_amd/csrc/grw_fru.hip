@@ -933,8 +933,9 @@ __device__ __forceinline__ T in_vgpr(T x) {
 // One sub-interval of every FRU with four lanes per FRU; gid is the lane's
 // index in its cell's four-lane grid (quad slot x 4 + cleft).  Returns this
 // lane's FRU-step count (nonzero only in lane 0 of a quad) and its flux terms
-// in acc.
-template <class R>
+// in acc.  NT is the block's thread count (the width of the lane-private LDS
+// columns).
+template <class R, int NT = 256>
 __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, double acc[17], int gid) {
   const int slot = gid >> 2;
   const int c = gid & 3;
@@ -989,13 +990,13 @@ __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, 
     // that a RyR's total is one indexed read instead of a chain of selects
     // over its state.  A lane reads only what it wrote itself, in program
     // order, so no barrier is needed.  States 4 and 6 have constant totals.
-    __shared__ R rtot[8][256];
+    __shared__ R rtot[8][NT];
     const int tl = threadIdx.x;
     rtot[3][tl] = R(k43 + k45);
     rtot[5][tl] = R(k65);
     // omega / bL^k of the mode-Ca LCC states CCa0..CCa4, likewise one column
     // per lane (fru_rates_local.c:70-76 constant expressions)
-    __shared__ R lom[5][256];
+    __shared__ R lom[5][NT];
     lom[0][tl] = R(omega);
     lom[1][tl] = R(omega / bL);
     lom[2][tl] = R(omega / bL2);
@@ -2599,6 +2600,58 @@ __global__ void __launch_bounds__(256) k_mc_queue(const SimArgs base, const Dev 
   }
 }
 
+// Single-wavefront blocks, first kernel of a stage: the Monte Carlo
+// sub-interval of every active cell with one wavefront per block -- block i
+// runs wavefront i & 3 of four-lane block i >> 2 (in the batch's dispatch
+// order, costliest first), 16 FRUs.  A 256-thread block holds the registers of
+// its four wavefronts until the slowest is done; here a wavefront's registers
+// return to its SIMD as soon as its own FRUs are, and the next block starts
+// there.  The partials of wavefront w of four-lane block b go to slot 4 b + w;
+// k_dev_tail (one block per cell, after this kernel) adds them in the
+// four-lane kernel's order, evaluates the stage's fcn and runs the integrator
+// stage: the same numbers as GRW_LAYOUT_QUAD.  No block waits for another.
+template <class R>
+__global__ void __launch_bounds__(64) k_mc_wave(const SimArgs base, const Dev d, RkCtl* ctls) {
+  const int i = (int)blockIdx.x;
+  const int blk = base.sorted ? d.border[i >> 2] : (i >> 2);
+  const int w = i & 3;
+  const int lane = threadIdx.x;
+  const int cell = base.ncells == 1 ? 0 : d.smap[blk];
+  const CellInfo ci = d.cells[cell];
+  const RkCtl* ctl = ctls + cell;
+  if (!ctl->active) return;
+  SimArgs a = base;
+  a.t0 = ctl->ta;
+  a.t1 = ctl->tb;
+  for (int q = 0; q < 3; ++q) {
+    a.dep0[q] = ctl->dep0[q];
+    a.depf[q] = ctl->depf[q];
+  }
+  a.nfru = ci.nfru;
+  a.f0 = ci.f0;
+  a.b0 = ci.sb0;
+  a.k23c = ci.k23c;
+  a.gcf = ci.gcf;
+  const int s0 = ctl->stage;
+  a.pro = (s0 == 1) ? ctl->pro : 0;
+  a.avg = (s0 == 5);
+  double acc[17] = {};
+  count_steps(d, simfru4_gid<R, 64>(a, d, acc, (blk - a.b0) * 256 + w * 64 + lane));
+  double* const slot = d.part + (size_t)(4 * blk + w) * kSlot;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const double v = wave_sum_xor(acc[k]);
+    if (lane == 0) slot[k] = v;
+  }
+  if (a.avg) {
+#pragma unroll
+    for (int k = 6; k < 17; ++k) {
+      const double v = wave_sum_xor(acc[k]);
+      if (lane == 0) slot[k] = v;
+    }
+  }
+}
+
 // The four-lane stage with its registers capped so that W wavefronts fit on a
 // SIMD (512 / W VGPRs): more wavefronts to hide latency when a batch has more
 // of them than the chip holds at once, against spills.  Same arithmetic, same
@@ -2936,6 +2989,7 @@ struct grw_ctx {
   bool ws = false;                 // device integrator: wave-specialised Monte Carlo kernel (single cells)
   int occ = 0;                     // 0: the stage kernel as compiled; 3, 4: registers capped for 3 / 4 waves per SIMD
   bool queue = false;              // device integrator: Monte Carlo work queue + per-cell tail kernel
+  bool wave1 = false;              // device integrator: single-wavefront Monte Carlo blocks + per-cell tail kernel
   int qgrid = 0;                   // blocks of the queue kernel: as many as the GPU holds at once
   hipGraphExec_t graph = nullptr;  // kGraphAttempts RK attempts, captured on first use
   SimArgs graph_base{};            // the launch arguments the graph holds
@@ -3165,7 +3219,18 @@ void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent
   const dim3 grid(c->ncells + mc_grid(c));
   for (int s5 = 0; s5 < 5; ++s5) {
     hipEvent_t e0 = ek ? ek[2 * s5] : nullptr, e1 = ek ? ek[2 * s5 + 1] : nullptr;
-    if (c->queue) {
+    if (c->wave1) {
+      const dim3 wgrid(4 * c->grid_s);
+      if (mixed) {
+        hipExtLaunchKernelGGL(k_mc_wave<float>, wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d, c->dctl);
+        hipExtLaunchKernelGGL(k_dev_tail<float>, dim3(c->ncells), dim3(256), 0, c->stream, nullptr, e1, 0, base, c->d,
+                              c->dctl);
+      } else {
+        hipExtLaunchKernelGGL(k_mc_wave<double>, wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d, c->dctl);
+        hipExtLaunchKernelGGL(k_dev_tail<double>, dim3(c->ncells), dim3(256), 0, c->stream, nullptr, e1, 0, base,
+                              c->d, c->dctl);
+      }
+    } else if (c->queue) {
       const int nitems = 4 * c->grid_s;
       if (mixed) {
         hipExtLaunchKernelGGL(k_mc_queue<float>, dim3(c->qgrid), dim3(256), 0, c->stream, e0, nullptr, 0, base, c->d,
@@ -3724,8 +3789,8 @@ int grw_set_timing(grw_ctx* c, int on) {
 
 int grw_set_layout(grw_ctx* c, int layout) {
   if (!c) return fail(GRW_EINVAL, "null context");
-  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_QUEUE)
-    return fail(GRW_EINVAL, "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4, _QUEUE");
+  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_WAVE1)
+    return fail(GRW_EINVAL, "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4, _QUEUE, _WAVE1");
   if (layout == GRW_LAYOUT_WAVES && (c->batch || c->ncells != 1))
     return fail(GRW_EINVAL, "the wave-specialised layout runs single-cell contexts only");
   // The queue layout is switched off: the one run of it on the device (a
@@ -3734,6 +3799,7 @@ int grw_set_layout(grw_ctx* c, int layout) {
     return fail(GRW_EINVAL, "the queue layout is switched off (a device run of it did not finish)");
   const bool ws = layout == GRW_LAYOUT_WAVES;
   const bool queue = layout == GRW_LAYOUT_QUEUE;
+  const bool wave1 = layout == GRW_LAYOUT_WAVE1;
   const int occ = layout == GRW_LAYOUT_QUAD3 ? 3 : layout == GRW_LAYOUT_QUAD4 ? 4 : 0;
   if (queue && c->qgrid == 0) {  // as many queue blocks as the GPU holds at once
     int dev = 0, ncu = 0, per = 0;
@@ -3746,13 +3812,15 @@ int grw_set_layout(grw_ctx* c, int layout) {
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc_queue<double>, 256, 0));
     c->qgrid = std::max(1, std::min(c->grid_s, std::max(per, 1) * std::max(ncu, 1)));
   }
-  if ((ws != c->ws || occ != c->occ || queue != c->queue) && c->graph) {  // the captured launches are another kernel's
+  if ((ws != c->ws || occ != c->occ || queue != c->queue || wave1 != c->wave1) && c->graph) {
+    // the captured launches are another kernel's
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
   }
   c->ws = ws;
   c->occ = occ;
   c->queue = queue;
+  c->wave1 = wave1;
   return 0;
 }
 
@@ -4101,6 +4169,7 @@ int autotune(grw_ctx* c, const std::vector<int>& layouts, const std::function<in
   const bool ws0 = c->ws, g0 = c->graphs;
   const int occ0 = c->occ;
   const int layout0 = c->queue ? GRW_LAYOUT_QUEUE
+                      : c->wave1  ? GRW_LAYOUT_WAVE1
                       : ws0       ? GRW_LAYOUT_WAVES
                       : occ0 == 3 ? GRW_LAYOUT_QUAD3
                       : occ0 == 4 ? GRW_LAYOUT_QUAD4
@@ -4160,7 +4229,7 @@ int grw_autotune_cell(grw_ctx* c, const GrwProtocol* p, const double errw[37], d
     return grw_run_cell_device(c, p, errw, t0, nsteps, st, &hh, nullptr, nullptr, nullptr, nullptr, 0, &nlog,
                                nullptr);
   };
-  return autotune(c, {GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES, GRW_LAYOUT_QUAD3}, run, trial_steps, out);
+  return autotune(c, {GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES, GRW_LAYOUT_QUAD3, GRW_LAYOUT_WAVE1}, run, trial_steps, out);
 }
 
 int grw_autotune_batch(grw_ctx* c, int trial_steps, GrwTune* out) {
@@ -4171,7 +4240,7 @@ int grw_autotune_batch(grw_ctx* c, int trial_steps, GrwTune* out) {
     std::vector<int32_t> ns((size_t)c->ncells, nsteps);
     return grw_batch_run(c, ns.data(), nullptr, nullptr);
   };
-  return autotune(c, {GRW_LAYOUT_QUAD, GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4}, run, trial_steps, out);
+  return autotune(c, {GRW_LAYOUT_QUAD, GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4, GRW_LAYOUT_WAVE1}, run, trial_steps, out);
 }
 
 }  // extern "C"

@@ -99,7 +99,11 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
  *                     that each wavefront takes one after another, then one
  *                     block per cell for the cell's fcn and integrator stage;
  *                     switched off: grw_set_layout refuses it (a device run of
- *                     it did not finish and the cause is not known).
+ *                     it did not finish and the cause is not known);
+ *   GRW_LAYOUT_WAVE1  the four-lane kernel's work with one wavefront (16 FRUs)
+ *                     per block, so that a wavefront's registers are free for
+ *                     the next block as soon as its own FRUs are done, then one
+ *                     block per cell for the cell's fcn and integrator stage.
  * Every layout gives the same results bit for bit; the speed differs with the
  * workload (grw_autotune_cell / grw_autotune_batch time them).  No counterpart
  * in the reference. */
@@ -108,6 +112,7 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
 #define GRW_LAYOUT_QUAD3 3
 #define GRW_LAYOUT_QUAD4 4
 #define GRW_LAYOUT_QUEUE 5
+#define GRW_LAYOUT_WAVE1 6
 int grw_set_layout(grw_ctx *ctx, int layout);
 
 /* Result of an autotuning pass: the choice (now set on the context) and every
@@ -122,7 +127,7 @@ typedef struct {
 
 /* Autotuner of the device integrator for a single cell: times trial_steps
  * output steps of grw_run_cell_device from (t0, state, h) under each of
- * GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES and GRW_LAYOUT_QUAD3, with and without
+ * GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES, GRW_LAYOUT_QUAD3 and GRW_LAYOUT_WAVE1, with and without
  * graph replays, every trial from a snapshot of the context's device state
  * (a warm-up step before each); restores the snapshot, so the context's FRU
  * state, beats and counters are as they were, and sets the fastest candidate
