@@ -68,8 +68,8 @@ class GrwRkStats(C.Structure):
     _fields_ = [("attempts", C.c_int64), ("rejects", C.c_int64), ("forced", C.c_int64)]
 
 
-TUNE_MAX = 8  # GRW_TUNE_MAX
-LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4", 5: "queue", 6: "wave1"}
+TUNE_MAX = 12  # GRW_TUNE_MAX
+LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4", 5: "queue", 6: "wave1", 7: "wave1_3"}
 
 
 class GrwTune(C.Structure):
@@ -253,8 +253,9 @@ class Cell:
         wavefronts per SIMD; "queue", the four-lane kernel's work as a queue
         of 16-FRU items per wavefront, then a per-cell stage kernel (switched
         off); "wave1", the four-lane kernel's work with one wavefront per
-        block, then a per-cell stage kernel.  All give the same results bit
-        for bit."""
+        block, then a per-cell stage kernel; "wave1_3", the same with
+        registers capped for three wavefronts per SIMD.  All give the same
+        results bit for bit."""
         codes = {v: k for k, v in LAYOUT_NAMES.items()}
         if layout not in codes:
             raise ValueError("layout must be one of %s" % sorted(codes))

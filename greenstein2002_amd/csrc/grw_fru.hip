@@ -2611,7 +2611,7 @@ __global__ void __launch_bounds__(256) k_mc_queue(const SimArgs base, const Dev 
 // four-lane kernel's order, evaluates the stage's fcn and runs the integrator
 // stage: the same numbers as GRW_LAYOUT_QUAD.  No block waits for another.
 template <class R>
-__global__ void __launch_bounds__(64) k_mc_wave(const SimArgs base, const Dev d, RkCtl* ctls) {
+__device__ __forceinline__ void mc_wave(const SimArgs& base, const Dev& d, const RkCtl* ctls) {
   const int i = (int)blockIdx.x;
   const int blk = base.sorted ? d.border[i >> 2] : (i >> 2);
   const int w = i & 3;
@@ -2650,6 +2650,18 @@ __global__ void __launch_bounds__(64) k_mc_wave(const SimArgs base, const Dev d,
       if (lane == 0) slot[k] = v;
     }
   }
+}
+
+template <class R>
+__global__ void __launch_bounds__(64) k_mc_wave(const SimArgs base, const Dev d, RkCtl* ctls) {
+  mc_wave<R>(base, d, ctls);
+}
+
+// The same with its registers capped so that W wavefronts fit on a SIMD.
+template <class R, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) k_mc_wave_occ(const SimArgs base,
+                                                                                       const Dev d, RkCtl* ctls) {
+  mc_wave<R>(base, d, ctls);
 }
 
 // The four-lane stage with its registers capped so that W wavefronts fit on a
@@ -3222,11 +3234,19 @@ void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent
     if (c->wave1) {
       const dim3 wgrid(4 * c->grid_s);
       if (mixed) {
-        hipExtLaunchKernelGGL(k_mc_wave<float>, wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d, c->dctl);
+        if (c->occ == 3)
+          hipExtLaunchKernelGGL((k_mc_wave_occ<float, 3>), wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d,
+                                c->dctl);
+        else
+          hipExtLaunchKernelGGL(k_mc_wave<float>, wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d, c->dctl);
         hipExtLaunchKernelGGL(k_dev_tail<float>, dim3(c->ncells), dim3(256), 0, c->stream, nullptr, e1, 0, base, c->d,
                               c->dctl);
       } else {
-        hipExtLaunchKernelGGL(k_mc_wave<double>, wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d, c->dctl);
+        if (c->occ == 3)
+          hipExtLaunchKernelGGL((k_mc_wave_occ<double, 3>), wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d,
+                                c->dctl);
+        else
+          hipExtLaunchKernelGGL(k_mc_wave<double>, wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d, c->dctl);
         hipExtLaunchKernelGGL(k_dev_tail<double>, dim3(c->ncells), dim3(256), 0, c->stream, nullptr, e1, 0, base,
                               c->d, c->dctl);
       }
@@ -3789,8 +3809,8 @@ int grw_set_timing(grw_ctx* c, int on) {
 
 int grw_set_layout(grw_ctx* c, int layout) {
   if (!c) return fail(GRW_EINVAL, "null context");
-  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_WAVE1)
-    return fail(GRW_EINVAL, "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4, _QUEUE, _WAVE1");
+  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_WAVE1_3)
+    return fail(GRW_EINVAL, "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4, _QUEUE, _WAVE1, _WAVE1_3");
   if (layout == GRW_LAYOUT_WAVES && (c->batch || c->ncells != 1))
     return fail(GRW_EINVAL, "the wave-specialised layout runs single-cell contexts only");
   // The queue layout is switched off: the one run of it on the device (a
@@ -3799,8 +3819,8 @@ int grw_set_layout(grw_ctx* c, int layout) {
     return fail(GRW_EINVAL, "the queue layout is switched off (a device run of it did not finish)");
   const bool ws = layout == GRW_LAYOUT_WAVES;
   const bool queue = layout == GRW_LAYOUT_QUEUE;
-  const bool wave1 = layout == GRW_LAYOUT_WAVE1;
-  const int occ = layout == GRW_LAYOUT_QUAD3 ? 3 : layout == GRW_LAYOUT_QUAD4 ? 4 : 0;
+  const bool wave1 = layout == GRW_LAYOUT_WAVE1 || layout == GRW_LAYOUT_WAVE1_3;
+  const int occ = (layout == GRW_LAYOUT_QUAD3 || layout == GRW_LAYOUT_WAVE1_3) ? 3 : layout == GRW_LAYOUT_QUAD4 ? 4 : 0;
   if (queue && c->qgrid == 0) {  // as many queue blocks as the GPU holds at once
     int dev = 0, ncu = 0, per = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -4169,7 +4189,7 @@ int autotune(grw_ctx* c, const std::vector<int>& layouts, const std::function<in
   const bool ws0 = c->ws, g0 = c->graphs;
   const int occ0 = c->occ;
   const int layout0 = c->queue ? GRW_LAYOUT_QUEUE
-                      : c->wave1  ? GRW_LAYOUT_WAVE1
+                      : c->wave1  ? (occ0 == 3 ? GRW_LAYOUT_WAVE1_3 : GRW_LAYOUT_WAVE1)
                       : ws0       ? GRW_LAYOUT_WAVES
                       : occ0 == 3 ? GRW_LAYOUT_QUAD3
                       : occ0 == 4 ? GRW_LAYOUT_QUAD4
@@ -4240,7 +4260,8 @@ int grw_autotune_batch(grw_ctx* c, int trial_steps, GrwTune* out) {
     std::vector<int32_t> ns((size_t)c->ncells, nsteps);
     return grw_batch_run(c, ns.data(), nullptr, nullptr);
   };
-  return autotune(c, {GRW_LAYOUT_QUAD, GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4, GRW_LAYOUT_WAVE1}, run, trial_steps, out);
+  return autotune(c, {GRW_LAYOUT_QUAD, GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4, GRW_LAYOUT_WAVE1, GRW_LAYOUT_WAVE1_3}, run,
+                  trial_steps, out);
 }
 
 }  // extern "C"

@@ -103,7 +103,9 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
  *   GRW_LAYOUT_WAVE1  the four-lane kernel's work with one wavefront (16 FRUs)
  *                     per block, so that a wavefront's registers are free for
  *                     the next block as soon as its own FRUs are done, then one
- *                     block per cell for the cell's fcn and integrator stage.
+ *                     block per cell for the cell's fcn and integrator stage;
+ *   GRW_LAYOUT_WAVE1_3  the same with its registers capped for three
+ *                     wavefronts per SIMD.
  * Every layout gives the same results bit for bit; the speed differs with the
  * workload (grw_autotune_cell / grw_autotune_batch time them).  No counterpart
  * in the reference. */
@@ -113,11 +115,12 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
 #define GRW_LAYOUT_QUAD4 4
 #define GRW_LAYOUT_QUEUE 5
 #define GRW_LAYOUT_WAVE1 6
+#define GRW_LAYOUT_WAVE1_3 7
 int grw_set_layout(grw_ctx *ctx, int layout);
 
 /* Result of an autotuning pass: the choice (now set on the context) and every
  * candidate timed, with the wall time of its trial run. */
-#define GRW_TUNE_MAX 8
+#define GRW_TUNE_MAX 12
 typedef struct {
   int32_t layout, graphs; /* the fastest candidate (grw_set_layout, grw_set_graphs) */
   int32_t ncand, reserved;
