@@ -254,8 +254,8 @@ __device__ __forceinline__ double swap_sum(double v, bool rows16) {
 template <int CTRL, bool MAX = false>
 __device__ __forceinline__ double dpp_add(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)b, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
   const double w = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
   return MAX ? rmax(v, w) : v + w;
 }
@@ -268,13 +268,13 @@ __device__ __forceinline__ unsigned wave_u32(unsigned v) {
   v = MAX ? max((unsigned)p32[0], (unsigned)p32[1]) : (unsigned)p32[0] + (unsigned)p32[1];
   const auto p16 = __builtin_amdgcn_permlane16_swap(v, v, false, false);
   v = MAX ? max((unsigned)p16[0], (unsigned)p16[1]) : (unsigned)p16[0] + (unsigned)p16[1];
-  unsigned w = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x128, 0xF, 0xF, false);
+  unsigned w = (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);
   v = MAX ? max(v, w) : v + w;
-  w = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x124, 0xF, 0xF, false);
+  w = (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xF, 0xF, false);
   v = MAX ? max(v, w) : v + w;
-  w = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false);
+  w = (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
   v = MAX ? max(v, w) : v + w;
-  w = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false);
+  w = (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
   return MAX ? max(v, w) : v + w;
 }
 
@@ -618,7 +618,7 @@ constexpr int QNEXT = 0x39, QPREV = 0x93, QX1 = 0xB1, QX2 = 0x4E;
 
 template <int CTRL>
 __device__ __forceinline__ int qperm(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
 }
 template <int CTRL>
 __device__ __forceinline__ float qperm(float v) {
