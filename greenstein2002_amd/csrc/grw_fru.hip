@@ -616,6 +616,11 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
 constexpr int QB0 = 0x00, QB1 = 0x55, QB2 = 0xAA, QB3 = 0xFF;
 constexpr int QNEXT = 0x39, QPREV = 0x93, QX1 = 0xB1, QX2 = 0x4E;
 
+// mov_dpp leaves a lane whose source lane is invalid undefined; every lane of
+// a quad reads a lane of its own quad and quads are active as a whole, so all
+// lanes are written.  (update_dpp with the value itself as the "old" operand
+// makes the compiler copy the value before every permute; the row rotations
+// of the reductions below read valid lanes of a full wavefront likewise.)
 template <int CTRL>
 __device__ __forceinline__ int qperm(int v) {
   return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
