@@ -64,10 +64,12 @@ struct MtLane {
   int idx;
   int nreg;
 
+  // (y ^ ((y << s) & B) as one three-input bit operation, truth table 0x78 =
+  // src0 ^ (src1 & src2), instead of an and with a literal and an xor)
   __device__ __forceinline__ static uint32_t temper(uint32_t y) {
     y ^= y >> 11;
-    y ^= (y << 7) & 0x9d2c5680u;
-    y ^= (y << 15) & 0xefc60000u;
+    y = __builtin_amdgcn_bitop3_b32(y, y << 7, 0x9d2c5680u, 0x78);
+    y = __builtin_amdgcn_bitop3_b32(y, y << 15, 0xefc60000u, 0x78);
     y ^= y >> 18;
     return y;
   }
@@ -801,19 +803,30 @@ __device__ __forceinline__ void mt_fetch8(const uint32_t* row, int g0, MtWin8& w
 }
 
 // Regeneration and stores for stream positions g0 .. g0+7 from a fetched window.
-// Branch-free: every word is twisted and a mask keeps the loaded word for a
-// position of the loaded generation (g < 624), which is stored back unchanged
-// (cur[k] is the word at row position g).  Row positions q + k past 623 are the
+// Once a row is past its loaded generation (g0 >= 624, every window but the
+// first ~20 after a load) all eight words are regenerated; otherwise a mask
+// keeps the loaded word for a position of the loaded generation (g < 624),
+// which is stored back unchanged (cur[k] is the word at row position g).  The
+// two cases are a branch because a wavefront's lanes almost never disagree on
+// it.  Row positions q + k past 623 are the
 // copies of words 0..7, so the stores go to q + k and, for a window at either
 // end of the row (rare), also to the other copy.
 __device__ __forceinline__ void mt_finish8(uint32_t* row, int g0, const MtWin8& w, uint32_t out[8]) {
   const int q = mt_row_pos(g0);
   uint32_t* const p = row + q;
   uint32_t wv[8];
+  if (g0 >= kMtN) {  // every word regenerated: all windows after a row's first generation
+#pragma unroll
+    for (int k = 0; k < 8; ++k) wv[k] = mt_twist(w.cur[k], w.cur[k + 1], w.far[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t m = 0u - (uint32_t)(g0 + k >= kMtN);
+      wv[k] = (mt_twist(w.cur[k], w.cur[k + 1], w.far[k]) & m) | (w.cur[k] & ~m);
+    }
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const uint32_t m = 0u - (uint32_t)(g0 + k >= kMtN);
-    wv[k] = (mt_twist(w.cur[k], w.cur[k + 1], w.far[k]) & m) | (w.cur[k] & ~m);
     p[k] = wv[k];
     out[k] = MtLane::temper(wv[k]);
   }
@@ -1094,17 +1107,17 @@ __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, 
       for (int r = 0; r < kNryr; ++r) {
         w = ch_set_ryr(w, r, st[r]);
         tr[r] = rtot[st[r] - 1][tl];  // total of the resolved state (fru_rates_local.c:390-727)
-        maxr = vmax(maxr, tr[r]);
+        maxr = vmax_nc(maxr, tr[r]);
       }
-      maxr = vmax(maxr, qperm<QX1>(maxr));
-      maxr = vmax(maxr, qperm<QX2>(maxr));
+      maxr = vmax_nc(maxr, qperm<QX1>(maxr));
+      maxr = vmax_nc(maxr, qperm<QX2>(maxr));
       bad |= qperm<QX1>(bad);
       bad |= qperm<QX2>(bad);
       // ---- step length (simfru_local.c:135-146)
       double dt = 0.1 / (double)maxr;
       dt = vmin(dt, 10.001e-3);
-      double cm = vmax(css, qperm<QX1>(css));
-      cm = vmax(cm, qperm<QX2>(cm));
+      double cm = vmax_nc(css, qperm<QX1>(css));
+      cm = vmax_nc(cm, qperm<QX2>(cm));
       const double cmax = vmax(0.e-3, cm);
       if (cmax > 1.5e-3) dt = vmin(dt, 5.001e-3);
       dt = vmin(dt, t1 - t);

@@ -83,6 +83,20 @@ __device__ __forceinline__ double vmax(double a, double b) { return __builtin_fm
 __device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
 __device__ __forceinline__ float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
+// The same instruction for operands the compiler cannot prove canonical (LDS
+// loads, lane permutes), before which it would insert a canonicalising
+// v_max x, x, x each: the two differ only for signalling NaNs, which no
+// arithmetic produces.
+__device__ __forceinline__ double vmax_nc(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float vmax_nc(float a, float b) {
+  float r;
+  asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
 #endif
 
 // Division and exp in the rate and local-Ca arithmetic: IEEE in FP64 (the
