@@ -93,6 +93,9 @@ def parse():
                     help="skip the rocprofv3 kernel-trace run that measures roofline.avg_launch_us")
     ap.add_argument("--batch-cells", type=int, default=256, help="cells of the batched run (0 skips it)")
     ap.add_argument("--batch-ms", type=int, default=30, help="simulated ms of the batched run")
+    ap.add_argument("--batch-tune-steps", type=int, default=12,
+                    help="simulated ms of each batch autotuner trial, from t = 0 (12: through the first "
+                         "stimulus at 10 ms, so that the trials see the costly phase the timed window holds)")
     ap.add_argument("--batch-seq-sample", type=int, default=8,
                     help="cells of the batch also run one at a time for the sequential rate")
     ap.add_argument("--ensemble", type=int, default=0,
@@ -192,7 +195,8 @@ def _run_profiled(prof_args, cmd, folder, timeout=300):
 
 
 def kernel_trace(args) -> dict | None:
-    """Mean duration of k_dev_stage over the timed window, from a kernel trace:
+    """Mean duration of k_dev_stage (or of k_mc_wave + k_dev_tail, the
+    single-wavefront layout's two launches per stage) over the timed window, from a kernel trace:
     this script's same window (--warmup .. --warmup + --steps) run again as a
     child under rocprofv3 --kernel-trace, without HIP events.  The window's
     dispatches are the last `launches` k_dev_stage dispatches that did work
@@ -210,12 +214,25 @@ def kernel_trace(args) -> dict | None:
             return None
         if not child:
             return None
-        disp = []
+        disp, mc, tail = [], [], []
         for path in glob.glob(os.path.join(tmp, "**", "*kernel_trace.csv"), recursive=True):
             with open(path) as f:
                 for r in csv.DictReader(f):
+                    ab = (int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
                     if "k_dev_stage" in r["Kernel_Name"]:
-                        disp.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+                        disp.append(ab)
+                    elif "k_mc_wave" in r["Kernel_Name"]:
+                        mc.append(ab)
+                    elif "k_dev_tail" in r["Kernel_Name"]:
+                        tail.append(ab)
+    if mc and not disp:
+        # single-wavefront layout: a stage is a Monte Carlo launch and its tail
+        # launch; its duration is the sum of the two (the gap between them left out)
+        mc.sort()
+        tail.sort()
+        if len(mc) != len(tail):
+            return None
+        disp = [(a, a + (b - a) + (d - c)) for (a, b), (c, d) in zip(mc, tail)]
     disp.sort()
     real = [(a, b) for a, b in disp if b - a > 3000]
     n = int(child["config"]["launches"])
@@ -562,7 +579,7 @@ def batched(args, precision: str | None = None, sequential: bool = True):
     b = Batch(specs, precision=prec)
     b.set_graphs(bool(args.graphs))
     if args.layout == "auto":
-        TUNED[key] = b.autotune(2)
+        TUNED[key] = b.autotune(args.batch_tune_steps)
         print(f"[bench] batch ({prec}): autotuner chose {TUNED[key]['layout']}, graphs {TUNED[key]['graphs']}",
               file=sys.stderr, flush=True)
     elif args.layout != "waves":
@@ -879,11 +896,13 @@ def main():
         achieved = alg_bytes / st["launches"] / avg_s / 1e9
         roof = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 6), traffic=None,
-                    kernel=(("k_dev_stage" if args.driver == "device" else
-                             "k_simfru4" if args.lanes == 4 else "k_simfru") +
+                    kernel=((stage_kernels(TUNED["fp64"]["layout"] if "fp64" in TUNED else args.layout,
+                                           args.precision) +
+                             " (FRU sub-interval + flux reduction + integrator stage)")
+                            if args.driver == "device" else
+                            ("k_simfru4" if args.lanes == 4 else "k_simfru") +
                             ("<double>" if args.precision == "fp64" else "<float>") +
-                            " (FRU sub-interval + fused flux reduction" +
-                            (" + integrator stage)" if args.driver == "device" else ")")),
+                            " (FRU sub-interval + fused flux reduction)"),
                     avg_launch_us=round(avg_s * 1e6, 3),
                     avg_launch_us_source=("rocprofv3 --kernel-trace of this window (child run, no events)"
                                           if kt else "HIP events, one RK attempt in eight"),
