@@ -81,7 +81,7 @@ def parse():
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
-    ap.add_argument("--layout", choices=["auto", "quad", "waves", "quad3", "quad4", "wave1", "wave1_3"], default="auto",
+    ap.add_argument("--layout", choices=["auto", "quad", "waves", "quad3", "quad4", "wave1", "wave1_3", "quadp"], default="auto",
                     help="Monte Carlo kernel of the device integrator (grw_set_layout); auto: the autotuner "
                          "picks it and the graph replays from timed trials (grw_autotune_cell / _batch)")
     ap.add_argument("--tune-steps", type=int, default=10, help="simulated ms of each single-cell autotuner trial")
@@ -558,6 +558,8 @@ def stage_kernels(layout: str, prec: str) -> str:
     t = "double" if prec == "fp64" else "float"
     if layout in ("wave1", "wave1_3"):
         return "k_mc_wave%s<%s> + k_dev_tail<%s> (one RK stage)" % ("_occ" if layout == "wave1_3" else "", t, t)
+    if layout == "quadp":
+        return "k_dev_stage_poll<%s>" % t
     return "k_dev_stage<%s>" % t
 
 
@@ -582,7 +584,7 @@ def batched(args, precision: str | None = None, sequential: bool = True):
         TUNED[key] = b.autotune(args.batch_tune_steps)
         print(f"[bench] batch ({prec}): autotuner chose {TUNED[key]['layout']}, graphs {TUNED[key]['graphs']}",
               file=sys.stderr, flush=True)
-    elif args.layout != "waves":
+    elif args.layout not in ("waves", "quadp"):
         b.set_layout(args.layout)
     b.set_timing(True)
     b.reset_stats()
@@ -630,7 +632,7 @@ def batched(args, precision: str | None = None, sequential: bool = True):
                            "divided by its cycle length, over the wall time; the window holds the stimulus, "
                            "upstroke and start of release, which cost more per simulated ms than the rest of "
                            "a beat, so whole beats run faster than this"),
-        layout=TUNED[key]["layout"] if key in TUNED else ("quad" if args.layout == "waves" else args.layout),
+        layout=TUNED[key]["layout"] if key in TUNED else ("quad" if args.layout in ("waves", "quadp") else args.layout),
         hip_graphs=bool(TUNED[key]["graphs"]) if key in TUNED else bool(args.graphs),
         autotune=TUNED.get(key),
         rk_attempts=att, fru_steps=st["fru_steps"],

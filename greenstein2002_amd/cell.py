@@ -69,7 +69,7 @@ class GrwRkStats(C.Structure):
 
 
 TUNE_MAX = 12  # GRW_TUNE_MAX
-LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4", 5: "queue", 6: "wave1", 7: "wave1_3"}
+LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4", 5: "queue", 6: "wave1", 7: "wave1_3", 8: "quadp"}
 
 
 class GrwTune(C.Structure):
@@ -254,8 +254,9 @@ class Cell:
         of 16-FRU items per wavefront, then a per-cell stage kernel (switched
         off); "wave1", the four-lane kernel's work with one wavefront per
         block, then a per-cell stage kernel; "wave1_3", the same with
-        registers capped for three wavefronts per SIMD.  All give the same
-        results bit for bit."""
+        registers capped for three wavefronts per SIMD; "quadp", the
+        four-lane kernel whose fcn block collects the partials by polling
+        (single cells).  All give the same results bit for bit."""
         codes = {v: k for k, v in LAYOUT_NAMES.items()}
         if layout not in codes:
             raise ValueError("layout must be one of %s" % sorted(codes))

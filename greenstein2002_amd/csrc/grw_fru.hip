@@ -184,6 +184,7 @@ struct Dev {
   int32_t* mtidx_h;
   uint32_t* mtw_h;
   double* part;              // [grid][kSlot] block partial sums
+  double* part2;             // [grid][kSlot] block partials collected by polling (kSlotEmpty when not yet written)
   unsigned int* ticket;      // last-block election counters, one per cell (self-resetting)
   unsigned int* qhead;       // next work item of a queue-layout Monte Carlo launch
   double* out;               // [32]: 0-3 fluxes, 4-8 flux sums, 16-26 avg sums
@@ -1774,6 +1775,136 @@ __device__ __forceinline__ void close_launch_stats(const Dev& d) {
   atomicAdd(d.critsteps, (unsigned long long)m);
 }
 
+// ---- Partials collected by polling (single cells).  The ticket election
+// (reduce_range) costs the launch's critical path three device-scope round
+// trips after the last wavefront's loop: the wait for the partial stores to
+// complete, the ticket's atomic, and the last block's loads of the partials
+// (and a fourth for the stage terms the fcn block left in the control block).
+// Here the Monte Carlo blocks store their partials without waiting, into slots
+// that hold a signalling-NaN marker until written (no arithmetic produces
+// one), and the cell's fcn block -- which runs the tail anyway -- loads the
+// slots until no marker is left, re-arms them for the next launch and keeps
+// its stage terms in LDS.  The blocks never wait for each other but the fcn
+// block does, so the grid must hold it beside at least one Monte Carlo block
+// (it does: one block per CU at most is the fcn block's, of a single cell);
+// its wait is bounded and ends the run with an error if a slot never fills.
+constexpr unsigned long long kSlotEmpty = 0x7ff4deadbeef0001ull;  // a signalling NaN
+constexpr int kSlotSteps = kSlot - 2, kSlotMaxSteps = kSlot - 1;  // the block's FRU-step sum and maximum
+constexpr int kPollSpin = 1 << 20;  // polls of one slot (each a device-scope round trip) before giving up
+
+__global__ void k_arm_slots(double* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = __longlong_as_double((long long)kSlotEmpty);
+}
+
+// A Monte Carlo block's partials: the wavefront butterflies and the waves in
+// order, as reduce_range forms them, and its FRU-step sum and maximum.
+template <int NV>
+__device__ __forceinline__ void publish_partials(double v[NV], unsigned steps, const Dev& d, int blk) {
+  static_assert(NV <= kSlotSteps, "block partials overflow their slot row");
+  __shared__ double wsum[4][NV];
+  __shared__ unsigned wst[4][2];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum_xor(v[k]);
+  const unsigned ss = wave_u32<false>(steps), sm = wave_u32<true>(steps);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) wsum[wave][k] = v[k];
+    wst[wave][0] = ss;
+    wst[wave][1] = sm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double* const slot = d.part2 + (size_t)blk * kSlot;
+    for (int k = 0; k < NV; ++k) {
+      double s = wsum[0][k];
+      for (int w = 1; w < nw; ++w) s += wsum[w][k];
+      __hip_atomic_store(slot + k, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    unsigned tsum = 0u, tmax = 0u;
+    for (int w = 0; w < nw; ++w) {
+      tsum += wst[w][0];
+      tmax = max(tmax, wst[w][1]);
+    }
+    __hip_atomic_store(slot + kSlotSteps, (double)tsum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(slot + kSlotMaxSteps, (double)tmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__device__ __forceinline__ double await_slot(double* p, bool& ok) {
+  for (int n = 0; ok && n < kPollSpin; ++n) {
+    const double v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((unsigned long long)__double_as_longlong(v) != kSlotEmpty) return v;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  ok = false;
+  return 0.0;
+}
+
+// The cell's fcn block: the partials of blocks b0 .. b0+nb-1 in reduce_range's
+// order (each thread its strided subset, the wavefront butterfly, the waves in
+// order: the same numbers), the slots re-armed behind it, and the launch's
+// step statistics.  Returns false (slots not all written in time) only in
+// thread 0.
+template <int NV>
+__device__ __forceinline__ bool collect_partials(const Dev& d, int b0, int nb, double* tot_sh) {
+  __shared__ double wsum[4][NV];
+  __shared__ unsigned long long wst[4][2];
+  __shared__ int bad_sh;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if (threadIdx.x == 0) bad_sh = 0;
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  unsigned long long ssum = 0ull, smax = 0ull;
+  bool ok = true;
+  const double empty = __longlong_as_double((long long)kSlotEmpty);
+  for (int b = b0 + (int)threadIdx.x; b < b0 + nb; b += blockDim.x) {
+    double* const slot = d.part2 + (size_t)b * kSlot;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] += await_slot(slot + k, ok);
+    ssum += (unsigned long long)await_slot(slot + kSlotSteps, ok);
+    smax = max(smax, (unsigned long long)await_slot(slot + kSlotMaxSteps, ok));
+#pragma unroll
+    for (int k = 0; k < NV; ++k) __hip_atomic_store(slot + k, empty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(slot + kSlotSteps, empty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(slot + kSlotMaxSteps, empty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();  // bad_sh cleared
+  if (!ok) bad_sh = 1;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = wave_sum_xor(s[k]);
+  // step statistics: 64-bit sums and maxima over the wavefront by lane shuffles
+  for (int o = 32; o > 0; o >>= 1) {
+    ssum += __shfl_xor(ssum, o);
+    smax = max(smax, __shfl_xor(smax, o));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) wsum[wave][k] = s[k];
+    wst[wave][0] = ssum;
+    wst[wave][1] = smax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NV; ++k) {
+      double a = wsum[0][k];
+      for (int w = 1; w < nw; ++w) a += wsum[w][k];
+      tot_sh[k] = a;
+    }
+    unsigned long long ts = 0ull, tm = 0ull;
+    for (int w = 0; w < nw; ++w) {
+      ts += wst[w][0];
+      tm = max(tm, wst[w][1]);
+    }
+    atomicAdd(d.nsteps, ts);
+    atomicAdd(d.critsteps, tm);
+  }
+  __syncthreads();
+  return bad_sh == 0;
+}
+
 template <class R, bool REDUCE>
 __global__ void __launch_bounds__(256) k_simfru4(const SimArgs a, const Dev d) {
   double acc[17] = {};
@@ -2232,7 +2363,7 @@ __global__ void __launch_bounds__(256) k_dev_begin(const SimArgs base, const Dev
 // flux reduction, and in the cell's last block fcn for this stage and then
 // the next stage or the step decision (rk_pd54m.c:124-333) and the caf.c
 // output loop (caf.c:150-251).
-template <class R, bool WS, bool TAIL = false>
+template <class R, bool WS, bool TAIL = false, bool POLL = false>
 __device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkCtl* ctls) {
   // blocks [0, ncells): each cell's fcn block, dispatched first; then the
   // Monte Carlo blocks
@@ -2261,6 +2392,7 @@ __device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkC
   const int j = threadIdx.x;
   __shared__ double st[kNglobal], yv[kNglobal], F[kNglobal], kk[6][kNglobal];
   __shared__ FcnTrans fq;
+  __shared__ double Jsh[4], pre[kPre];
   if (fblock) {
     // This stage's fcn at (t1, y) but for the assembly, the one part that
     // needs the fluxes of this launch's sub-interval (the stage's y is known
@@ -2276,25 +2408,42 @@ __device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkC
       FcnCa ca;
       double V, Istim;
       fcn_block_groups(a.t1, yv, F, ctl->prot, &fq, c, ca, V, Istim, &mset_sh);
-      double* pre = ctl->pre;
-      if (j < kNglobal) __hip_atomic_store(pre + j, F[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (j == 0) {
-        const FcnAsm m = fcn_asm_pre(c, ca, Istim, ctl->prot.chf != 0);
-        const double mv[kFcnAsm] = {m.dNai, m.dKi, m.a3ca, m.mca, m.nsr, m.beta_i, m.Jup,
-                                    m.Jtrpn, m.INa, m.IKr, m.IKs, m.p2, m.a3v};
-        for (int q = 0; q < kFcnAsm; ++q)
-          __hip_atomic_store(pre + kPreAsm + q, mv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(pre + kPreV, V, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(pre + kPreIstim, Istim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (POLL) {  // this block runs the tail: the terms stay in LDS (F in place)
+        if (j == 0) {
+          const FcnAsm m = fcn_asm_pre(c, ca, Istim, ctl->prot.chf != 0);
+          const double mv[kFcnAsm] = {m.dNai, m.dKi, m.a3ca, m.mca, m.nsr, m.beta_i, m.Jup,
+                                      m.Jtrpn, m.INa, m.IKr, m.IKs, m.p2, m.a3v};
+          for (int q = 0; q < kFcnAsm; ++q) pre[kPreAsm + q] = mv[q];
+          pre[kPreV] = V;
+          pre[kPreIstim] = Istim;
+        }
+        if (j == 64) pre[kPreE1] = flux_exp(a.depf[0]);
+      } else {
+        double* pre_g = ctl->pre;
+        if (j < kNglobal) __hip_atomic_store(pre_g + j, F[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (j == 0) {
+          const FcnAsm m = fcn_asm_pre(c, ca, Istim, ctl->prot.chf != 0);
+          const double mv[kFcnAsm] = {m.dNai, m.dKi, m.a3ca, m.mca, m.nsr, m.beta_i, m.Jup,
+                                      m.Jtrpn, m.INa, m.IKr, m.IKs, m.p2, m.a3v};
+          for (int q = 0; q < kFcnAsm; ++q)
+            __hip_atomic_store(pre_g + kPreAsm + q, mv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(pre_g + kPreV, V, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(pre_g + kPreIstim, Istim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (j == 64)
+          __hip_atomic_store(pre_g + kPreE1, flux_exp(a.depf[0]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // complete the stores before this block arrives at the cell's ticket;
+        // the compiler-only fences keep them on this side of the wait
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
       }
-      if (j == 64)
-        __hip_atomic_store(pre + kPreE1, flux_exp(a.depf[0]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      // complete the stores before this block arrives at the cell's ticket;
-      // the compiler-only fences keep them on this side of the wait
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      __builtin_amdgcn_s_waitcnt(0);
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
     }
+  } else if (POLL) {  // publish this block's partials and leave: the fcn block collects them
+    const unsigned steps = simfru4_body<R>(a, d, acc, blk);
+    if (a.avg) publish_partials<17>(acc, steps, d, blk);
+    else publish_partials<6>(acc, steps, d, blk);
+    return;
   } else {
     count_steps(d, WS ? simfru_ws<R>(a, d, acc, blk) : simfru4_body<R>(a, d, acc, blk));
   }
@@ -2318,6 +2467,15 @@ __device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkC
       sum_wave_slots<17>(d, ci.sb0, ci.snb, tot);
     else
       sum_wave_slots<6>(d, ci.sb0, ci.snb, tot);
+  } else if (POLL) {  // the fcn block: the Monte Carlo blocks' partials, as they arrive
+    const bool okp = a.avg ? collect_partials<17>(d, ci.sb0, ci.snb, tot) : collect_partials<6>(d, ci.sb0, ci.snb, tot);
+    if (!okp) {
+      if (j == 0) {
+        ctl->err = 1;
+        ctl->active = 0;
+      }
+      return;
+    }
   } else if (WS ? (a.avg ? !reduce_ws<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, tot, 1)
                          : !reduce_ws<6>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, tot, 1))
                 : (a.avg ? !reduce_range<17>(acc, d, d.ticket + cell, blk, ci.sb0, ci.snb, 0, tot, 1)
@@ -2326,7 +2484,6 @@ __device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkC
   }
   const unsigned long long tail0 = wall_clock64();
   // ---- last block of the cell: fcn of this stage and the next stage / the step decision
-  __shared__ double Jsh[4], pre[kPre];
   __shared__ int decision;
   const int s = s0;
   if (s < 1 || s > 5) {  // launches out of sequence: stop the run
@@ -2351,14 +2508,15 @@ __device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkC
   // the fcn block's part of this stage's fcn (agent-scope loads: written in
   // this launch), the flux conversion (mpi_master.c:401-468) on wavefront 3
   // and the launch's step statistics (a returning atomic) on wavefront 2
-  if (j < kPreE1) {
+  if (!POLL && j < kPreE1) {
     const double v = __hip_atomic_load(ctl->pre + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (j < kNglobal) F[j] = v;
     else pre[j] = v;
   }
-  if (j == 128 && base.ncells == 1) close_launch_stats(d);
+  if (!POLL && j == 128 && base.ncells == 1) close_launch_stats(d);
   if (j == 192) {
-    const double e1 = __hip_atomic_load(ctl->pre + kPreE1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double e1 =
+        POLL ? pre[kPreE1] : __hip_atomic_load(ctl->pre + kPreE1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     flux_currents_e(tot, a.depf[0], e1, a.depf[1], a.depf[2], ci.nfru, ci.scale, Jsh);
   }
   __syncthreads();
@@ -2549,6 +2707,15 @@ __device__ __forceinline__ void dev_stage(const SimArgs& base, const Dev& d, RkC
 template <class R, bool WS>
 __global__ void __launch_bounds__(256) k_dev_stage(const SimArgs base, const Dev d, RkCtl* ctls) {
   dev_stage<R, WS>(base, d, ctls);
+}
+
+// The four-lane stage of a single cell whose fcn block collects the Monte
+// Carlo blocks' partials by polling (publish_partials / collect_partials) and
+// runs the tail: the same numbers, two to three fewer device-scope round trips
+// on the launch's critical path.
+template <class R>
+__global__ void __launch_bounds__(256) k_dev_stage_poll(const SimArgs base, const Dev d, RkCtl* ctls) {
+  dev_stage<R, false, false, true>(base, d, ctls);
 }
 
 // Queue layout (switched off: grw_set_layout refuses GRW_LAYOUT_QUEUE, since
@@ -3020,6 +3187,7 @@ struct grw_ctx {
   int occ = 0;                     // 0: the stage kernel as compiled; 3, 4: registers capped for 3 / 4 waves per SIMD
   bool queue = false;              // device integrator: Monte Carlo work queue + per-cell tail kernel
   bool wave1 = false;              // device integrator: single-wavefront Monte Carlo blocks + per-cell tail kernel
+  bool poll = false;               // device integrator (single cells): the fcn block collects the partials by polling
   int qgrid = 0;                   // blocks of the queue kernel: as many as the GPU holds at once
   hipGraphExec_t graph = nullptr;  // kGraphAttempts RK attempts, captured on first use
   SimArgs graph_base{};            // the launch arguments the graph holds
@@ -3120,6 +3288,7 @@ int alloc_fru_state(grw_ctx* c) {
       (rc = dalloc(c, &c->d.mt, ld * kMtStride)) || (rc = dalloc(c, &c->d.mtw_h, ld)) ||
       (rc = dalloc(c, &c->d.conc_h, 5 * ld)) || (rc = dalloc(c, &c->d.chan_h, 4 * ld)) ||
       (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, nslots * kSlot)) ||
+      (rc = dalloc(c, &c->d.part2, (size_t)c->grid_s * kSlot)) ||
       (rc = dalloc(c, &c->d.ticket, (size_t)c->ncells)) || (rc = dalloc(c, &c->d.qhead, 1)) || (rc = dalloc(c, &c->d.out, 32)) ||
       (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
       (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
@@ -3249,7 +3418,13 @@ void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent
   const dim3 grid(c->ncells + mc_grid(c));
   for (int s5 = 0; s5 < 5; ++s5) {
     hipEvent_t e0 = ek ? ek[2 * s5] : nullptr, e1 = ek ? ek[2 * s5 + 1] : nullptr;
-    if (c->wave1) {
+    if (c->poll) {
+      if (mixed)
+        hipExtLaunchKernelGGL(k_dev_stage_poll<float>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d, c->dctl);
+      else
+        hipExtLaunchKernelGGL(k_dev_stage_poll<double>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d,
+                              c->dctl);
+    } else if (c->wave1) {
       const dim3 wgrid(4 * c->grid_s);
       if (mixed) {
         if (c->occ == 3)
@@ -3347,6 +3522,11 @@ int ensure_graph(grw_ctx* c, const SimArgs& base, bool mixed) {
 int run_device(grw_ctx* c) {
   const int nc = c->ncells;
   HIPCHK(hipMemcpyAsync(c->dctl, c->hctl.data(), sizeof(RkCtl) * nc, hipMemcpyHostToDevice, c->stream));
+  if (c->poll) {  // every polled slot empty (a run stopped by an error may have left some written)
+    const int n = c->grid_s * kSlot;
+    hipLaunchKernelGGL(k_arm_slots, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.part2, n);
+    HIPCHK(hipGetLastError());
+  }
   SimArgs base = base_args(c);
   base.ncells = nc;
   base.sorted = c->batch ? 1 : 0;  // batch cells regroup their FRUs between attempts
@@ -3827,8 +4007,11 @@ int grw_set_timing(grw_ctx* c, int on) {
 
 int grw_set_layout(grw_ctx* c, int layout) {
   if (!c) return fail(GRW_EINVAL, "null context");
-  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_WAVE1_3)
-    return fail(GRW_EINVAL, "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4, _QUEUE, _WAVE1, _WAVE1_3");
+  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_QUADP)
+    return fail(GRW_EINVAL,
+                "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4, _QUEUE, _WAVE1, _WAVE1_3, _QUADP");
+  if (layout == GRW_LAYOUT_QUADP && (c->batch || c->ncells != 1))
+    return fail(GRW_EINVAL, "the polling layout runs single-cell contexts only");
   if (layout == GRW_LAYOUT_WAVES && (c->batch || c->ncells != 1))
     return fail(GRW_EINVAL, "the wave-specialised layout runs single-cell contexts only");
   // The queue layout is switched off: the one run of it on the device (a
@@ -3838,6 +4021,7 @@ int grw_set_layout(grw_ctx* c, int layout) {
   const bool ws = layout == GRW_LAYOUT_WAVES;
   const bool queue = layout == GRW_LAYOUT_QUEUE;
   const bool wave1 = layout == GRW_LAYOUT_WAVE1 || layout == GRW_LAYOUT_WAVE1_3;
+  const bool poll = layout == GRW_LAYOUT_QUADP;
   const int occ = (layout == GRW_LAYOUT_QUAD3 || layout == GRW_LAYOUT_WAVE1_3) ? 3 : layout == GRW_LAYOUT_QUAD4 ? 4 : 0;
   if (queue && c->qgrid == 0) {  // as many queue blocks as the GPU holds at once
     int dev = 0, ncu = 0, per = 0;
@@ -3850,7 +4034,7 @@ int grw_set_layout(grw_ctx* c, int layout) {
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc_queue<double>, 256, 0));
     c->qgrid = std::max(1, std::min(c->grid_s, std::max(per, 1) * std::max(ncu, 1)));
   }
-  if ((ws != c->ws || occ != c->occ || queue != c->queue || wave1 != c->wave1) && c->graph) {
+  if ((ws != c->ws || occ != c->occ || queue != c->queue || wave1 != c->wave1 || poll != c->poll) && c->graph) {
     // the captured launches are another kernel's
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -3859,6 +4043,12 @@ int grw_set_layout(grw_ctx* c, int layout) {
   c->occ = occ;
   c->queue = queue;
   c->wave1 = wave1;
+  if (poll && !c->poll) {  // every slot empty before the first polled launch
+    const int n = c->grid_s * kSlot;
+    hipLaunchKernelGGL(k_arm_slots, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.part2, n);
+    HIPCHK(hipGetLastError());
+  }
+  c->poll = poll;
   return 0;
 }
 
@@ -4206,7 +4396,8 @@ int autotune(grw_ctx* c, const std::vector<int>& layouts, const std::function<in
              GrwTune* out) {
   const bool ws0 = c->ws, g0 = c->graphs;
   const int occ0 = c->occ;
-  const int layout0 = c->queue ? GRW_LAYOUT_QUEUE
+  const int layout0 = c->poll ? GRW_LAYOUT_QUADP
+                      : c->queue ? GRW_LAYOUT_QUEUE
                       : c->wave1  ? (occ0 == 3 ? GRW_LAYOUT_WAVE1_3 : GRW_LAYOUT_WAVE1)
                       : ws0       ? GRW_LAYOUT_WAVES
                       : occ0 == 3 ? GRW_LAYOUT_QUAD3
@@ -4267,7 +4458,10 @@ int grw_autotune_cell(grw_ctx* c, const GrwProtocol* p, const double errw[37], d
     return grw_run_cell_device(c, p, errw, t0, nsteps, st, &hh, nullptr, nullptr, nullptr, nullptr, 0, &nlog,
                                nullptr);
   };
-  return autotune(c, {GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES, GRW_LAYOUT_QUAD3, GRW_LAYOUT_WAVE1}, run, trial_steps, out);
+  // the polling variant first: it is never slower than the four-lane kernel
+  // it derives from (launches shorter by ~2 us), so ties keep it
+  return autotune(c, {GRW_LAYOUT_QUADP, GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES, GRW_LAYOUT_QUAD3, GRW_LAYOUT_WAVE1}, run,
+                  trial_steps, out);
 }
 
 int grw_autotune_batch(grw_ctx* c, int trial_steps, GrwTune* out) {

@@ -105,7 +105,11 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
  *                     the next block as soon as its own FRUs are done, then one
  *                     block per cell for the cell's fcn and integrator stage;
  *   GRW_LAYOUT_WAVE1_3  the same with its registers capped for three
- *                     wavefronts per SIMD.
+ *                     wavefronts per SIMD;
+ *   GRW_LAYOUT_QUADP  single-cell contexts only: the four-lane kernel whose
+ *                     Monte Carlo blocks leave their partial sums without the
+ *                     last-block election, and the cell's fcn block collects
+ *                     them as they arrive and runs the integrator stage.
  * Every layout gives the same results bit for bit; the speed differs with the
  * workload (grw_autotune_cell / grw_autotune_batch time them).  No counterpart
  * in the reference. */
@@ -116,6 +120,7 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
 #define GRW_LAYOUT_QUEUE 5
 #define GRW_LAYOUT_WAVE1 6
 #define GRW_LAYOUT_WAVE1_3 7
+#define GRW_LAYOUT_QUADP 8
 int grw_set_layout(grw_ctx *ctx, int layout);
 
 /* Result of an autotuning pass: the choice (now set on the context) and every
@@ -130,12 +135,13 @@ typedef struct {
 
 /* Autotuner of the device integrator for a single cell: times trial_steps
  * output steps of grw_run_cell_device from (t0, state, h) under each of
- * GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES, GRW_LAYOUT_QUAD3 and GRW_LAYOUT_WAVE1, with and without
- * graph replays, every trial from a snapshot of the context's device state
- * (a warm-up step before each); restores the snapshot, so the context's FRU
- * state, beats and counters are as they were, and sets the fastest candidate
- * (a later candidate must beat the one kept by more than 1%: ties keep the
- * default, GRW_LAYOUT_QUAD without graphs).
+ * GRW_LAYOUT_QUADP, GRW_LAYOUT_QUAD, GRW_LAYOUT_WAVES, GRW_LAYOUT_QUAD3 and
+ * GRW_LAYOUT_WAVE1, with and without graph replays, every trial from a
+ * snapshot of the context's device state (a warm-up step before each);
+ * restores the snapshot, so the context's FRU state, beats and counters are as
+ * they were, and sets the fastest candidate (a later candidate must beat the
+ * one kept by more than 1%: ties keep the first, GRW_LAYOUT_QUADP without
+ * graphs).
  * The layouts give identical results, so the choice changes only the speed.
  * No counterpart in the reference. */
 int grw_autotune_cell(grw_ctx *ctx, const GrwProtocol *p, const double errw[37], double t0,

@@ -95,8 +95,9 @@ def test_wave_layout_is_refused_for_batches():
         assert L.grw_set_layout(bt._h, 1) == 0
         assert L.grw_set_layout(bt._h, 4) == 0
         assert L.grw_set_layout(bt._h, 5) != 0  # the queue layout is switched off
+        assert L.grw_set_layout(bt._h, 8) != 0  # the polling layout runs single cells only
     with Cell(64) as cell:
-        assert L.grw_set_layout(cell.fru._h, 8) != 0
+        assert L.grw_set_layout(cell.fru._h, 9) != 0
         assert L.grw_set_layout(cell.fru._h, 0) != 0
 
 
@@ -106,7 +107,7 @@ QUEUE_OFF = pytest.mark.skip(reason="the queue layout is switched off: its one d
 
 @pytest.mark.parametrize("layout,precision", [("quad3", "fp64"), ("quad4", "fp64"), ("quad3", "mixed"),
                                               ("wave1", "fp64"), ("wave1", "mixed"), ("wave1_3", "fp64"),
-                                              ("wave1_3", "mixed"),
+                                              ("wave1_3", "mixed"), ("quadp", "fp64"), ("quadp", "mixed"),
                                               pytest.param("queue", "fp64", marks=QUEUE_OFF),
                                               pytest.param("queue", "mixed", marks=QUEUE_OFF)])
 def test_other_layouts_equal_quad(layout, precision):
@@ -161,8 +162,8 @@ def test_cell_autotune_leaves_the_cell_as_it_was():
         ra0 = a.run(6)
         rb0 = b.run(6)
         info = b.autotune(trial_steps=4)
-        assert info["layout"] in ("quad", "waves", "quad3", "wave1")
-        assert len(info["candidates"]) == 8 and all(c["ms"] > 0 for c in info["candidates"])
+        assert info["layout"] in ("quad", "waves", "quad3", "wave1", "quadp")
+        assert len(info["candidates"]) == 10 and all(c["ms"] > 0 for c in info["candidates"])
         assert b.layout == info["layout"]
         ra = a.run(14)
         rb = b.run(14)
