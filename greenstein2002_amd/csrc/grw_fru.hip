@@ -948,13 +948,18 @@ __device__ __forceinline__ T in_vgpr(T x) {
   asm volatile("" : "+v"(x));
   return x;
 }
+template <bool V, class T>
+__device__ __forceinline__ T vinv(T x) {
+  if constexpr (V) return in_vgpr(x);
+  else return x;
+}
 
 // One sub-interval of every FRU with four lanes per FRU; gid is the lane's
 // index in its cell's four-lane grid (quad slot x 4 + cleft).  Returns this
 // lane's FRU-step count (nonzero only in lane 0 of a quad) and its flux terms
 // in acc.  NT is the block's thread count (the width of the lane-private LDS
-// columns).
-template <class R, int NT = 256>
+// columns); VINV keeps the launch's uniform values in vector registers.
+template <class R, int NT = 256, bool VINV = true>
 __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, double acc[17], int gid) {
   const int slot = gid >> 2;
   const int c = gid & 3;
@@ -991,17 +996,17 @@ __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, 
         }
       }
     }
-    const double t0 = in_vgpr(a.t0), t1 = in_vgpr(a.t1);
-    const double o0 = in_vgpr(a.dep0[0]), o1 = in_vgpr(a.dep0[1]), o2 = in_vgpr(a.dep0[2]);
-    const double span0 = in_vgpr(a.depf[0] - a.dep0[0]);
-    const double span1 = in_vgpr(a.depf[1] - a.dep0[1]);
-    const double span2 = in_vgpr(a.depf[2] - a.dep0[2]);
+    const double t0 = vinv<VINV>(a.t0), t1 = vinv<VINV>(a.t1);
+    const double o0 = vinv<VINV>(a.dep0[0]), o1 = vinv<VINV>(a.dep0[1]), o2 = vinv<VINV>(a.dep0[2]);
+    const double span0 = vinv<VINV>(a.depf[0] - a.dep0[0]);
+    const double span1 = vinv<VINV>(a.depf[1] - a.dep0[1]);
+    const double span2 = vinv<VINV>(a.depf[2] - a.dep0[2]);
     const double tint = a.t1 - a.t0;
-    const DivK kint{in_vgpr(tint), in_vgpr(1.0 / tint)};  // the sub-interval length, fixed for the launch
+    const DivK kint{vinv<VINV>(tint), vinv<VINV>(1.0 / tint)};  // the sub-interval length, fixed for the launch
     double dV = o0, dCai = o1, dNSR = o2;
     bool lopen = ch_lcc_open(w);
     int nopen = ch_ryr_open(w);
-    const R k23c = in_vgpr((R)a.k23c), gcf = in_vgpr((R)a.gcf);
+    const R k23c = vinv<VINV>((R)a.k23c), gcf = vinv<VINV>((R)a.gcf);
     double t = t0;
     int bad = 0;
     int nguard = 0;
@@ -2795,7 +2800,7 @@ __global__ void __launch_bounds__(256) k_mc_queue(const SimArgs base, const Dev 
 // k_dev_tail (one block per cell, after this kernel) adds them in the
 // four-lane kernel's order, evaluates the stage's fcn and runs the integrator
 // stage: the same numbers as GRW_LAYOUT_QUAD.  No block waits for another.
-template <class R>
+template <class R, bool VINV = true>
 __device__ __forceinline__ void mc_wave(const SimArgs& base, const Dev& d, const RkCtl* ctls) {
   const int i = (int)blockIdx.x;
   const int blk = base.sorted ? d.border[i >> 2] : (i >> 2);
@@ -2821,7 +2826,7 @@ __device__ __forceinline__ void mc_wave(const SimArgs& base, const Dev& d, const
   a.pro = (s0 == 1) ? ctl->pro : 0;
   a.avg = (s0 == 5);
   double acc[17] = {};
-  count_steps(d, simfru4_gid<R, 64>(a, d, acc, (blk - a.b0) * 256 + w * 64 + lane));
+  count_steps(d, simfru4_gid<R, 64, VINV>(a, d, acc, (blk - a.b0) * 256 + w * 64 + lane));
   double* const slot = d.part + (size_t)(4 * blk + w) * kSlot;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
@@ -2846,7 +2851,7 @@ __global__ void __launch_bounds__(64) k_mc_wave(const SimArgs base, const Dev d,
 template <class R, int W>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) k_mc_wave_occ(const SimArgs base,
                                                                                        const Dev d, RkCtl* ctls) {
-  mc_wave<R>(base, d, ctls);
+  mc_wave<R, false>(base, d, ctls);
 }
 
 // The four-lane stage with its registers capped so that W wavefronts fit on a
