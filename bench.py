@@ -553,6 +553,20 @@ def run_cell(args, precision, timing):
     return wall, chunks, st, beats
 
 
+def batch_layout(b, args, tag: str) -> str:
+    """The batch's Monte Carlo layout: the autotuner's choice from trials of
+    --batch-tune-steps ms (--layout auto), else the requested one (a
+    single-cell-only layout leaves the default)."""
+    if args.layout == "auto":
+        info = b.autotune(args.batch_tune_steps)
+        print(f"[bench] {tag}: autotuner chose {info['layout']}, graphs {info['graphs']}", file=sys.stderr, flush=True)
+        return info["layout"] + (" + graphs" if info["graphs"] else "")
+    if args.layout in ("waves", "quadp"):
+        return "quad"
+    b.set_layout(args.layout)
+    return args.layout
+
+
 def stage_kernels(layout: str, prec: str) -> str:
     """The kernels of one RK stage under a layout (what a stage's timed span covers)."""
     t = "double" if prec == "fp64" else "float"
@@ -622,12 +636,17 @@ def batched(args, precision: str | None = None, sequential: bool = True):
         print(f"[bench] batch cell {k} alone: {seq[-1]:.2f} s wall", file=sys.stderr, flush=True)
     seq_wall = float(np.mean(seq)) * n if seq else None
     beats = float(np.sum(args.batch_ms / cl))
+    # whole beats inside the window: beat k of a cell runs from its stimulus
+    # at 10 + k CL ms to the next one
+    done_beats = int(np.sum(np.floor(np.maximum(args.batch_ms - 10.0, 0.0) / cl)))
     return dict(
         workload=f"{n} cells, FRUs uniform in [1250, 12500] (total {int(nf.sum())}), cycle lengths uniform "
                  f"in [300, 2000] ms, t = 0..{args.batch_ms} ms from ic/stable, {prec}",
         cells=n, frus=int(nf.sum()), window_ms=args.batch_ms, wall_s=round(wall, 3),
         cell_ms_per_s=round(n * args.batch_ms / wall, 2),
         beats_per_s_est=round(beats / wall, 4),
+        beats_completed=done_beats,
+        beats_completed_per_s=round(done_beats / wall, 4),
         beats_per_s_basis=("estimate, not a count of finished beats: each cell's simulated ms over the window "
                            "divided by its cycle length, over the wall time; the window holds the stimulus, "
                            "upstroke and start of release, which cost more per simulated ms than the rest of "
@@ -662,10 +681,11 @@ def ensemble(args) -> dict:
     seeds = np.arange(1234, 1234 + args.ensemble)
     keys = ["APD90", "V_peak", "Cai_peak", "ICa_peak", "CaNSR_rest", "CaJSR_rest"]
     idx = [FIELDS.index(k) for k in keys]
-    res, walls = {}, {}
+    res, walls, layouts = {}, {}, {}
     for prec in ("fp64", "mixed"):
         specs = [CellSpec(args.nfru, seed=int(s)) for s in seeds]
         b = Batch(specs, precision=prec)
+        layouts[prec] = batch_layout(b, args, f"ensemble {prec}")
         t0 = time.perf_counter()
         done = 0
         while done < args.ensemble_ms:
@@ -680,7 +700,7 @@ def ensemble(args) -> dict:
     out = dict(metric="beat statistics of a seed ensemble, FP64 vs mixed vs CPU reference",
                workload=f"{len(seeds)} cells x {args.nfru} FRUs, seeds {seeds[0]}..{seeds[-1]}, 1 Hz, "
                         f"t = 0..{args.ensemble_ms} ms from ic/stable, beat at 10 ms",
-               wall_s={k: round(v, 2) for k, v in walls.items()}, stats={},
+               wall_s={k: round(v, 2) for k, v in walls.items()}, layout=layouts, stats={},
                beats={k: v.tolist() for k, v in res.items()}, fields=list(FIELDS))
     f64, mx = res["fp64"][:, idx], res["mixed"][:, idx]
     for j, k in enumerate(keys):
@@ -721,9 +741,10 @@ def clamp(args) -> dict:
             prot = Protocol(vclamp=True, vclamp_hold=-80.0, vclamp_set=float(vt), vclamp_shift=args.clamp_shift,
                             vclamp_duration=args.clamp_duration, vclamp_period=1000.0)
             specs.append(CellSpec(args.nfru, seed=int(sd), protocol=prot))
-    res, walls = {}, {}
+    res, walls, layouts = {}, {}, {}
     for prec in ("fp64", "mixed"):
         b = Batch(specs, precision=prec)
+        layouts[prec] = batch_layout(b, args, f"clamp {prec}")
         t0 = time.perf_counter()
         done = 0
         while done < ms:
@@ -738,7 +759,7 @@ def clamp(args) -> dict:
     out = dict(metric="voltage clamp, FP64 vs mixed",
                workload=f"{len(vts)} test potentials x {len(seeds)} seeds x {args.nfru} FRUs, hold -80 mV, "
                         f"step at {args.clamp_shift} ms for {args.clamp_duration} ms, t = 0..{ms} ms",
-               wall_s={k: round(v, 2) for k, v in walls.items()}, rows=[])
+               wall_s={k: round(v, 2) for k, v in walls.items()}, layout=layouts, rows=[])
     for i, vt in enumerate(vts):
         row = dict(Vt=float(vt))
         for name, col in (("ICa_peak", 8), ("Ca_in", 12), ("Ca_rel", 13)):
@@ -775,6 +796,7 @@ def sweep(args) -> dict:
     def run(cells, prec):
         b = Batch(cells, precision=prec)
         b.set_graphs(bool(args.graphs))
+        lay = batch_layout(b, args, f"sweep {prec}")
         b.set_timing(True)
         b.reset_stats()
         t0 = time.perf_counter()
@@ -792,7 +814,7 @@ def sweep(args) -> dict:
         alg = BYTES_PER_STEP * st["fru_steps"] + BYTES_PER_CALL * st["simfru_calls"]
         avg_s = st["kernel_ms"] / 1e3 / max(st["timed_launches"], 1)
         achieved = alg / max(st["launches"], 1) / avg_s / 1e9 if st["kernel_ms"] else 0.0
-        line = dict(precision=prec, cells=len(cells), wall_s=round(wall, 2),
+        line = dict(precision=prec, cells=len(cells), layout=lay, wall_s=round(wall, 2),
                     cell_ms_per_s=round(len(cells) * args.sweep_ms / wall, 2),
                     fru_steps=st["fru_steps"], fru_steps_per_s=round(st["fru_steps"] / wall, 1),
                     roofline=dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
