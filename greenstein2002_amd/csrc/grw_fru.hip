@@ -959,7 +959,7 @@ __device__ __forceinline__ T vinv(T x) {
 // lane's FRU-step count (nonzero only in lane 0 of a quad) and its flux terms
 // in acc.  NT is the block's thread count (the width of the lane-private LDS
 // columns); VINV keeps the launch's uniform values in vector registers.
-template <class R, int NT = 256, bool VINV = true>
+template <class R, int NT = 256, bool VINV = true, bool LATE = false>
 __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, double acc[17], int gid) {
   const int slot = gid >> 2;
   const int c = gid & 3;
@@ -1031,9 +1031,10 @@ __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, 
     // with the prefetch below and the first use of the state in every step
     // waits for all outstanding loads, prefetch included.
     __builtin_amdgcn_s_waitcnt(0);
-    if (t < t1) mt_fetch8(row, idx + 8 * c, win);
+    if (!LATE && t < t1) mt_fetch8(row, idx + 8 * c, win);
     Ghk<R> gv = ghk_terms<R>((R)dV);
     while (t < t1) {
+      if (LATE) mt_fetch8(row, idx + 8 * c, win);  // this step's window, fetched at its start
       // ---- rates at the step start (fru_rates_local.c:126-777)
       const R V = (R)dV;
       const R vm35 = V - R(35.0);
@@ -1152,7 +1153,7 @@ __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, 
       // positions idx+1 .. idx+32 (not yet regenerated) and words at least 195
       // positions older than idx, none of which this step's stores (positions
       // idx-32 .. idx-1, in either copy) or any store before their use touches.
-      mt_fetch8(row, idx + 8 * c, win);
+      if (!LATE) mt_fetch8(row, idx + 8 * c, win);
       // ---- transition decisions of this cleft (simfru_local.c:215-276); they
       // use only the step-start state, so they are taken ahead of the Heun step
       // in the same straight-line code and the two interleave
@@ -2800,7 +2801,7 @@ __global__ void __launch_bounds__(256) k_mc_queue(const SimArgs base, const Dev 
 // k_dev_tail (one block per cell, after this kernel) adds them in the
 // four-lane kernel's order, evaluates the stage's fcn and runs the integrator
 // stage: the same numbers as GRW_LAYOUT_QUAD.  No block waits for another.
-template <class R, bool VINV = true>
+template <class R, bool VINV = true, bool LATE = false>
 __device__ __forceinline__ void mc_wave(const SimArgs& base, const Dev& d, const RkCtl* ctls) {
   const int i = (int)blockIdx.x;
   const int blk = base.sorted ? d.border[i >> 2] : (i >> 2);
@@ -2826,7 +2827,7 @@ __device__ __forceinline__ void mc_wave(const SimArgs& base, const Dev& d, const
   a.pro = (s0 == 1) ? ctl->pro : 0;
   a.avg = (s0 == 5);
   double acc[17] = {};
-  count_steps(d, simfru4_gid<R, 64, VINV>(a, d, acc, (blk - a.b0) * 256 + w * 64 + lane));
+  count_steps(d, simfru4_gid<R, 64, VINV, LATE>(a, d, acc, (blk - a.b0) * 256 + w * 64 + lane));
   double* const slot = d.part + (size_t)(4 * blk + w) * kSlot;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
@@ -2851,7 +2852,7 @@ __global__ void __launch_bounds__(64) k_mc_wave(const SimArgs base, const Dev d,
 template <class R, int W>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) k_mc_wave_occ(const SimArgs base,
                                                                                        const Dev d, RkCtl* ctls) {
-  mc_wave<R, false>(base, d, ctls);
+  mc_wave<R, false, true>(base, d, ctls);
 }
 
 // The four-lane stage with its registers capped so that W wavefronts fit on a
