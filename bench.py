@@ -81,7 +81,7 @@ def parse():
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
-    ap.add_argument("--layout", choices=["auto", "quad", "waves", "quad3", "quad4", "wave1", "wave1_3", "quadp"], default="auto",
+    ap.add_argument("--layout", choices=["auto", "quad", "waves", "quad3", "quad4", "wave1", "wave1_3", "quadp", "wave1_4"], default="auto",
                     help="Monte Carlo kernel of the device integrator (grw_set_layout); auto: the autotuner "
                          "picks it and the graph replays from timed trials (grw_autotune_cell / _batch)")
     ap.add_argument("--tune-steps", type=int, default=10, help="simulated ms of each single-cell autotuner trial")
@@ -570,8 +570,8 @@ def batch_layout(b, args, tag: str) -> str:
 def stage_kernels(layout: str, prec: str) -> str:
     """The kernels of one RK stage under a layout (what a stage's timed span covers)."""
     t = "double" if prec == "fp64" else "float"
-    if layout in ("wave1", "wave1_3"):
-        return "k_mc_wave%s<%s> + k_dev_tail<%s> (one RK stage)" % ("_occ" if layout == "wave1_3" else "", t, t)
+    if layout in ("wave1", "wave1_3", "wave1_4"):
+        return "k_mc_wave%s<%s> + k_dev_tail<%s> (one RK stage)" % ("_occ" if layout != "wave1" else "", t, t)
     if layout == "quadp":
         return "k_dev_stage_poll<%s>" % t
     return "k_dev_stage<%s>" % t

@@ -166,7 +166,7 @@ class Batch:
 
     def set_layout(self, layout: str) -> None:
         """Monte Carlo kernel of the batch ("quad", "quad3", "quad4",
-        "queue", "wave1", "wave1_3"; see Cell.set_layout)."""
+        "queue", "wave1", "wave1_3", "wave1_4"; see Cell.set_layout)."""
         codes = {v: k for k, v in LAYOUT_NAMES.items()}
         if layout not in codes:
             raise ValueError("layout must be one of %s" % sorted(codes))

@@ -69,7 +69,7 @@ class GrwRkStats(C.Structure):
 
 
 TUNE_MAX = 12  # GRW_TUNE_MAX
-LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4", 5: "queue", 6: "wave1", 7: "wave1_3", 8: "quadp"}
+LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4", 5: "queue", 6: "wave1", 7: "wave1_3", 8: "quadp", 9: "wave1_4"}
 
 
 class GrwTune(C.Structure):
@@ -253,8 +253,8 @@ class Cell:
         wavefronts per SIMD; "queue", the four-lane kernel's work as a queue
         of 16-FRU items per wavefront, then a per-cell stage kernel (switched
         off); "wave1", the four-lane kernel's work with one wavefront per
-        block, then a per-cell stage kernel; "wave1_3", the same with
-        registers capped for three wavefronts per SIMD; "quadp", the
+        block, then a per-cell stage kernel; "wave1_3" / "wave1_4", the same
+        with registers capped for three / four wavefronts per SIMD; "quadp", the
         four-lane kernel whose fcn block collects the partials by polling
         (single cells).  All give the same results bit for bit."""
         codes = {v: k for k, v in LAYOUT_NAMES.items()}

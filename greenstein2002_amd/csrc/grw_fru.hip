@@ -3436,6 +3436,9 @@ void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent
         if (c->occ == 3)
           hipExtLaunchKernelGGL((k_mc_wave_occ<float, 3>), wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d,
                                 c->dctl);
+        else if (c->occ == 4)
+          hipExtLaunchKernelGGL((k_mc_wave_occ<float, 4>), wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d,
+                                c->dctl);
         else
           hipExtLaunchKernelGGL(k_mc_wave<float>, wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d, c->dctl);
         hipExtLaunchKernelGGL(k_dev_tail<float>, dim3(c->ncells), dim3(256), 0, c->stream, nullptr, e1, 0, base, c->d,
@@ -3443,6 +3446,9 @@ void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent
       } else {
         if (c->occ == 3)
           hipExtLaunchKernelGGL((k_mc_wave_occ<double, 3>), wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d,
+                                c->dctl);
+        else if (c->occ == 4)
+          hipExtLaunchKernelGGL((k_mc_wave_occ<double, 4>), wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d,
                                 c->dctl);
         else
           hipExtLaunchKernelGGL(k_mc_wave<double>, wgrid, dim3(64), 0, c->stream, e0, nullptr, 0, base, c->d, c->dctl);
@@ -4013,9 +4019,9 @@ int grw_set_timing(grw_ctx* c, int on) {
 
 int grw_set_layout(grw_ctx* c, int layout) {
   if (!c) return fail(GRW_EINVAL, "null context");
-  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_QUADP)
-    return fail(GRW_EINVAL,
-                "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4, _QUEUE, _WAVE1, _WAVE1_3, _QUADP");
+  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_WAVE1_4)
+    return fail(GRW_EINVAL, "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4, _QUEUE, _WAVE1, _WAVE1_3, "
+                            "_QUADP, _WAVE1_4");
   if (layout == GRW_LAYOUT_QUADP && (c->batch || c->ncells != 1))
     return fail(GRW_EINVAL, "the polling layout runs single-cell contexts only");
   if (layout == GRW_LAYOUT_WAVES && (c->batch || c->ncells != 1))
@@ -4026,9 +4032,11 @@ int grw_set_layout(grw_ctx* c, int layout) {
     return fail(GRW_EINVAL, "the queue layout is switched off (a device run of it did not finish)");
   const bool ws = layout == GRW_LAYOUT_WAVES;
   const bool queue = layout == GRW_LAYOUT_QUEUE;
-  const bool wave1 = layout == GRW_LAYOUT_WAVE1 || layout == GRW_LAYOUT_WAVE1_3;
+  const bool wave1 = layout == GRW_LAYOUT_WAVE1 || layout == GRW_LAYOUT_WAVE1_3 || layout == GRW_LAYOUT_WAVE1_4;
   const bool poll = layout == GRW_LAYOUT_QUADP;
-  const int occ = (layout == GRW_LAYOUT_QUAD3 || layout == GRW_LAYOUT_WAVE1_3) ? 3 : layout == GRW_LAYOUT_QUAD4 ? 4 : 0;
+  const int occ = (layout == GRW_LAYOUT_QUAD3 || layout == GRW_LAYOUT_WAVE1_3)   ? 3
+                  : (layout == GRW_LAYOUT_QUAD4 || layout == GRW_LAYOUT_WAVE1_4) ? 4
+                                                                                  : 0;
   if (queue && c->qgrid == 0) {  // as many queue blocks as the GPU holds at once
     int dev = 0, ncu = 0, per = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -4404,7 +4412,7 @@ int autotune(grw_ctx* c, const std::vector<int>& layouts, const std::function<in
   const int occ0 = c->occ;
   const int layout0 = c->poll ? GRW_LAYOUT_QUADP
                       : c->queue ? GRW_LAYOUT_QUEUE
-                      : c->wave1  ? (occ0 == 3 ? GRW_LAYOUT_WAVE1_3 : GRW_LAYOUT_WAVE1)
+                      : c->wave1  ? (occ0 == 3 ? GRW_LAYOUT_WAVE1_3 : occ0 == 4 ? GRW_LAYOUT_WAVE1_4 : GRW_LAYOUT_WAVE1)
                       : ws0       ? GRW_LAYOUT_WAVES
                       : occ0 == 3 ? GRW_LAYOUT_QUAD3
                       : occ0 == 4 ? GRW_LAYOUT_QUAD4
@@ -4478,8 +4486,10 @@ int grw_autotune_batch(grw_ctx* c, int trial_steps, GrwTune* out) {
     std::vector<int32_t> ns((size_t)c->ncells, nsteps);
     return grw_batch_run(c, ns.data(), nullptr, nullptr);
   };
-  return autotune(c, {GRW_LAYOUT_QUAD, GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4, GRW_LAYOUT_WAVE1, GRW_LAYOUT_WAVE1_3}, run,
-                  trial_steps, out);
+  return autotune(c,
+                  {GRW_LAYOUT_QUAD, GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4, GRW_LAYOUT_WAVE1, GRW_LAYOUT_WAVE1_3,
+                   GRW_LAYOUT_WAVE1_4},
+                  run, trial_steps, out);
 }
 
 }  // extern "C"

@@ -104,8 +104,8 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
  *                     per block, so that a wavefront's registers are free for
  *                     the next block as soon as its own FRUs are done, then one
  *                     block per cell for the cell's fcn and integrator stage;
- *   GRW_LAYOUT_WAVE1_3  the same with its registers capped for three
- *                     wavefronts per SIMD;
+ *   GRW_LAYOUT_WAVE1_3, GRW_LAYOUT_WAVE1_4  the same with its registers
+ *                     capped for three or four wavefronts per SIMD;
  *   GRW_LAYOUT_QUADP  single-cell contexts only: the four-lane kernel whose
  *                     Monte Carlo blocks leave their partial sums without the
  *                     last-block election, and the cell's fcn block collects
@@ -121,6 +121,7 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
 #define GRW_LAYOUT_WAVE1 6
 #define GRW_LAYOUT_WAVE1_3 7
 #define GRW_LAYOUT_QUADP 8
+#define GRW_LAYOUT_WAVE1_4 9
 int grw_set_layout(grw_ctx *ctx, int layout);
 
 /* Result of an autotuning pass: the choice (now set on the context) and every
