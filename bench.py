@@ -81,7 +81,8 @@ def parse():
     ap.add_argument("--no-timing", action="store_true",
                     help="do not record per-launch HIP events (for profiler runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
-    ap.add_argument("--layout", choices=["auto", "quad", "waves", "quad3", "quad4", "wave1", "wave1_3", "quadp", "wave1_4"], default="auto",
+    ap.add_argument("--layout", choices=["auto", "quad", "waves", "quad3", "quad4", "wave1", "wave1_3", "quadp", "wave1_4",
+                                         "refill", "refill_3"], default="auto",
                     help="Monte Carlo kernel of the device integrator (grw_set_layout); auto: the autotuner "
                          "picks it and the graph replays from timed trials (grw_autotune_cell / _batch)")
     ap.add_argument("--tune-steps", type=int, default=10, help="simulated ms of each single-cell autotuner trial")
@@ -572,6 +573,8 @@ def stage_kernels(layout: str, prec: str) -> str:
     t = "double" if prec == "fp64" else "float"
     if layout in ("wave1", "wave1_3", "wave1_4"):
         return "k_mc_wave%s<%s> + k_dev_tail<%s> (one RK stage)" % ("_occ" if layout != "wave1" else "", t, t)
+    if layout in ("refill", "refill_3"):
+        return "k_mc_refill%s<%s> + k_dev_tail<%s> (one RK stage)" % ("3" if layout == "refill_3" else "", t, t)
     if layout == "quadp":
         return "k_dev_stage_poll<%s>" % t
     return "k_dev_stage<%s>" % t
@@ -661,7 +664,8 @@ def batched(args, precision: str | None = None, sequential: bool = True):
                       kernel=stage_kernels(TUNED[key]["layout"] if key in TUNED else args.layout, prec) +
                       " over all cells",
                       avg_launch_us=round(avg_s * 1e6, 3),
-                      launches=st["launches"], timed_launches=st["timed_launches"]),
+                      launches=st["launches"], timed_launches=st["timed_launches"],
+                      lane_use=round(st["fru_steps"] / max(st["slot_steps"], 1), 4)),
         sequential=dict(sample_cells=[int(k) for k in pick], sample_walls_s=[round(x, 3) for x in seq],
                         est_wall_s=round(seq_wall, 2), beats_per_s_est=round(beats / seq_wall, 4)) if seq else None,
         speedup_vs_sequential=round(seq_wall / wall, 2) if seq else None)
@@ -935,6 +939,7 @@ def main():
                     critical_steps_per_launch=round(st["critical_steps"] / st["launches"], 2),
                     us_per_critical_step=round(avg_s * 1e6 * st["launches"] / max(st["critical_steps"], 1), 3),
                     serial_tail_us=round(st["tail_ms"] * 1e3 / st["launches"], 3),
+                    lane_use=round(st["fru_steps"] / max(st["slot_steps"], 1), 4),
                     alg_bytes_per_launch=round(alg_bytes / st["launches"], 1))
         if kt:
             roof["kernel_busy_ms"] = round(kt["busy_ms"], 3)

@@ -63,6 +63,7 @@ class GrwStats(C.Structure):
         ("critical_steps", C.c_uint64),
         ("tail_ms", C.c_double),
         ("timed_launches", C.c_uint64),
+        ("slot_steps", C.c_uint64),
     ]
 
 

@@ -156,7 +156,7 @@ class Batch:
         check(self._L.grw_get_stats(self._h, C.byref(st)))
         return dict(fru_steps=st.fru_steps, simfru_calls=st.simfru_calls, launches=st.launches,
                     kernel_ms=st.kernel_ms, critical_steps=st.critical_steps, tail_ms=st.tail_ms,
-                    timed_launches=st.timed_launches)
+                    timed_launches=st.timed_launches, slot_steps=st.slot_steps)
 
     def reset_stats(self) -> None:
         check(self._L.grw_reset_stats(self._h))
@@ -166,7 +166,8 @@ class Batch:
 
     def set_layout(self, layout: str) -> None:
         """Monte Carlo kernel of the batch ("quad", "quad3", "quad4",
-        "queue", "wave1", "wave1_3", "wave1_4"; see Cell.set_layout)."""
+        "queue", "wave1", "wave1_3", "wave1_4", "refill", "refill_3"; see
+        Cell.set_layout)."""
         codes = {v: k for k, v in LAYOUT_NAMES.items()}
         if layout not in codes:
             raise ValueError("layout must be one of %s" % sorted(codes))

@@ -68,8 +68,9 @@ class GrwRkStats(C.Structure):
     _fields_ = [("attempts", C.c_int64), ("rejects", C.c_int64), ("forced", C.c_int64)]
 
 
-TUNE_MAX = 12  # GRW_TUNE_MAX
-LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4", 5: "queue", 6: "wave1", 7: "wave1_3", 8: "quadp", 9: "wave1_4"}
+TUNE_MAX = 16  # GRW_TUNE_MAX
+LAYOUT_NAMES = {1: "quad", 2: "waves", 3: "quad3", 4: "quad4", 5: "queue", 6: "wave1", 7: "wave1_3", 8: "quadp", 9: "wave1_4",
+                10: "refill", 11: "refill_3"}
 
 
 class GrwTune(C.Structure):
@@ -256,7 +257,10 @@ class Cell:
         block, then a per-cell stage kernel; "wave1_3" / "wave1_4", the same
         with registers capped for three / four wavefronts per SIMD; "quadp", the
         four-lane kernel whose fcn block collects the partials by polling
-        (single cells).  All give the same results bit for bit."""
+        (single cells); "refill", one wavefront per 64 FRUs whose quads take
+        the next FRU as soon as theirs is done, then a per-cell stage kernel;
+        "refill_3", the same with registers capped for three wavefronts per
+        SIMD.  All give the same results bit for bit."""
         codes = {v: k for k, v in LAYOUT_NAMES.items()}
         if layout not in codes:
             raise ValueError("layout must be one of %s" % sorted(codes))

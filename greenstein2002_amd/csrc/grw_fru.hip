@@ -192,6 +192,7 @@ struct Dev {
   int* err;
   unsigned int* maxsteps;           // largest FRU-step count of the current launch
   unsigned long long* critsteps;    // sum over launches of maxsteps
+  unsigned long long* slotsteps;    // sum over wavefronts of (FRU slots x the slowest slot's micro-steps)
   unsigned long long* tailticks;    // wall-clock ticks spent in the serial launch tails
   const CellInfo* cells;            // [ncells]
   const int* smap;                  // Monte Carlo grid block -> cell
@@ -593,8 +594,15 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
   if (SIM) {
     // FRU-step count for the throughput figures
     unsigned long long s = steps;
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(d.nsteps, s);
+    unsigned m = steps;
+    for (int off = 32; off >= 1; off >>= 1) {
+      s += __shfl_xor(s, off);
+      m = max(m, (unsigned)__shfl_xor((int)m, off));
+    }
+    if ((threadIdx.x & 63) == 0 && s) {
+      atomicAdd(d.nsteps, s);
+      atomicAdd(d.slotsteps, 64ull * m);  // one FRU per lane
+    }
   }
   if (REDUCE) {
     double tot[5];
@@ -959,6 +967,363 @@ __device__ __forceinline__ T vinv(T x) {
 // lane's FRU-step count (nonzero only in lane 0 of a quad) and its flux terms
 // in acc.  NT is the block's thread count (the width of the lane-private LDS
 // columns); VINV keeps the launch's uniform values in vector registers.
+// One quad's FRU through a sub-interval (four lanes, one per cleft; CaJSR,
+// time, step length and MT position are kept identically in the four): the
+// state the micro-step loop carries.
+template <class R>
+struct QuadFru {
+  uint32_t* row;
+  double cj, css, t, dV, dCai, dNSR;
+  uint32_t w;
+  int idx, nreg, nopen, bad, nguard;
+  bool lopen;
+  unsigned steps;
+  Ghk<R> gv;
+  MtWin8 win;
+};
+
+// What every FRU of a cell shares in one launch: the sub-interval, the
+// interpolation end points of (V, Cai, CaNSR), the cell's rate scales, and the
+// lane's cleft and LDS column.
+template <class R>
+struct QuadStage {
+  int c, tl;
+  double t0, t1, o0, o1, o2, span0, span1, span2;
+  DivK kint;
+  R k23c, gcf;
+};
+
+template <class R, bool VINV>
+__device__ __forceinline__ QuadStage<R> quad_stage(const SimArgs& a, int c, int tl) {
+  QuadStage<R> k;
+  k.c = c;
+  k.tl = tl;
+  k.t0 = vinv<VINV>(a.t0);
+  k.t1 = vinv<VINV>(a.t1);
+  k.o0 = vinv<VINV>(a.dep0[0]);
+  k.o1 = vinv<VINV>(a.dep0[1]);
+  k.o2 = vinv<VINV>(a.dep0[2]);
+  k.span0 = vinv<VINV>(a.depf[0] - a.dep0[0]);
+  k.span1 = vinv<VINV>(a.depf[1] - a.dep0[1]);
+  k.span2 = vinv<VINV>(a.depf[2] - a.dep0[2]);
+  const double tint = a.t1 - a.t0;
+  k.kint = DivK{vinv<VINV>(tint), vinv<VINV>(1.0 / tint)};  // the sub-interval length, fixed for the launch
+  k.k23c = vinv<VINV>((R)a.k23c);
+  k.gcf = vinv<VINV>((R)a.gcf);
+  return k;
+}
+
+// The lane's constant LDS entries: the RyR totals of states 4 and 6, and
+// omega / bL^k of the mode-Ca LCC states CCa0..CCa4 (fru_rates_local.c:70-76
+// constant expressions), one column per lane.
+template <class R, int NT>
+__device__ __forceinline__ void quad_tables(R (*rtot)[NT], R (*lom)[NT], int tl) {
+  rtot[3][tl] = R(k43 + k45);
+  rtot[5][tl] = R(k65);
+  lom[0][tl] = R(omega);
+  lom[1][tl] = R(omega / bL);
+  lom[2][tl] = R(omega / bL2);
+  lom[3][tl] = R(omega / bL3);
+  lom[4][tl] = R(omega / bL4);
+}
+
+// The loop state of an FRU at the start of its sub-interval, its
+// concentrations, channel word and MT position given.
+template <class R>
+__device__ __forceinline__ void quad_init(const QuadStage<R>& k, QuadFru<R>& s) {
+  s.dV = k.o0;
+  s.dCai = k.o1;
+  s.dNSR = k.o2;
+  s.lopen = ch_lcc_open(s.w);
+  s.nopen = ch_ryr_open(s.w);
+  s.t = k.t0;
+  s.bad = 0;
+  s.nguard = 0;
+  s.steps = 0;
+}
+
+// Load FRU f's state at the start of its sub-interval, with the checkpoint
+// (a.pro == 1) or the rollback to it (a.pro == 2) of mpi_slave.c:150-226.
+template <class R, bool LATE>
+__device__ __forceinline__ void quad_begin(const SimArgs& a, const Dev& d, int f, const QuadStage<R>& k,
+                                           QuadFru<R>& s) {
+  const int c = k.c;
+  s.row = d.mt + (size_t)f * kMtStride;
+  if (a.pro == 2) {
+    // rollback (mpi_slave.c:150-184): start from the checkpoint; lane 0
+    // restores the MT words, the fence makes them visible to its quad
+    s.cj = d.conc_h[f];
+    s.css = d.conc_h[(c + 1) * a.ld + f];
+    s.w = d.chan_h[c * a.ld + f];
+    s.idx = d.mtidx_h[f];
+    s.nreg = 0;
+    if (c == 0) mt_rollback(s.row, s.idx, d.nreg[f], d.mtw_h[f]);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  } else {
+    s.cj = d.conc[f];
+    s.css = d.conc[(c + 1) * a.ld + f];
+    s.w = d.chan[c * a.ld + f];
+    s.idx = d.mtidx[f];
+    s.nreg = a.pro ? 0 : d.nreg[f];
+    if (a.pro == 1) {  // checkpoint (mpi_slave.c:192-226)
+      d.conc_h[(c + 1) * a.ld + f] = s.css;
+      d.chan_h[c * a.ld + f] = s.w;
+      if (c == 0) {
+        d.conc_h[f] = s.cj;
+        d.mtidx_h[f] = s.idx;
+        d.mtw_h[f] = s.row[mt_first_pos(s.idx)];
+      }
+    }
+  }
+  quad_init<R>(k, s);
+}
+
+// The first window of draws (one step ahead unless LATE) and the V-only flux
+// factors, once the state loads are complete.
+template <class R, bool LATE>
+__device__ __forceinline__ void quad_prime(const QuadStage<R>& k, QuadFru<R>& s) {
+  if (!LATE && s.t < k.t1) mt_fetch8(s.row, s.idx + 8 * k.c, s.win);
+  s.gv = ghk_terms<R>((R)s.dV);
+}
+
+// One micro-step of the quad's FRU (simfru_local.c:127-277).  Returns false,
+// with s.bad set, on an unknown channel state, non-finite rates or a runaway.
+template <class R, int NT, bool LATE>
+__device__ __forceinline__ bool quad_step(const QuadStage<R>& k, QuadFru<R>& s, R (*rtot)[NT], R (*lom)[NT]) {
+  const int c = k.c, tl = k.tl;
+  uint32_t* const row = s.row;
+  uint32_t w = s.w;
+  int idx = s.idx;
+  const double t1 = k.t1;
+  if (LATE) mt_fetch8(row, idx + 8 * c, s.win);  // this step's window, fetched at its start
+  // ---- rates at the step start (fru_rates_local.c:126-777)
+  const R V = (R)s.dV;
+  const R vm35 = V - R(35.0);
+  const R e0 = R(0.012) * vm35, e1 = R(-0.05) * vm35;
+  const R e2 = rdiv(V + R(12.5), divk(5.0)), e3 = rdiv(V + R(30.0), divk(12.0));
+  // One exp per lane.  Independent selects: a chained conditional on a
+  // lane-varying value becomes a branch tree, and since the lanes of a
+  // wavefront disagree every arm of it runs, each behind its own exec-mask
+  // handling.
+  R earg = e3;
+  earg = (c == 2) ? e2 : earg;
+  earg = (c == 1) ? e1 : earg;
+  earg = (c == 0) ? e0 : earg;
+  const R ex = xexp(earg);
+  const R alpha = R(alphacf) * qperm<QB0>(ex);
+  const R beta = R(betacf) * qperm<QB1>(ex);
+  const R yinf = rdiv(R(yCa_frac), R(1.0) + qperm<QB2>(ex)) + (R(1.0) - R(yCa_frac));
+  const R tau = R(60.0) + rdiv(R(340.0), R(1.0) + qperm<QB3>(ex));
+  const R bp = rdiv(beta, divk(bL * 1.0));
+  const R cc = (R)s.css;
+  const int ls = ch_lcc(w);
+  int bad = s.bad | (ls < 1 || ls > 12);
+  R lb, lf, lm, lt;
+  const int lk = ls - 7;  // CCa index; other states read entry 0 and ignore it
+  lcc_arith<R>(ls, alpha, beta, bp, k.gcf * cc, lom[(lk >= 0 && lk <= 4) ? lk : 0][tl], lb, lf, lm, lt);
+  const R vr = rdiv(ch_y(w) == 2 ? (R(1.0) - yinf) : yinf, tau);
+  const R ir = (ch_ito2(w) == 1) ? R(kfIto2) * cc : R(kbIto2);
+  R maxr = vmax(vmax(vmax(R(0.0), lt), vr), ir);
+  const RyrK<R> rk = ryr_k(cc, k.k23c);
+  int st[kNryr];
+  int nd = 0;
+  unsigned need = 0;
+  uint32_t rbits = ryr_resolve_bits(3);  // independent selects, as for earg
+  rbits = (rk.regime == 2) ? ryr_resolve_bits(2) : rbits;
+  rbits = (rk.regime == 1) ? ryr_resolve_bits(1) : rbits;
+#pragma unroll
+  for (int r = 0; r < kNryr; ++r) {
+    const uint32_t b = (rbits >> (4u * ((w >> (5 + 3 * r)) & 7u))) & 15u;  // raw state ch_ryr(w, r)
+    st[r] = (int)(b & 7u) + 1;
+    nd += (int)(b >> 3);
+    need |= (b >> 3) << r;
+  }
+  // merged-state uniforms come first, cleft by cleft (fru_rates_local.c:452,476,581)
+  const int n0 = qperm<QB0>(nd), n1 = qperm<QB1>(nd), n2 = qperm<QB2>(nd), n3 = qperm<QB3>(nd);
+  const int E = n0 + n1 + n2 + n3;
+  const int lazy0 = idx > kMtN ? idx : kMtN;
+  // totals of the resolved RyR states (fru_rates_local.c:390-727); they
+  // depend on CaSS alone, so they are computed ahead of the merged draws
+  const R T2 = (R(k21) + rk.k23 + rk.k25);
+  const R T3 = (R(k32) + rk.k34);
+  const R T5 = (R(k52) + rk.k54 + rk.k56);
+  const R a7 = rdiv(rk.k34, R(k43) + rk.k34);
+  const R T7 = ((R(1.0) - a7) * R(k32)) + (a7 * R(k45));
+  const R a8 = rdiv(R(k65), rk.k56 + R(k65));
+  const R T8 = (a8 * R(k52)) + (a8 * rk.k54);
+  rtot[0][tl] = rk.k12;
+  rtot[1][tl] = T2;
+  rtot[2][tl] = T3;
+  rtot[4][tl] = T5;
+  rtot[6][tl] = T7;
+  rtot[7][tl] = T8;
+  if (nd) {
+    const int Ec = (c > 0 ? n0 : 0) + (c > 1 ? n1 : 0) + (c > 2 ? n2 : 0);
+    uint32_t vw[kNryr];
+    mt_run<kNryr>(row, idx + Ec, nd, vw);
+#pragma unroll
+    for (int r = 0; r < kNryr; ++r) {
+      if ((need >> r) & 1u) {
+        st[r] = ryr_split(st[r], unif(vw[0]), rk);
+#pragma unroll
+        for (int j = 0; j < kNryr - 1; ++j) vw[j] = vw[j + 1];
+      }
+    }
+  }
+  R tr[kNryr];
+#pragma unroll
+  for (int r = 0; r < kNryr; ++r) {
+    w = ch_set_ryr(w, r, st[r]);
+    tr[r] = rtot[st[r] - 1][tl];  // total of the resolved state (fru_rates_local.c:390-727)
+    maxr = vmax_nc(maxr, tr[r]);
+  }
+  maxr = vmax_nc(maxr, qperm<QX1>(maxr));
+  maxr = vmax_nc(maxr, qperm<QX2>(maxr));
+  bad |= qperm<QX1>(bad);
+  bad |= qperm<QX2>(bad);
+  // ---- step length (simfru_local.c:135-146)
+  double dt = 0.1 / (double)maxr;
+  dt = vmin(dt, 10.001e-3);
+  double cm = vmax_nc(s.css, qperm<QX1>(s.css));
+  cm = vmax_nc(cm, qperm<QX2>(cm));
+  const double cmax = vmax(0.e-3, cm);
+  if (cmax > 1.5e-3) dt = vmin(dt, 5.001e-3);
+  dt = vmin(dt, t1 - s.t);
+  // an unknown channel state, non-finite rates or a runaway (quad-uniform)
+  if (bad || !(s.t + dt > s.t) || ++s.nguard > kStepCap) {
+    s.w = w;
+    s.bad = 1;
+    return false;
+  }
+  // ---- this cleft's 8 of the step's 32 uniforms (simfru_local.c:211), from
+  // the words fetched during the previous step; when merged-state draws
+  // moved the stream (E > 0, rare) the window is fetched again at the new
+  // position first, so that both cases run the same code and the common
+  // one waits for nothing but its window
+  uint32_t dw[8];
+  if (E != 0) mt_fetch8(row, idx + E + 8 * c, s.win);
+  mt_finish8(row, idx + E + 8 * c, s.win, dw);
+  {
+    const int idn = idx + E + 32;
+    const int adv = idn - lazy0;
+    s.nreg += (adv > 0 ? adv : 0);
+    idx = (idn >= 2 * kMtN) ? idn - kMtN : idn;
+  }
+  // Next step's words, fetched now so their latency hides behind this
+  // step's Heun step and transitions and the next step's rates.  The words
+  // used are positions idx .. idx+31 and those they regenerate from:
+  // positions idx+1 .. idx+32 (not yet regenerated) and words at least 195
+  // positions older than idx, none of which this step's stores (positions
+  // idx-32 .. idx-1, in either copy) or any store before their use touches.
+  if (!LATE) mt_fetch8(row, idx + 8 * c, s.win);
+  // ---- transition decisions of this cleft (simfru_local.c:215-276); they
+  // use only the step-start state, so they are taken ahead of the Heun step
+  // in the same straight-line code and the two interleave
+  const double u0 = unif(dw[0]);
+  const bool mv_lcc = u0 > 1.0 - dt * (double)lt;
+  const bool mv_y = unif(dw[1]) >= 1.0 - dt * (double)vr;
+  const bool mv_ito = unif(dw[7]) >= 1.0 - dt * (double)ir;
+  double ur[kNryr];
+  bool mv_r[kNryr];
+  bool moved = false;
+#pragma unroll
+  for (int r = 0; r < kNryr; ++r) {
+    ur[r] = unif(dw[2 + r]);
+    mv_r[r] = ur[r] > 1.0 - dt * (double)tr[r];
+    moved |= mv_r[r];
+  }
+  // ---- Heun step (simfru_local.c:149-173)
+  R djs, dss;
+  fcn_fru4<R>((R)s.cj, cc, s.gv, (R)s.dCai, (R)s.dNSR, s.lopen, s.nopen, djs, dss);
+  const double k1j = dt * (double)djs, k1s = dt * (double)dss;
+  const double y1j = s.cj + k1j, y1s = s.css + k1s;
+  s.t = s.t + dt;
+  const double frac = rdiv(s.t - k.t0, k.kint);  // (t - t0) / tint, correctly rounded
+  s.dV = k.o0 + frac * k.span0;
+  s.dCai = k.o1 + frac * k.span1;
+  s.dNSR = k.o2 + frac * k.span2;
+  s.gv = ghk_terms<R>((R)s.dV);  // also the next step's first evaluation
+  fcn_fru4<R>((R)y1j, (R)y1s, s.gv, (R)s.dCai, (R)s.dNSR, s.lopen, s.nopen, djs, dss);
+  s.cj = s.cj + (k1j + dt * (double)djs) / 2.0;
+  s.css = s.css + (k1s + dt * (double)dss) / 2.0;
+  // ---- the transitions taken
+  if (mv_lcc) w = ch_set_lcc(w, lcc_move<R>(ls, u0, dt, lb, lf, lm, lt));
+  if (mv_y) w = ch_flip_y(w);
+  // the RyRs that move, lowest first, one exit walk per round: lanes moving
+  // different RyRs share a round instead of running one body per RyR index
+  // (the moves touch disjoint fields of w, so the order is immaterial)
+  unsigned mm = 0;
+#pragma unroll
+  for (int r = 0; r < kNryr; ++r) mm |= mv_r[r] ? (1u << r) : 0u;
+  while (mm) {
+    const int r = __builtin_ctz(mm);
+    mm &= mm - 1u;
+    int sr = st[0];
+    double ru = ur[0];
+#pragma unroll
+    for (int q = 1; q < kNryr; ++q) {
+      sr = (r == q) ? st[q] : sr;
+      ru = (r == q) ? ur[q] : ru;
+    }
+    Exits<R> e;
+    ryr_exits_sel(sr, rk, a7, a8, rtot[sr - 1][tl], e);
+    w = ch_set_ryr(w, r, pick<R>(ru, dt, e));
+  }
+  if (mv_ito) w = ch_flip_ito2(w);
+  s.lopen = ch_lcc_open(w);
+  if (moved) s.nopen = ch_ryr_open(w);
+  s.steps += (c == 0);
+  s.w = w;
+  s.idx = idx;
+  s.bad = bad;
+  return true;
+}
+
+// The flux-sum terms of one cleft of a finished FRU (calc_fru_flux_local.c,
+// and calc_fru_avg_local.c:135-176 when avg is set; the CaJSR ones in lane 0).
+__device__ __forceinline__ void quad_acc(double css, double cj, uint32_t w, bool lopen, int nopen, int c, bool avg,
+                                         double acc[17]) {
+  acc[0] = css;
+  acc[1] = (c == 0) ? cj : 0.0;
+  acc[2] = lopen ? css : 0.0;
+  acc[3] = lopen ? 1.0 : 0.0;
+  acc[4] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
+  acc[5] = JRyRmax * ((double)nopen) * (cj - css);  // this cleft's release flux (calc_fru_avg_local.c)
+  if (avg) {
+    int nr = 0;
+#pragma unroll
+    for (int r = 0; r < kNryr; ++r) {
+      const int st = ch_ryr(w, r);
+      nr += (st == 1 || st == 2);
+    }
+    acc[6] = css;
+    acc[7] = (c == 0) ? cj : 0.0;
+    acc[8] = JRyRmax * ((double)nopen) * (cj - css);
+    acc[9] = (double)nopen;
+    acc[10] = (double)nr;
+    acc[11] = css * (1.0 + BSRtot / (KBSR + css) + BSLtot / (KBSL + css));
+    acc[12] = (c == 0) ? cj * (1.0 + CSQNtot / (KmCSQN + cj)) : 0.0;
+    acc[13] = (ch_lcc(w) <= 6) ? 1.0 : 0.0;
+    acc[14] = (ch_y(w) == 2) ? 1.0 : 0.0;
+    acc[15] = lopen ? 1.0 : 0.0;
+    acc[16] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
+  }
+}
+
+// Store the FRU's state at the end of its sub-interval.
+template <class R>
+__device__ __forceinline__ void quad_end(const SimArgs& a, const Dev& d, int f, int c, const QuadFru<R>& s) {
+  if (s.bad && c == 0) atomicOr(d.err, 1);
+  d.conc[(c + 1) * a.ld + f] = s.css;
+  d.chan[c * a.ld + f] = s.w;
+  if (c == 0) {
+    d.conc[f] = s.cj;
+    d.mtidx[f] = s.idx;
+    d.nreg[f] = s.nreg;
+    if (a.sorted && a.t1 > a.t0) d.cost[f] = s.steps;
+  }
+}
+
 template <class R, int NT = 256, bool VINV = true, bool LATE = false>
 __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, double acc[17], int gid) {
   const int slot = gid >> 2;
@@ -966,286 +1331,29 @@ __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, 
   unsigned steps = 0;
   if (slot < a.nfru) {  // quads are active or inactive as a whole
     const int f = a.f0 + (a.sorted ? d.perm[a.f0 + slot] : slot);
-    uint32_t* const row = d.mt + (size_t)f * kMtStride;
-    double cj, css;
-    uint32_t w;
-    int idx, nreg;
-    if (a.pro == 2) {
-      // rollback (mpi_slave.c:150-184): start from the checkpoint; lane 0
-      // restores the MT words, the fence makes them visible to its quad
-      cj = d.conc_h[f];
-      css = d.conc_h[(c + 1) * a.ld + f];
-      w = d.chan_h[c * a.ld + f];
-      idx = d.mtidx_h[f];
-      nreg = 0;
-      if (c == 0) mt_rollback(row, idx, d.nreg[f], d.mtw_h[f]);
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    } else {
-      cj = d.conc[f];
-      css = d.conc[(c + 1) * a.ld + f];
-      w = d.chan[c * a.ld + f];
-      idx = d.mtidx[f];
-      nreg = a.pro ? 0 : d.nreg[f];
-      if (a.pro == 1) {  // checkpoint (mpi_slave.c:192-226)
-        d.conc_h[(c + 1) * a.ld + f] = css;
-        d.chan_h[c * a.ld + f] = w;
-        if (c == 0) {
-          d.conc_h[f] = cj;
-          d.mtidx_h[f] = idx;
-          d.mtw_h[f] = row[mt_first_pos(idx)];
-        }
-      }
-    }
-    const double t0 = vinv<VINV>(a.t0), t1 = vinv<VINV>(a.t1);
-    const double o0 = vinv<VINV>(a.dep0[0]), o1 = vinv<VINV>(a.dep0[1]), o2 = vinv<VINV>(a.dep0[2]);
-    const double span0 = vinv<VINV>(a.depf[0] - a.dep0[0]);
-    const double span1 = vinv<VINV>(a.depf[1] - a.dep0[1]);
-    const double span2 = vinv<VINV>(a.depf[2] - a.dep0[2]);
-    const double tint = a.t1 - a.t0;
-    const DivK kint{vinv<VINV>(tint), vinv<VINV>(1.0 / tint)};  // the sub-interval length, fixed for the launch
-    double dV = o0, dCai = o1, dNSR = o2;
-    bool lopen = ch_lcc_open(w);
-    int nopen = ch_ryr_open(w);
-    const R k23c = vinv<VINV>((R)a.k23c), gcf = vinv<VINV>((R)a.gcf);
-    double t = t0;
-    int bad = 0;
-    int nguard = 0;
+    const int tl = threadIdx.x;
+    const QuadStage<R> k = quad_stage<R, VINV>(a, c, tl);
+    QuadFru<R> s;
+    quad_begin<R, LATE>(a, d, f, k, s);
     // Each lane's totals of the eight RyR states, one LDS column per lane, so
     // that a RyR's total is one indexed read instead of a chain of selects
     // over its state.  A lane reads only what it wrote itself, in program
     // order, so no barrier is needed.  States 4 and 6 have constant totals.
     __shared__ R rtot[8][NT];
-    const int tl = threadIdx.x;
-    rtot[3][tl] = R(k43 + k45);
-    rtot[5][tl] = R(k65);
-    // omega / bL^k of the mode-Ca LCC states CCa0..CCa4, likewise one column
-    // per lane (fru_rates_local.c:70-76 constant expressions)
+    // omega / bL^k of the mode-Ca LCC states, likewise one column per lane
     __shared__ R lom[5][NT];
-    lom[0][tl] = R(omega);
-    lom[1][tl] = R(omega / bL);
-    lom[2][tl] = R(omega / bL2);
-    lom[3][tl] = R(omega / bL3);
-    lom[4][tl] = R(omega / bL4);
-    MtWin8 win;  // words of the next step's draws, fetched one step ahead
+    quad_tables<R, NT>(rtot, lom, tl);
     // Complete the FRU-state loads here: otherwise the loop header merges them
     // with the prefetch below and the first use of the state in every step
     // waits for all outstanding loads, prefetch included.
     __builtin_amdgcn_s_waitcnt(0);
-    if (!LATE && t < t1) mt_fetch8(row, idx + 8 * c, win);
-    Ghk<R> gv = ghk_terms<R>((R)dV);
-    while (t < t1) {
-      if (LATE) mt_fetch8(row, idx + 8 * c, win);  // this step's window, fetched at its start
-      // ---- rates at the step start (fru_rates_local.c:126-777)
-      const R V = (R)dV;
-      const R vm35 = V - R(35.0);
-      const R e0 = R(0.012) * vm35, e1 = R(-0.05) * vm35;
-      const R e2 = rdiv(V + R(12.5), divk(5.0)), e3 = rdiv(V + R(30.0), divk(12.0));
-      // One exp per lane.  Independent selects: a chained conditional on a
-      // lane-varying value becomes a branch tree, and since the lanes of a
-      // wavefront disagree every arm of it runs, each behind its own exec-mask
-      // handling.
-      R earg = e3;
-      earg = (c == 2) ? e2 : earg;
-      earg = (c == 1) ? e1 : earg;
-      earg = (c == 0) ? e0 : earg;
-      const R ex = xexp(earg);
-      const R alpha = R(alphacf) * qperm<QB0>(ex);
-      const R beta = R(betacf) * qperm<QB1>(ex);
-      const R yinf = rdiv(R(yCa_frac), R(1.0) + qperm<QB2>(ex)) + (R(1.0) - R(yCa_frac));
-      const R tau = R(60.0) + rdiv(R(340.0), R(1.0) + qperm<QB3>(ex));
-      const R bp = rdiv(beta, divk(bL * 1.0));
-      const R cc = (R)css;
-      const int ls = ch_lcc(w);
-      bad |= (ls < 1 || ls > 12);
-      R lb, lf, lm, lt;
-      const int lk = ls - 7;  // CCa index; other states read entry 0 and ignore it
-      lcc_arith<R>(ls, alpha, beta, bp, gcf * cc, lom[(lk >= 0 && lk <= 4) ? lk : 0][tl], lb, lf, lm, lt);
-      const R vr = rdiv(ch_y(w) == 2 ? (R(1.0) - yinf) : yinf, tau);
-      const R ir = (ch_ito2(w) == 1) ? R(kfIto2) * cc : R(kbIto2);
-      R maxr = vmax(vmax(vmax(R(0.0), lt), vr), ir);
-      const RyrK<R> k = ryr_k(cc, k23c);
-      int st[kNryr];
-      int nd = 0;
-      unsigned need = 0;
-      uint32_t rbits = ryr_resolve_bits(3);  // independent selects, as for earg
-      rbits = (k.regime == 2) ? ryr_resolve_bits(2) : rbits;
-      rbits = (k.regime == 1) ? ryr_resolve_bits(1) : rbits;
-#pragma unroll
-      for (int r = 0; r < kNryr; ++r) {
-        const uint32_t b = (rbits >> (4u * ((w >> (5 + 3 * r)) & 7u))) & 15u;  // raw state ch_ryr(w, r)
-        st[r] = (int)(b & 7u) + 1;
-        nd += (int)(b >> 3);
-        need |= (b >> 3) << r;
-      }
-      // merged-state uniforms come first, cleft by cleft (fru_rates_local.c:452,476,581)
-      const int n0 = qperm<QB0>(nd), n1 = qperm<QB1>(nd), n2 = qperm<QB2>(nd), n3 = qperm<QB3>(nd);
-      const int E = n0 + n1 + n2 + n3;
-      const int lazy0 = idx > kMtN ? idx : kMtN;
-      // totals of the resolved RyR states (fru_rates_local.c:390-727); they
-      // depend on CaSS alone, so they are computed ahead of the merged draws
-      const R T2 = (R(k21) + k.k23 + k.k25);
-      const R T3 = (R(k32) + k.k34);
-      const R T5 = (R(k52) + k.k54 + k.k56);
-      const R a7 = rdiv(k.k34, R(k43) + k.k34);
-      const R T7 = ((R(1.0) - a7) * R(k32)) + (a7 * R(k45));
-      const R a8 = rdiv(R(k65), k.k56 + R(k65));
-      const R T8 = (a8 * R(k52)) + (a8 * k.k54);
-      rtot[0][tl] = k.k12;
-      rtot[1][tl] = T2;
-      rtot[2][tl] = T3;
-      rtot[4][tl] = T5;
-      rtot[6][tl] = T7;
-      rtot[7][tl] = T8;
-      if (nd) {
-        const int Ec = (c > 0 ? n0 : 0) + (c > 1 ? n1 : 0) + (c > 2 ? n2 : 0);
-        uint32_t vw[kNryr];
-        mt_run<kNryr>(row, idx + Ec, nd, vw);
-#pragma unroll
-        for (int r = 0; r < kNryr; ++r) {
-          if ((need >> r) & 1u) {
-            st[r] = ryr_split(st[r], unif(vw[0]), k);
-#pragma unroll
-            for (int j = 0; j < kNryr - 1; ++j) vw[j] = vw[j + 1];
-          }
-        }
-      }
-      R tr[kNryr];
-#pragma unroll
-      for (int r = 0; r < kNryr; ++r) {
-        w = ch_set_ryr(w, r, st[r]);
-        tr[r] = rtot[st[r] - 1][tl];  // total of the resolved state (fru_rates_local.c:390-727)
-        maxr = vmax_nc(maxr, tr[r]);
-      }
-      maxr = vmax_nc(maxr, qperm<QX1>(maxr));
-      maxr = vmax_nc(maxr, qperm<QX2>(maxr));
-      bad |= qperm<QX1>(bad);
-      bad |= qperm<QX2>(bad);
-      // ---- step length (simfru_local.c:135-146)
-      double dt = 0.1 / (double)maxr;
-      dt = vmin(dt, 10.001e-3);
-      double cm = vmax_nc(css, qperm<QX1>(css));
-      cm = vmax_nc(cm, qperm<QX2>(cm));
-      const double cmax = vmax(0.e-3, cm);
-      if (cmax > 1.5e-3) dt = vmin(dt, 5.001e-3);
-      dt = vmin(dt, t1 - t);
-      // an unknown channel state, non-finite rates or a runaway (quad-uniform)
-      if (bad || !(t + dt > t) || ++nguard > kStepCap) {
-        bad = 1;
-        break;
-      }
-      // ---- this cleft's 8 of the step's 32 uniforms (simfru_local.c:211), from
-      // the words fetched during the previous step; when merged-state draws
-      // moved the stream (E > 0, rare) the window is fetched again at the new
-      // position first, so that both cases run the same code and the common
-      // one waits for nothing but its window
-      uint32_t dw[8];
-      if (E != 0) mt_fetch8(row, idx + E + 8 * c, win);
-      mt_finish8(row, idx + E + 8 * c, win, dw);
-      {
-        const int idn = idx + E + 32;
-        const int adv = idn - lazy0;
-        nreg += (adv > 0 ? adv : 0);
-        idx = (idn >= 2 * kMtN) ? idn - kMtN : idn;
-      }
-      // Next step's words, fetched now so their latency hides behind this
-      // step's Heun step and transitions and the next step's rates.  The words
-      // used are positions idx .. idx+31 and those they regenerate from:
-      // positions idx+1 .. idx+32 (not yet regenerated) and words at least 195
-      // positions older than idx, none of which this step's stores (positions
-      // idx-32 .. idx-1, in either copy) or any store before their use touches.
-      if (!LATE) mt_fetch8(row, idx + 8 * c, win);
-      // ---- transition decisions of this cleft (simfru_local.c:215-276); they
-      // use only the step-start state, so they are taken ahead of the Heun step
-      // in the same straight-line code and the two interleave
-      const double u0 = unif(dw[0]);
-      const bool mv_lcc = u0 > 1.0 - dt * (double)lt;
-      const bool mv_y = unif(dw[1]) >= 1.0 - dt * (double)vr;
-      const bool mv_ito = unif(dw[7]) >= 1.0 - dt * (double)ir;
-      double ur[kNryr];
-      bool mv_r[kNryr];
-      bool moved = false;
-#pragma unroll
-      for (int r = 0; r < kNryr; ++r) {
-        ur[r] = unif(dw[2 + r]);
-        mv_r[r] = ur[r] > 1.0 - dt * (double)tr[r];
-        moved |= mv_r[r];
-      }
-      // ---- Heun step (simfru_local.c:149-173)
-      R djs, dss;
-      fcn_fru4<R>((R)cj, cc, gv, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
-      const double k1j = dt * (double)djs, k1s = dt * (double)dss;
-      const double y1j = cj + k1j, y1s = css + k1s;
-      t = t + dt;
-      const double frac = rdiv(t - t0, kint);  // (t - t0) / tint, correctly rounded
-      dV = o0 + frac * span0;
-      dCai = o1 + frac * span1;
-      dNSR = o2 + frac * span2;
-      gv = ghk_terms<R>((R)dV);  // also the next step's first evaluation
-      fcn_fru4<R>((R)y1j, (R)y1s, gv, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
-      cj = cj + (k1j + dt * (double)djs) / 2.0;
-      css = css + (k1s + dt * (double)dss) / 2.0;
-      // ---- the transitions taken
-      if (mv_lcc) w = ch_set_lcc(w, lcc_move<R>(ls, u0, dt, lb, lf, lm, lt));
-      if (mv_y) w = ch_flip_y(w);
-      // the RyRs that move, lowest first, one exit walk per round: lanes moving
-      // different RyRs share a round instead of running one body per RyR index
-      // (the moves touch disjoint fields of w, so the order is immaterial)
-      unsigned mm = 0;
-#pragma unroll
-      for (int r = 0; r < kNryr; ++r) mm |= mv_r[r] ? (1u << r) : 0u;
-      while (mm) {
-        const int r = __builtin_ctz(mm);
-        mm &= mm - 1u;
-        int sr = st[0];
-        double ru = ur[0];
-#pragma unroll
-        for (int q = 1; q < kNryr; ++q) {
-          sr = (r == q) ? st[q] : sr;
-          ru = (r == q) ? ur[q] : ru;
-        }
-        Exits<R> e;
-        ryr_exits_sel(sr, k, a7, a8, rtot[sr - 1][tl], e);
-        w = ch_set_ryr(w, r, pick<R>(ru, dt, e));
-      }
-      if (mv_ito) w = ch_flip_ito2(w);
-      lopen = ch_lcc_open(w);
-      if (moved) nopen = ch_ryr_open(w);
-      steps += (c == 0);
+    quad_prime<R, LATE>(k, s);
+    while (s.t < k.t1) {
+      if (!quad_step<R, NT, LATE>(k, s, rtot, lom)) break;
     }
-    if (bad && c == 0) atomicOr(d.err, 1);
-    d.conc[(c + 1) * a.ld + f] = css;
-    d.chan[c * a.ld + f] = w;
-    if (c == 0) {
-      d.conc[f] = cj;
-      d.mtidx[f] = idx;
-      d.nreg[f] = nreg;
-      if (a.sorted && a.t1 > a.t0) d.cost[f] = steps;
-    }
-    acc[0] = css;
-    acc[1] = (c == 0) ? cj : 0.0;
-    acc[2] = lopen ? css : 0.0;
-    acc[3] = lopen ? 1.0 : 0.0;
-    acc[4] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
-    acc[5] = JRyRmax * ((double)nopen) * (cj - css);  // this cleft's release flux (calc_fru_avg_local.c)
-    if (a.avg) {  // calc_fru_avg_local.c:135-176, this cleft's terms (CaJSR ones in lane 0)
-      int nr = 0;
-#pragma unroll
-      for (int r = 0; r < kNryr; ++r) {
-        const int st = ch_ryr(w, r);
-        nr += (st == 1 || st == 2);
-      }
-      acc[6] = css;
-      acc[7] = (c == 0) ? cj : 0.0;
-      acc[8] = JRyRmax * ((double)nopen) * (cj - css);
-      acc[9] = (double)nopen;
-      acc[10] = (double)nr;
-      acc[11] = css * (1.0 + BSRtot / (KBSR + css) + BSLtot / (KBSL + css));
-      acc[12] = (c == 0) ? cj * (1.0 + CSQNtot / (KmCSQN + cj)) : 0.0;
-      acc[13] = (ch_lcc(w) <= 6) ? 1.0 : 0.0;
-      acc[14] = (ch_y(w) == 2) ? 1.0 : 0.0;
-      acc[15] = lopen ? 1.0 : 0.0;
-      acc[16] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
-    }
+    quad_end<R>(a, d, f, c, s);
+    steps = s.steps;
+    quad_acc(s.css, s.cj, s.w, s.lopen, s.nopen, c, a.avg, acc);
   }
   return steps;
 }
@@ -1771,6 +1879,7 @@ __device__ __forceinline__ void count_steps(const Dev& d, unsigned steps) {
   if ((threadIdx.x & 63) == 0 && s) {
     atomicAdd(d.nsteps, (unsigned long long)s);
     atomicMax(d.maxsteps, m);
+    atomicAdd(d.slotsteps, 16ull * m);  // a wavefront of four-lane quads holds 16 FRU slots
   }
 }
 
@@ -1828,13 +1937,15 @@ __device__ __forceinline__ void publish_partials(double v[NV], unsigned steps, c
       for (int w = 1; w < nw; ++w) s += wsum[w][k];
       __hip_atomic_store(slot + k, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    unsigned tsum = 0u, tmax = 0u;
+    unsigned tsum = 0u, tmax = 0u, tslot = 0u;
     for (int w = 0; w < nw; ++w) {
       tsum += wst[w][0];
       tmax = max(tmax, wst[w][1]);
+      tslot += 16u * wst[w][1];
     }
     __hip_atomic_store(slot + kSlotSteps, (double)tsum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(slot + kSlotMaxSteps, (double)tmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tslot) atomicAdd(d.slotsteps, (unsigned long long)tslot);  // lane-use statistic, nobody waits on it
   }
 }
 
@@ -1906,6 +2017,7 @@ __device__ __forceinline__ bool collect_partials(const Dev& d, int b0, int nb, d
     }
     atomicAdd(d.nsteps, ts);
     atomicAdd(d.critsteps, tm);
+    if (ts) atomicAdd(d.slotsteps, 16ull * tm);  // the block's 16 FRUs step together
   }
   __syncthreads();
   return bad_sh == 0;
@@ -2855,6 +2967,203 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) k_
   mc_wave<R, false, true>(base, d, ctls);
 }
 
+// Refill blocks (GRW_LAYOUT_REFILL): one wavefront per four-lane
+// block, whose 16 quads work through the block's 64 FRUs and take the next one
+// as soon as their own is done.  In the one-wavefront layout a wavefront runs
+// as long as the slowest of its 16 FRUs, and an FRU's micro-step count in the
+// next sub-interval is poorly predicted by its last one (a channel opening
+// raises CaSS and with it the rates), so most FRU lanes of a batch wait.  Here
+// a quad finishing its FRU stores it and takes the next from the block's
+// states, staged in LDS at the start: the block's slots are handed out
+// costliest first (they ascend by cost class), in quad order among the quads
+// free in the same micro-step.  Each FRU's arithmetic is the
+// four-lane kernel's; the flux sums are formed afterwards from the final
+// states kept in LDS, lane for lane as the one-wavefront layout's four
+// wavefronts form them, into the same slots, so k_dev_tail adds the same
+// numbers.  The loop ends when every quad is out of FRUs: each pass either
+// advances every working quad's time (a micro-step, or a stop for a bad state
+// or after kStepCap steps) or hands out slots, of which there are at most 64.
+template <class R, bool LATE>
+__device__ __forceinline__ void mc_refill(const SimArgs& base, const Dev& d, const RkCtl* ctls) {
+  const int i = (int)blockIdx.x;
+  const int blk = base.sorted ? d.border[i] : i;
+  const int lane = threadIdx.x;
+  const int q = lane >> 2, c = lane & 3;
+  const int cell = base.ncells == 1 ? 0 : d.smap[blk];
+  const CellInfo ci = d.cells[cell];
+  const RkCtl* ctl = ctls + cell;
+  if (!ctl->active) return;
+  SimArgs a = base;
+  a.t0 = ctl->ta;
+  a.t1 = ctl->tb;
+  for (int r = 0; r < 3; ++r) {
+    a.dep0[r] = ctl->dep0[r];
+    a.depf[r] = ctl->depf[r];
+  }
+  a.nfru = ci.nfru;
+  a.f0 = ci.f0;
+  a.b0 = ci.sb0;
+  a.k23c = ci.k23c;
+  a.gcf = ci.gcf;
+  const int s0 = ctl->stage;
+  a.pro = (s0 == 1) ? ctl->pro : 0;
+  a.avg = (s0 == 5);
+  const int first = (blk - a.b0) * 64;        // the block's first quad slot in the cell
+  const int nslot = min(64, a.nfru - first);  // FRUs of the block
+  const QuadStage<R> k = quad_stage<R, !LATE>(a, c, lane);
+  __shared__ R rtot[8][64];
+  __shared__ R lom[5][64];
+  quad_tables<R, 64>(rtot, lom, lane);
+  // Every slot's state, staged in LDS at the start so that a quad taking a
+  // slot waits for no memory round trip (lane l loads slot l, with the
+  // checkpoint or the rollback of mpi_slave.c:150-226); slot s's clefts at
+  // entries 4 s + c.  A finished slot's entries then hold its final state,
+  // entry 4 s + c as lane 4 (s mod 16) + c of the one-wavefront layout's
+  // wavefront s / 16 holds it.
+  __shared__ double scj[64], scss[256];
+  __shared__ uint32_t sw[256];
+  __shared__ int sidx[64], snreg[64], sf[64], sopen[256];  // sopen: lopen | nopen << 1 at the end
+  if (lane < nslot) {
+    const int fl = a.f0 + (a.sorted ? d.perm[a.f0 + first + lane] : first + lane);
+    uint32_t* const row = d.mt + (size_t)fl * kMtStride;
+    double cjl, cs[4];
+    uint32_t wl[4];
+    int idxl, nregl;
+    if (a.pro == 2) {  // rollback (mpi_slave.c:150-184): from the checkpoint, MT words restored
+      cjl = d.conc_h[fl];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cs[r] = d.conc_h[(r + 1) * a.ld + fl];
+        wl[r] = d.chan_h[r * a.ld + fl];
+      }
+      idxl = d.mtidx_h[fl];
+      nregl = 0;
+      mt_rollback(row, idxl, d.nreg[fl], d.mtw_h[fl]);
+    } else {
+      cjl = d.conc[fl];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cs[r] = d.conc[(r + 1) * a.ld + fl];
+        wl[r] = d.chan[r * a.ld + fl];
+      }
+      idxl = d.mtidx[fl];
+      nregl = a.pro ? 0 : d.nreg[fl];
+      if (a.pro == 1) {  // checkpoint (mpi_slave.c:192-226)
+        d.conc_h[fl] = cjl;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          d.conc_h[(r + 1) * a.ld + fl] = cs[r];
+          d.chan_h[r * a.ld + fl] = wl[r];
+        }
+        d.mtidx_h[fl] = idxl;
+        d.mtw_h[fl] = row[mt_first_pos(idxl)];
+      }
+    }
+    scj[lane] = cjl;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      scss[4 * lane + r] = cs[r];
+      sw[4 * lane + r] = wl[r];
+    }
+    sidx[lane] = idxl;
+    snreg[lane] = nregl;
+    sf[lane] = fl;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // restored MT words seen by every quad
+  __syncthreads();
+  const Ghk<R> gv0 = ghk_terms<R>((R)k.o0);  // every FRU's first flux factors (V at the sub-interval start)
+  QuadFru<R> s;
+  int sl = nslot - 1 - q;  // this quad's slot (costliest first), < 0: none
+  bool live = sl >= 0;
+  int f = 0;
+  auto take = [&](int slot) {
+    f = sf[slot];
+    s.row = d.mt + (size_t)f * kMtStride;
+    s.cj = scj[slot];
+    s.css = scss[4 * slot + c];
+    s.w = sw[4 * slot + c];
+    s.idx = sidx[slot];
+    s.nreg = snreg[slot];
+    quad_init<R>(k, s);
+    s.gv = gv0;
+    if (!LATE && s.t < k.t1) mt_fetch8(s.row, s.idx + 8 * c, s.win);
+  };
+  if (live) take(sl);
+  int next = 16;  // slots handed out (wave-uniform; may run past nslot)
+  unsigned stot = 0u, smax = 0u, passes = 0u;
+  for (;;) {
+    const bool work = live && s.t < k.t1 && !s.bad;
+    passes += __ballot(work) != 0ull ? 1u : 0u;
+    if (work) (void)quad_step<R, 64, LATE>(k, s, rtot, lom);
+    if (live && !(s.t < k.t1 && !s.bad)) {  // this quad's FRU is done
+      quad_end<R>(a, d, f, c, s);
+      const int e = 4 * sl + c;
+      scss[e] = s.css;
+      sw[e] = s.w;
+      sopen[e] = (s.lopen ? 1 : 0) | (s.nopen << 1);
+      if (c == 0) {
+        scj[sl] = s.cj;
+        stot += s.steps;
+        smax = max(smax, s.steps);
+      }
+      live = false;
+    }
+    const unsigned long long freeq = __ballot(!live && c == 0);  // lane 0 of each free quad
+    if (next < nslot && freeq != 0ull) {
+      const int ns = next + __popcll(freeq & ((1ull << (4 * q)) - 1ull));
+      if (!live && ns < nslot) {
+        sl = nslot - 1 - ns;
+        live = true;
+        take(sl);
+      }
+      next += __popcll(freeq);
+    }
+    if (__ballot(live) == 0ull) break;
+  }
+  __syncthreads();  // the final states in LDS, written by other quads
+#pragma unroll 1
+  for (int w = 0; w < 4; ++w) {
+    const int e = 64 * w + lane;
+    double acc[17] = {};
+    if ((e >> 2) < nslot) {
+      const int o = sopen[e];
+      quad_acc(scss[e], scj[e >> 2], sw[e], (o & 1) != 0, o >> 1, c, a.avg, acc);
+    }
+    double* const slot = d.part + (size_t)(4 * blk + w) * kSlot;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      const double v = wave_sum_xor(acc[r]);
+      if (lane == 0) slot[r] = v;
+    }
+    if (a.avg) {
+#pragma unroll
+      for (int r = 6; r < 17; ++r) {
+        const double v = wave_sum_xor(acc[r]);
+        if (lane == 0) slot[r] = v;
+      }
+    }
+  }
+  const unsigned ss = wave_u32<false>(stot), sm = wave_u32<true>(smax);
+  if (lane == 0 && ss) {
+    atomicAdd(d.nsteps, (unsigned long long)ss);
+    atomicMax(d.maxsteps, sm);
+    atomicAdd(d.slotsteps, 16ull * passes);  // 16 FRU slots per pass of the wavefront
+  }
+}
+
+template <class R>
+__global__ void __launch_bounds__(64) k_mc_refill(const SimArgs base, const Dev d, RkCtl* ctls) {
+  mc_refill<R, false>(base, d, ctls);
+}
+
+// The same with its registers capped so that three wavefronts fit on a SIMD,
+// each step's MT window fetched at the step's start.
+template <class R>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_mc_refill3(const SimArgs base,
+                                                                                      const Dev d, RkCtl* ctls) {
+  mc_refill<R, true>(base, d, ctls);
+}
+
 // The four-lane stage with its registers capped so that W wavefronts fit on a
 // SIMD (512 / W VGPRs): more wavefronts to hide latency when a batch has more
 // of them than the chip holds at once, against spills.  Same arithmetic, same
@@ -3193,6 +3502,7 @@ struct grw_ctx {
   int occ = 0;                     // 0: the stage kernel as compiled; 3, 4: registers capped for 3 / 4 waves per SIMD
   bool queue = false;              // device integrator: Monte Carlo work queue + per-cell tail kernel
   bool wave1 = false;              // device integrator: single-wavefront Monte Carlo blocks + per-cell tail kernel
+  bool refill = false;             // device integrator: refill blocks (quads take the block's next FRU) + tail kernel
   bool poll = false;               // device integrator (single cells): the fcn block collects the partials by polling
   int qgrid = 0;                   // blocks of the queue kernel: as many as the GPU holds at once
   hipGraphExec_t graph = nullptr;  // kGraphAttempts RK attempts, captured on first use
@@ -3298,6 +3608,7 @@ int alloc_fru_state(grw_ctx* c) {
       (rc = dalloc(c, &c->d.ticket, (size_t)c->ncells)) || (rc = dalloc(c, &c->d.qhead, 1)) || (rc = dalloc(c, &c->d.out, 32)) ||
       (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
       (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
+      (rc = dalloc(c, &c->d.slotsteps, 1)) ||
       (rc = dalloc(c, &c->d.tailticks, 1)) || (rc = dalloc(c, &c->d.perm, ld)) ||
       (rc = dalloc(c, &c->d.cost, ld)) || (rc = dalloc(c, &c->d.bkey, (size_t)c->grid_s)) ||
       (rc = dalloc(c, &c->d.border, (size_t)c->grid_s)))
@@ -3430,6 +3741,26 @@ void enqueue_attempt(grw_ctx* c, const SimArgs& base, bool mixed, const hipEvent
       else
         hipExtLaunchKernelGGL(k_dev_stage_poll<double>, grid, dim3(256), 0, c->stream, e0, e1, 0, base, c->d,
                               c->dctl);
+    } else if (c->refill) {
+      if (mixed) {
+        if (c->occ == 3)
+          hipExtLaunchKernelGGL(k_mc_refill3<float>, dim3(c->grid_s), dim3(64), 0, c->stream, e0, nullptr, 0, base,
+                                c->d, c->dctl);
+        else
+          hipExtLaunchKernelGGL(k_mc_refill<float>, dim3(c->grid_s), dim3(64), 0, c->stream, e0, nullptr, 0, base,
+                                c->d, c->dctl);
+        hipExtLaunchKernelGGL(k_dev_tail<float>, dim3(c->ncells), dim3(256), 0, c->stream, nullptr, e1, 0, base, c->d,
+                              c->dctl);
+      } else {
+        if (c->occ == 3)
+          hipExtLaunchKernelGGL(k_mc_refill3<double>, dim3(c->grid_s), dim3(64), 0, c->stream, e0, nullptr, 0, base,
+                                c->d, c->dctl);
+        else
+          hipExtLaunchKernelGGL(k_mc_refill<double>, dim3(c->grid_s), dim3(64), 0, c->stream, e0, nullptr, 0, base,
+                                c->d, c->dctl);
+        hipExtLaunchKernelGGL(k_dev_tail<double>, dim3(c->ncells), dim3(256), 0, c->stream, nullptr, e1, 0, base,
+                              c->d, c->dctl);
+      }
     } else if (c->wave1) {
       const dim3 wgrid(4 * c->grid_s);
       if (mixed) {
@@ -3986,9 +4317,10 @@ int grw_resume_state(grw_ctx* c) {
 
 int grw_get_stats(grw_ctx* c, grw_stats* st) {
   if (!c || !st) return fail(GRW_EINVAL, "null argument");
-  unsigned long long s = 0, cs = 0, tt = 0;
+  unsigned long long s = 0, cs = 0, tt = 0, ss = 0;
   HIPCHK(hipMemcpyAsync(&s, c->d.nsteps, sizeof s, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&cs, c->d.critsteps, sizeof cs, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&ss, c->d.slotsteps, sizeof ss, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&tt, c->d.tailticks, sizeof tt, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   int khz = 0;
@@ -3997,6 +4329,7 @@ int grw_get_stats(grw_ctx* c, grw_stats* st) {
   *st = c->stats;
   st->fru_steps = s;
   st->critical_steps = cs;
+  st->slot_steps = ss;
   st->tail_ms = (double)tt / (double)khz;
   return 0;
 }
@@ -4005,6 +4338,7 @@ int grw_reset_stats(grw_ctx* c) {
   if (!c) return fail(GRW_EINVAL, "null context");
   HIPCHK(hipMemsetAsync(c->d.nsteps, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(hipMemsetAsync(c->d.critsteps, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMemsetAsync(c->d.slotsteps, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(hipMemsetAsync(c->d.tailticks, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->stats = grw_stats{};
@@ -4019,9 +4353,9 @@ int grw_set_timing(grw_ctx* c, int on) {
 
 int grw_set_layout(grw_ctx* c, int layout) {
   if (!c) return fail(GRW_EINVAL, "null context");
-  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_WAVE1_4)
+  if (layout < GRW_LAYOUT_QUAD || layout > GRW_LAYOUT_REFILL_3)
     return fail(GRW_EINVAL, "layout must be one of GRW_LAYOUT_QUAD, _WAVES, _QUAD3, _QUAD4, _QUEUE, _WAVE1, _WAVE1_3, "
-                            "_QUADP, _WAVE1_4");
+                            "_QUADP, _WAVE1_4, _REFILL, _REFILL_3");
   if (layout == GRW_LAYOUT_QUADP && (c->batch || c->ncells != 1))
     return fail(GRW_EINVAL, "the polling layout runs single-cell contexts only");
   if (layout == GRW_LAYOUT_WAVES && (c->batch || c->ncells != 1))
@@ -4034,7 +4368,8 @@ int grw_set_layout(grw_ctx* c, int layout) {
   const bool queue = layout == GRW_LAYOUT_QUEUE;
   const bool wave1 = layout == GRW_LAYOUT_WAVE1 || layout == GRW_LAYOUT_WAVE1_3 || layout == GRW_LAYOUT_WAVE1_4;
   const bool poll = layout == GRW_LAYOUT_QUADP;
-  const int occ = (layout == GRW_LAYOUT_QUAD3 || layout == GRW_LAYOUT_WAVE1_3)   ? 3
+  const bool refill = layout == GRW_LAYOUT_REFILL || layout == GRW_LAYOUT_REFILL_3;
+  const int occ = (layout == GRW_LAYOUT_QUAD3 || layout == GRW_LAYOUT_WAVE1_3 || layout == GRW_LAYOUT_REFILL_3) ? 3
                   : (layout == GRW_LAYOUT_QUAD4 || layout == GRW_LAYOUT_WAVE1_4) ? 4
                                                                                   : 0;
   if (queue && c->qgrid == 0) {  // as many queue blocks as the GPU holds at once
@@ -4048,7 +4383,9 @@ int grw_set_layout(grw_ctx* c, int layout) {
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_mc_queue<double>, 256, 0));
     c->qgrid = std::max(1, std::min(c->grid_s, std::max(per, 1) * std::max(ncu, 1)));
   }
-  if ((ws != c->ws || occ != c->occ || queue != c->queue || wave1 != c->wave1 || poll != c->poll) && c->graph) {
+  if ((ws != c->ws || occ != c->occ || queue != c->queue || wave1 != c->wave1 || poll != c->poll ||
+       refill != c->refill) &&
+      c->graph) {
     // the captured launches are another kernel's
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -4057,6 +4394,7 @@ int grw_set_layout(grw_ctx* c, int layout) {
   c->occ = occ;
   c->queue = queue;
   c->wave1 = wave1;
+  c->refill = refill;
   if (poll && !c->poll) {  // every slot empty before the first polled launch
     const int n = c->grid_s * kSlot;
     hipLaunchKernelGGL(k_arm_slots, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.part2, n);
@@ -4366,6 +4704,7 @@ int snapshot_take(grw_ctx* c, CtxSnapshot& s) {
                {c->d.err, sizeof(int)},
                {c->d.maxsteps, sizeof(unsigned int)},
                {c->d.critsteps, sizeof(unsigned long long)},
+               {c->d.slotsteps, sizeof(unsigned long long)},
                {c->d.tailticks, sizeof(unsigned long long)},
                {c->d.perm, ld * sizeof(int32_t)},
                {c->d.cost, ld * sizeof(uint32_t)},
@@ -4410,8 +4749,9 @@ int autotune(grw_ctx* c, const std::vector<int>& layouts, const std::function<in
              GrwTune* out) {
   const bool ws0 = c->ws, g0 = c->graphs;
   const int occ0 = c->occ;
-  const int layout0 = c->poll ? GRW_LAYOUT_QUADP
-                      : c->queue ? GRW_LAYOUT_QUEUE
+  const int layout0 = c->poll     ? GRW_LAYOUT_QUADP
+                      : c->refill ? (occ0 == 3 ? GRW_LAYOUT_REFILL_3 : GRW_LAYOUT_REFILL)
+                      : c->queue  ? GRW_LAYOUT_QUEUE
                       : c->wave1  ? (occ0 == 3 ? GRW_LAYOUT_WAVE1_3 : occ0 == 4 ? GRW_LAYOUT_WAVE1_4 : GRW_LAYOUT_WAVE1)
                       : ws0       ? GRW_LAYOUT_WAVES
                       : occ0 == 3 ? GRW_LAYOUT_QUAD3
@@ -4488,7 +4828,7 @@ int grw_autotune_batch(grw_ctx* c, int trial_steps, GrwTune* out) {
   };
   return autotune(c,
                   {GRW_LAYOUT_QUAD, GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4, GRW_LAYOUT_WAVE1, GRW_LAYOUT_WAVE1_3,
-                   GRW_LAYOUT_WAVE1_4},
+                   GRW_LAYOUT_WAVE1_4, GRW_LAYOUT_REFILL, GRW_LAYOUT_REFILL_3},
                   run, trial_steps, out);
 }
 

@@ -163,7 +163,7 @@ class FruCell:
         check(self._L.grw_get_stats(self._h, C.byref(st)))
         return dict(fru_steps=st.fru_steps, simfru_calls=st.simfru_calls, launches=st.launches,
                     kernel_ms=st.kernel_ms, critical_steps=st.critical_steps, tail_ms=st.tail_ms,
-                    timed_launches=st.timed_launches)
+                    timed_launches=st.timed_launches, slot_steps=st.slot_steps)
 
     def reset_stats(self) -> None:
         check(self._L.grw_reset_stats(self._h))

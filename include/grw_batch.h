@@ -65,10 +65,10 @@ int grw_batch_run(grw_ctx *ctx, const int32_t *nsteps, double *states, GrwRkStat
 
 /* Autotuner for a batch: as grw_autotune_cell, timing grw_batch_run of
  * trial_steps output steps of every cell under GRW_LAYOUT_QUAD,
- * GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4, GRW_LAYOUT_WAVE1, GRW_LAYOUT_WAVE1_3 and GRW_LAYOUT_WAVE1_4, with and
- * without graph replays, from
- * a snapshot of the batch (which needs device memory for a second copy of its
- * FRU state). */
+ * GRW_LAYOUT_QUAD3, GRW_LAYOUT_QUAD4, GRW_LAYOUT_WAVE1, GRW_LAYOUT_WAVE1_3,
+ * GRW_LAYOUT_WAVE1_4, GRW_LAYOUT_REFILL and GRW_LAYOUT_REFILL_3, with and
+ * without graph replays, from a snapshot of the batch (which needs device
+ * memory for a second copy of its FRU state). */
 int grw_autotune_batch(grw_ctx *ctx, int trial_steps, GrwTune *out);
 
 /* Beat records of cell k, as grw_get_beats. */

@@ -109,7 +109,14 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
  *   GRW_LAYOUT_QUADP  single-cell contexts only: the four-lane kernel whose
  *                     Monte Carlo blocks leave their partial sums without the
  *                     last-block election, and the cell's fcn block collects
- *                     them as they arrive and runs the integrator stage.
+ *                     them as they arrive and runs the integrator stage;
+ *   GRW_LAYOUT_REFILL one wavefront per 64 FRUs whose 16 quads each take the
+ *                     block's next FRU as soon as their own is done (so that
+ *                     FRU lanes do not wait for the slowest FRU of a fixed
+ *                     group), then one block per cell for the cell's fcn and
+ *                     integrator stage;
+ *   GRW_LAYOUT_REFILL_3  the same with its registers capped for three
+ *                     wavefronts per SIMD.
  * Every layout gives the same results bit for bit; the speed differs with the
  * workload (grw_autotune_cell / grw_autotune_batch time them).  No counterpart
  * in the reference. */
@@ -122,11 +129,13 @@ int grw_run_cell_device(grw_ctx *ctx, const GrwProtocol *p, const double errw[37
 #define GRW_LAYOUT_WAVE1_3 7
 #define GRW_LAYOUT_QUADP 8
 #define GRW_LAYOUT_WAVE1_4 9
+#define GRW_LAYOUT_REFILL 10
+#define GRW_LAYOUT_REFILL_3 11
 int grw_set_layout(grw_ctx *ctx, int layout);
 
 /* Result of an autotuning pass: the choice (now set on the context) and every
  * candidate timed, with the wall time of its trial run. */
-#define GRW_TUNE_MAX 12
+#define GRW_TUNE_MAX 16
 typedef struct {
   int32_t layout, graphs; /* the fastest candidate (grw_set_layout, grw_set_graphs) */
   int32_t ncand, reserved;

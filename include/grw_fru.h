@@ -57,6 +57,9 @@ typedef struct {
   uint64_t timed_launches; /* launches that kernel_ms covers: every launch of the host-driven
                               path; the device integrator times the five launches of one RK
                               attempt in eight, a systematic sample */
+  uint64_t slot_steps;   /* sum over wavefronts of their FRU slots times their slowest FRU's
+                            micro-steps: fru_steps / slot_steps is the fraction of FRU lanes
+                            doing micro-steps while their wavefronts run */
 } grw_stats;
 
 const char *grw_last_error(void);

@@ -42,6 +42,8 @@ def test_cell_graph_replays_equal_direct_launches(precision, timing):
     assert got[1].states[:, 0].max() > 0.0  # the stimulus at 10 ms fired in the second piece
     if timing:
         assert st["timed_launches"] > 0 and st["kernel_ms"] > 0.0
+    # every FRU-step sits in a wavefront slot that ran as long as its slowest FRU
+    assert 0 < st["fru_steps"] <= st["slot_steps"] <= 16 * st["fru_steps"]
 
 
 def test_batch_graph_replays_equal_direct_launches():

@@ -97,7 +97,7 @@ def test_wave_layout_is_refused_for_batches():
         assert L.grw_set_layout(bt._h, 5) != 0  # the queue layout is switched off
         assert L.grw_set_layout(bt._h, 8) != 0  # the polling layout runs single cells only
     with Cell(64) as cell:
-        assert L.grw_set_layout(cell.fru._h, 10) != 0
+        assert L.grw_set_layout(cell.fru._h, 12) != 0
         assert L.grw_set_layout(cell.fru._h, 0) != 0
 
 
@@ -108,7 +108,8 @@ QUEUE_OFF = pytest.mark.skip(reason="the queue layout is switched off: its one d
 @pytest.mark.parametrize("layout,precision", [("quad3", "fp64"), ("quad4", "fp64"), ("quad3", "mixed"),
                                               ("wave1", "fp64"), ("wave1", "mixed"), ("wave1_3", "fp64"),
                                               ("wave1_3", "mixed"), ("quadp", "fp64"), ("quadp", "mixed"),
-                                              ("wave1_4", "mixed"),
+                                              ("wave1_4", "mixed"), ("refill", "fp64"), ("refill", "mixed"),
+                                              ("refill_3", "fp64"), ("refill_3", "mixed"),
                                               pytest.param("queue", "fp64", marks=QUEUE_OFF),
                                               pytest.param("queue", "mixed", marks=QUEUE_OFF)])
 def test_other_layouts_equal_quad(layout, precision):
@@ -143,14 +144,14 @@ def test_batch_layouts_and_autotune_give_the_same_run():
     restores it) end in the same states and beats as the default."""
     print("batch layouts + autotune", flush=True)
     ref, rb, _ = _batch_run()
-    for lay in ("quad3", "quad4", "wave1", "wave1_3", "wave1_4"):
+    for lay in ("quad3", "quad4", "wave1", "wave1_3", "wave1_4", "refill", "refill_3"):
         st, b, _ = _batch_run(layout=lay)
         np.testing.assert_array_equal(st, ref, err_msg=lay)
         for x, y in zip(b, rb):
             np.testing.assert_array_equal(x, y)
     st, b, info = _batch_run(tune=True)
-    assert info["layout"] in ("quad", "quad3", "quad4", "wave1", "wave1_3", "wave1_4")
-    assert len(info["candidates"]) == 12 and all(c["ms"] > 0 for c in info["candidates"])
+    assert info["layout"] in ("quad", "quad3", "quad4", "wave1", "wave1_3", "wave1_4", "refill", "refill_3")
+    assert len(info["candidates"]) == 16 and all(c["ms"] > 0 for c in info["candidates"])
     np.testing.assert_array_equal(st, ref)
     for x, y in zip(b, rb):
         np.testing.assert_array_equal(x, y)
