@@ -665,7 +665,7 @@ def batched(args, precision: str | None = None, sequential: bool = True):
                       " over all cells",
                       avg_launch_us=round(avg_s * 1e6, 3),
                       launches=st["launches"], timed_launches=st["timed_launches"],
-                      lane_use=round(st["fru_steps"] / max(st["slot_steps"], 1), 4)),
+                      lane_use=round(st["fru_steps"] / st["slot_steps"], 4) if st["slot_steps"] else None),
         sequential=dict(sample_cells=[int(k) for k in pick], sample_walls_s=[round(x, 3) for x in seq],
                         est_wall_s=round(seq_wall, 2), beats_per_s_est=round(beats / seq_wall, 4)) if seq else None,
         speedup_vs_sequential=round(seq_wall / wall, 2) if seq else None)
@@ -939,7 +939,7 @@ def main():
                     critical_steps_per_launch=round(st["critical_steps"] / st["launches"], 2),
                     us_per_critical_step=round(avg_s * 1e6 * st["launches"] / max(st["critical_steps"], 1), 3),
                     serial_tail_us=round(st["tail_ms"] * 1e3 / st["launches"], 3),
-                    lane_use=round(st["fru_steps"] / max(st["slot_steps"], 1), 4),
+                    lane_use=round(st["fru_steps"] / st["slot_steps"], 4) if st["slot_steps"] else None,
                     alg_bytes_per_launch=round(alg_bytes / st["launches"], 1))
         if kt:
             roof["kernel_busy_ms"] = round(kt["busy_ms"], 3)

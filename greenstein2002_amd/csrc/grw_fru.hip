@@ -1042,12 +1042,11 @@ __device__ __forceinline__ void quad_init(const QuadStage<R>& k, QuadFru<R>& s) 
   s.steps = 0;
 }
 
-// Load FRU f's state at the start of its sub-interval, with the checkpoint
-// (a.pro == 1) or the rollback to it (a.pro == 2) of mpi_slave.c:150-226.
-template <class R, bool LATE>
-__device__ __forceinline__ void quad_begin(const SimArgs& a, const Dev& d, int f, const QuadStage<R>& k,
-                                           QuadFru<R>& s) {
-  const int c = k.c;
+// Load FRU f's state at the start of its sub-interval (cleft c of the quad),
+// with the checkpoint (a.pro == 1) or the rollback to it (a.pro == 2) of
+// mpi_slave.c:150-226.
+template <class R>
+__device__ __forceinline__ void quad_load(const SimArgs& a, const Dev& d, int f, int c, QuadFru<R>& s) {
   s.row = d.mt + (size_t)f * kMtStride;
   if (a.pro == 2) {
     // rollback (mpi_slave.c:150-184): start from the checkpoint; lane 0
@@ -1075,7 +1074,6 @@ __device__ __forceinline__ void quad_begin(const SimArgs& a, const Dev& d, int f
       }
     }
   }
-  quad_init<R>(k, s);
 }
 
 // The first window of draws (one step ahead unless LATE) and the V-only flux
@@ -1332,9 +1330,10 @@ __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, 
   if (slot < a.nfru) {  // quads are active or inactive as a whole
     const int f = a.f0 + (a.sorted ? d.perm[a.f0 + slot] : slot);
     const int tl = threadIdx.x;
-    const QuadStage<R> k = quad_stage<R, VINV>(a, c, tl);
     QuadFru<R> s;
-    quad_begin<R, LATE>(a, d, f, k, s);
+    quad_load<R>(a, d, f, c, s);
+    const QuadStage<R> k = quad_stage<R, VINV>(a, c, tl);
+    quad_init<R>(k, s);
     // Each lane's totals of the eight RyR states, one LDS column per lane, so
     // that a RyR's total is one indexed read instead of a chain of selects
     // over its state.  A lane reads only what it wrote itself, in program
@@ -1937,15 +1936,13 @@ __device__ __forceinline__ void publish_partials(double v[NV], unsigned steps, c
       for (int w = 1; w < nw; ++w) s += wsum[w][k];
       __hip_atomic_store(slot + k, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    unsigned tsum = 0u, tmax = 0u, tslot = 0u;
+    unsigned tsum = 0u, tmax = 0u;
     for (int w = 0; w < nw; ++w) {
       tsum += wst[w][0];
       tmax = max(tmax, wst[w][1]);
-      tslot += 16u * wst[w][1];
     }
     __hip_atomic_store(slot + kSlotSteps, (double)tsum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(slot + kSlotMaxSteps, (double)tmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (tslot) atomicAdd(d.slotsteps, (unsigned long long)tslot);  // lane-use statistic, nobody waits on it
   }
 }
 
@@ -2017,7 +2014,6 @@ __device__ __forceinline__ bool collect_partials(const Dev& d, int b0, int nb, d
     }
     atomicAdd(d.nsteps, ts);
     atomicAdd(d.critsteps, tm);
-    if (ts) atomicAdd(d.slotsteps, 16ull * tm);  // the block's 16 FRUs step together
   }
   __syncthreads();
   return bad_sh == 0;

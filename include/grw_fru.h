@@ -59,7 +59,9 @@ typedef struct {
                               attempt in eight, a systematic sample */
   uint64_t slot_steps;   /* sum over wavefronts of their FRU slots times their slowest FRU's
                             micro-steps: fru_steps / slot_steps is the fraction of FRU lanes
-                            doing micro-steps while their wavefronts run */
+                            doing micro-steps while their wavefronts run; not counted (0) by
+                            the single-cell polling layout, where it would lengthen every
+                            launch's critical path */
 } grw_stats;
 
 const char *grw_last_error(void);
