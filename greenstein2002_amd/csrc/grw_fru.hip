@@ -188,11 +188,10 @@ struct Dev {
   unsigned int* ticket;      // last-block election counters, one per cell (self-resetting)
   unsigned int* qhead;       // next work item of a queue-layout Monte Carlo launch
   double* out;               // [32]: 0-3 fluxes, 4-8 flux sums, 16-26 avg sums
-  unsigned long long* nsteps;
+  unsigned long long* nsteps;        // [2]: FRU-steps, and FRU slots x passes of the wavefronts (lane use)
   int* err;
   unsigned int* maxsteps;           // largest FRU-step count of the current launch
   unsigned long long* critsteps;    // sum over launches of maxsteps
-  unsigned long long* slotsteps;    // sum over wavefronts of (FRU slots x the slowest slot's micro-steps)
   unsigned long long* tailticks;    // wall-clock ticks spent in the serial launch tails
   const CellInfo* cells;            // [ncells]
   const int* smap;                  // Monte Carlo grid block -> cell
@@ -601,7 +600,7 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
     }
     if ((threadIdx.x & 63) == 0 && s) {
       atomicAdd(d.nsteps, s);
-      atomicAdd(d.slotsteps, 64ull * m);  // one FRU per lane
+      atomicAdd(d.nsteps + 1, 64ull * m);  // one FRU per lane
     }
   }
   if (REDUCE) {
@@ -1042,50 +1041,11 @@ __device__ __forceinline__ void quad_init(const QuadStage<R>& k, QuadFru<R>& s) 
   s.steps = 0;
 }
 
-// Load FRU f's state at the start of its sub-interval (cleft c of the quad),
-// with the checkpoint (a.pro == 1) or the rollback to it (a.pro == 2) of
-// mpi_slave.c:150-226.
-template <class R>
-__device__ __forceinline__ void quad_load(const SimArgs& a, const Dev& d, int f, int c, QuadFru<R>& s) {
-  s.row = d.mt + (size_t)f * kMtStride;
-  if (a.pro == 2) {
-    // rollback (mpi_slave.c:150-184): start from the checkpoint; lane 0
-    // restores the MT words, the fence makes them visible to its quad
-    s.cj = d.conc_h[f];
-    s.css = d.conc_h[(c + 1) * a.ld + f];
-    s.w = d.chan_h[c * a.ld + f];
-    s.idx = d.mtidx_h[f];
-    s.nreg = 0;
-    if (c == 0) mt_rollback(s.row, s.idx, d.nreg[f], d.mtw_h[f]);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  } else {
-    s.cj = d.conc[f];
-    s.css = d.conc[(c + 1) * a.ld + f];
-    s.w = d.chan[c * a.ld + f];
-    s.idx = d.mtidx[f];
-    s.nreg = a.pro ? 0 : d.nreg[f];
-    if (a.pro == 1) {  // checkpoint (mpi_slave.c:192-226)
-      d.conc_h[(c + 1) * a.ld + f] = s.css;
-      d.chan_h[c * a.ld + f] = s.w;
-      if (c == 0) {
-        d.conc_h[f] = s.cj;
-        d.mtidx_h[f] = s.idx;
-        d.mtw_h[f] = s.row[mt_first_pos(s.idx)];
-      }
-    }
-  }
-}
-
-// The first window of draws (one step ahead unless LATE) and the V-only flux
-// factors, once the state loads are complete.
-template <class R, bool LATE>
-__device__ __forceinline__ void quad_prime(const QuadStage<R>& k, QuadFru<R>& s) {
-  if (!LATE && s.t < k.t1) mt_fetch8(s.row, s.idx + 8 * k.c, s.win);
-  s.gv = ghk_terms<R>((R)s.dV);
-}
-
-// One micro-step of the quad's FRU (simfru_local.c:127-277).  Returns false,
-// with s.bad set, on an unknown channel state, non-finite rates or a runaway.
+// One micro-step of the quad's FRU (simfru_local.c:127-277), for the refill
+// layout: the four-lane kernel's micro-step (simfru4_gid's loop body, with
+// the same operations in the same order) on the quad's state in a QuadFru.
+// Returns false, with s.bad set, on an unknown channel state, non-finite
+// rates or a runaway.
 template <class R, int NT, bool LATE>
 __device__ __forceinline__ bool quad_step(const QuadStage<R>& k, QuadFru<R>& s, R (*rtot)[NT], R (*lom)[NT]) {
   const int c = k.c, tl = k.tl;
@@ -1278,7 +1238,8 @@ __device__ __forceinline__ bool quad_step(const QuadStage<R>& k, QuadFru<R>& s, 
 }
 
 // The flux-sum terms of one cleft of a finished FRU (calc_fru_flux_local.c,
-// and calc_fru_avg_local.c:135-176 when avg is set; the CaJSR ones in lane 0).
+// and calc_fru_avg_local.c:135-176 when avg is set; the CaJSR ones in lane 0),
+// as simfru4_gid forms them at the end of its loop.
 __device__ __forceinline__ void quad_acc(double css, double cj, uint32_t w, bool lopen, int nopen, int c, bool avg,
                                          double acc[17]) {
   acc[0] = css;
@@ -1308,7 +1269,7 @@ __device__ __forceinline__ void quad_acc(double css, double cj, uint32_t w, bool
   }
 }
 
-// Store the FRU's state at the end of its sub-interval.
+// Store the FRU's state at the end of its sub-interval (as simfru4_gid does).
 template <class R>
 __device__ __forceinline__ void quad_end(const SimArgs& a, const Dev& d, int f, int c, const QuadFru<R>& s) {
   if (s.bad && c == 0) atomicOr(d.err, 1);
@@ -1329,30 +1290,290 @@ __device__ __forceinline__ unsigned simfru4_gid(const SimArgs& a, const Dev& d, 
   unsigned steps = 0;
   if (slot < a.nfru) {  // quads are active or inactive as a whole
     const int f = a.f0 + (a.sorted ? d.perm[a.f0 + slot] : slot);
-    const int tl = threadIdx.x;
-    QuadFru<R> s;
-    quad_load<R>(a, d, f, c, s);
-    const QuadStage<R> k = quad_stage<R, VINV>(a, c, tl);
-    quad_init<R>(k, s);
+    uint32_t* const row = d.mt + (size_t)f * kMtStride;
+    double cj, css;
+    uint32_t w;
+    int idx, nreg;
+    if (a.pro == 2) {
+      // rollback (mpi_slave.c:150-184): start from the checkpoint; lane 0
+      // restores the MT words, the fence makes them visible to its quad
+      cj = d.conc_h[f];
+      css = d.conc_h[(c + 1) * a.ld + f];
+      w = d.chan_h[c * a.ld + f];
+      idx = d.mtidx_h[f];
+      nreg = 0;
+      if (c == 0) mt_rollback(row, idx, d.nreg[f], d.mtw_h[f]);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    } else {
+      cj = d.conc[f];
+      css = d.conc[(c + 1) * a.ld + f];
+      w = d.chan[c * a.ld + f];
+      idx = d.mtidx[f];
+      nreg = a.pro ? 0 : d.nreg[f];
+      if (a.pro == 1) {  // checkpoint (mpi_slave.c:192-226)
+        d.conc_h[(c + 1) * a.ld + f] = css;
+        d.chan_h[c * a.ld + f] = w;
+        if (c == 0) {
+          d.conc_h[f] = cj;
+          d.mtidx_h[f] = idx;
+          d.mtw_h[f] = row[mt_first_pos(idx)];
+        }
+      }
+    }
+    const double t0 = vinv<VINV>(a.t0), t1 = vinv<VINV>(a.t1);
+    const double o0 = vinv<VINV>(a.dep0[0]), o1 = vinv<VINV>(a.dep0[1]), o2 = vinv<VINV>(a.dep0[2]);
+    const double span0 = vinv<VINV>(a.depf[0] - a.dep0[0]);
+    const double span1 = vinv<VINV>(a.depf[1] - a.dep0[1]);
+    const double span2 = vinv<VINV>(a.depf[2] - a.dep0[2]);
+    const double tint = a.t1 - a.t0;
+    const DivK kint{vinv<VINV>(tint), vinv<VINV>(1.0 / tint)};  // the sub-interval length, fixed for the launch
+    double dV = o0, dCai = o1, dNSR = o2;
+    bool lopen = ch_lcc_open(w);
+    int nopen = ch_ryr_open(w);
+    const R k23c = vinv<VINV>((R)a.k23c), gcf = vinv<VINV>((R)a.gcf);
+    double t = t0;
+    int bad = 0;
+    int nguard = 0;
     // Each lane's totals of the eight RyR states, one LDS column per lane, so
     // that a RyR's total is one indexed read instead of a chain of selects
     // over its state.  A lane reads only what it wrote itself, in program
     // order, so no barrier is needed.  States 4 and 6 have constant totals.
     __shared__ R rtot[8][NT];
-    // omega / bL^k of the mode-Ca LCC states, likewise one column per lane
+    const int tl = threadIdx.x;
+    rtot[3][tl] = R(k43 + k45);
+    rtot[5][tl] = R(k65);
+    // omega / bL^k of the mode-Ca LCC states CCa0..CCa4, likewise one column
+    // per lane (fru_rates_local.c:70-76 constant expressions)
     __shared__ R lom[5][NT];
-    quad_tables<R, NT>(rtot, lom, tl);
+    lom[0][tl] = R(omega);
+    lom[1][tl] = R(omega / bL);
+    lom[2][tl] = R(omega / bL2);
+    lom[3][tl] = R(omega / bL3);
+    lom[4][tl] = R(omega / bL4);
+    MtWin8 win;  // words of the next step's draws, fetched one step ahead
     // Complete the FRU-state loads here: otherwise the loop header merges them
     // with the prefetch below and the first use of the state in every step
     // waits for all outstanding loads, prefetch included.
     __builtin_amdgcn_s_waitcnt(0);
-    quad_prime<R, LATE>(k, s);
-    while (s.t < k.t1) {
-      if (!quad_step<R, NT, LATE>(k, s, rtot, lom)) break;
+    if (!LATE && t < t1) mt_fetch8(row, idx + 8 * c, win);
+    Ghk<R> gv = ghk_terms<R>((R)dV);
+    // The refill layout runs the same micro-step through quad_step (its
+    // results are held to these bit for bit by the layout parity tests); this
+    // kernel keeps the loop inline, which compiles the single-cell launch 1%
+    // faster than the same code through quad_step.
+    while (t < t1) {
+      if (LATE) mt_fetch8(row, idx + 8 * c, win);  // this step's window, fetched at its start
+      // ---- rates at the step start (fru_rates_local.c:126-777)
+      const R V = (R)dV;
+      const R vm35 = V - R(35.0);
+      const R e0 = R(0.012) * vm35, e1 = R(-0.05) * vm35;
+      const R e2 = rdiv(V + R(12.5), divk(5.0)), e3 = rdiv(V + R(30.0), divk(12.0));
+      // One exp per lane.  Independent selects: a chained conditional on a
+      // lane-varying value becomes a branch tree, and since the lanes of a
+      // wavefront disagree every arm of it runs, each behind its own exec-mask
+      // handling.
+      R earg = e3;
+      earg = (c == 2) ? e2 : earg;
+      earg = (c == 1) ? e1 : earg;
+      earg = (c == 0) ? e0 : earg;
+      const R ex = xexp(earg);
+      const R alpha = R(alphacf) * qperm<QB0>(ex);
+      const R beta = R(betacf) * qperm<QB1>(ex);
+      const R yinf = rdiv(R(yCa_frac), R(1.0) + qperm<QB2>(ex)) + (R(1.0) - R(yCa_frac));
+      const R tau = R(60.0) + rdiv(R(340.0), R(1.0) + qperm<QB3>(ex));
+      const R bp = rdiv(beta, divk(bL * 1.0));
+      const R cc = (R)css;
+      const int ls = ch_lcc(w);
+      bad |= (ls < 1 || ls > 12);
+      R lb, lf, lm, lt;
+      const int lk = ls - 7;  // CCa index; other states read entry 0 and ignore it
+      lcc_arith<R>(ls, alpha, beta, bp, gcf * cc, lom[(lk >= 0 && lk <= 4) ? lk : 0][tl], lb, lf, lm, lt);
+      const R vr = rdiv(ch_y(w) == 2 ? (R(1.0) - yinf) : yinf, tau);
+      const R ir = (ch_ito2(w) == 1) ? R(kfIto2) * cc : R(kbIto2);
+      R maxr = vmax(vmax(vmax(R(0.0), lt), vr), ir);
+      const RyrK<R> k = ryr_k(cc, k23c);
+      int st[kNryr];
+      int nd = 0;
+      unsigned need = 0;
+      uint32_t rbits = ryr_resolve_bits(3);  // independent selects, as for earg
+      rbits = (k.regime == 2) ? ryr_resolve_bits(2) : rbits;
+      rbits = (k.regime == 1) ? ryr_resolve_bits(1) : rbits;
+#pragma unroll
+      for (int r = 0; r < kNryr; ++r) {
+        const uint32_t b = (rbits >> (4u * ((w >> (5 + 3 * r)) & 7u))) & 15u;  // raw state ch_ryr(w, r)
+        st[r] = (int)(b & 7u) + 1;
+        nd += (int)(b >> 3);
+        need |= (b >> 3) << r;
+      }
+      // merged-state uniforms come first, cleft by cleft (fru_rates_local.c:452,476,581)
+      const int n0 = qperm<QB0>(nd), n1 = qperm<QB1>(nd), n2 = qperm<QB2>(nd), n3 = qperm<QB3>(nd);
+      const int E = n0 + n1 + n2 + n3;
+      const int lazy0 = idx > kMtN ? idx : kMtN;
+      // totals of the resolved RyR states (fru_rates_local.c:390-727); they
+      // depend on CaSS alone, so they are computed ahead of the merged draws
+      const R T2 = (R(k21) + k.k23 + k.k25);
+      const R T3 = (R(k32) + k.k34);
+      const R T5 = (R(k52) + k.k54 + k.k56);
+      const R a7 = rdiv(k.k34, R(k43) + k.k34);
+      const R T7 = ((R(1.0) - a7) * R(k32)) + (a7 * R(k45));
+      const R a8 = rdiv(R(k65), k.k56 + R(k65));
+      const R T8 = (a8 * R(k52)) + (a8 * k.k54);
+      rtot[0][tl] = k.k12;
+      rtot[1][tl] = T2;
+      rtot[2][tl] = T3;
+      rtot[4][tl] = T5;
+      rtot[6][tl] = T7;
+      rtot[7][tl] = T8;
+      if (nd) {
+        const int Ec = (c > 0 ? n0 : 0) + (c > 1 ? n1 : 0) + (c > 2 ? n2 : 0);
+        uint32_t vw[kNryr];
+        mt_run<kNryr>(row, idx + Ec, nd, vw);
+#pragma unroll
+        for (int r = 0; r < kNryr; ++r) {
+          if ((need >> r) & 1u) {
+            st[r] = ryr_split(st[r], unif(vw[0]), k);
+#pragma unroll
+            for (int j = 0; j < kNryr - 1; ++j) vw[j] = vw[j + 1];
+          }
+        }
+      }
+      R tr[kNryr];
+#pragma unroll
+      for (int r = 0; r < kNryr; ++r) {
+        w = ch_set_ryr(w, r, st[r]);
+        tr[r] = rtot[st[r] - 1][tl];  // total of the resolved state (fru_rates_local.c:390-727)
+        maxr = vmax_nc(maxr, tr[r]);
+      }
+      maxr = vmax_nc(maxr, qperm<QX1>(maxr));
+      maxr = vmax_nc(maxr, qperm<QX2>(maxr));
+      bad |= qperm<QX1>(bad);
+      bad |= qperm<QX2>(bad);
+      // ---- step length (simfru_local.c:135-146)
+      double dt = 0.1 / (double)maxr;
+      dt = vmin(dt, 10.001e-3);
+      double cm = vmax_nc(css, qperm<QX1>(css));
+      cm = vmax_nc(cm, qperm<QX2>(cm));
+      const double cmax = vmax(0.e-3, cm);
+      if (cmax > 1.5e-3) dt = vmin(dt, 5.001e-3);
+      dt = vmin(dt, t1 - t);
+      // an unknown channel state, non-finite rates or a runaway (quad-uniform)
+      if (bad || !(t + dt > t) || ++nguard > kStepCap) {
+        bad = 1;
+        break;
+      }
+      // ---- this cleft's 8 of the step's 32 uniforms (simfru_local.c:211), from
+      // the words fetched during the previous step; when merged-state draws
+      // moved the stream (E > 0, rare) the window is fetched again at the new
+      // position first, so that both cases run the same code and the common
+      // one waits for nothing but its window
+      uint32_t dw[8];
+      if (E != 0) mt_fetch8(row, idx + E + 8 * c, win);
+      mt_finish8(row, idx + E + 8 * c, win, dw);
+      {
+        const int idn = idx + E + 32;
+        const int adv = idn - lazy0;
+        nreg += (adv > 0 ? adv : 0);
+        idx = (idn >= 2 * kMtN) ? idn - kMtN : idn;
+      }
+      // Next step's words, fetched now so their latency hides behind this
+      // step's Heun step and transitions and the next step's rates.  The words
+      // used are positions idx .. idx+31 and those they regenerate from:
+      // positions idx+1 .. idx+32 (not yet regenerated) and words at least 195
+      // positions older than idx, none of which this step's stores (positions
+      // idx-32 .. idx-1, in either copy) or any store before their use touches.
+      if (!LATE) mt_fetch8(row, idx + 8 * c, win);
+      // ---- transition decisions of this cleft (simfru_local.c:215-276); they
+      // use only the step-start state, so they are taken ahead of the Heun step
+      // in the same straight-line code and the two interleave
+      const double u0 = unif(dw[0]);
+      const bool mv_lcc = u0 > 1.0 - dt * (double)lt;
+      const bool mv_y = unif(dw[1]) >= 1.0 - dt * (double)vr;
+      const bool mv_ito = unif(dw[7]) >= 1.0 - dt * (double)ir;
+      double ur[kNryr];
+      bool mv_r[kNryr];
+      bool moved = false;
+#pragma unroll
+      for (int r = 0; r < kNryr; ++r) {
+        ur[r] = unif(dw[2 + r]);
+        mv_r[r] = ur[r] > 1.0 - dt * (double)tr[r];
+        moved |= mv_r[r];
+      }
+      // ---- Heun step (simfru_local.c:149-173)
+      R djs, dss;
+      fcn_fru4<R>((R)cj, cc, gv, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
+      const double k1j = dt * (double)djs, k1s = dt * (double)dss;
+      const double y1j = cj + k1j, y1s = css + k1s;
+      t = t + dt;
+      const double frac = rdiv(t - t0, kint);  // (t - t0) / tint, correctly rounded
+      dV = o0 + frac * span0;
+      dCai = o1 + frac * span1;
+      dNSR = o2 + frac * span2;
+      gv = ghk_terms<R>((R)dV);  // also the next step's first evaluation
+      fcn_fru4<R>((R)y1j, (R)y1s, gv, (R)dCai, (R)dNSR, lopen, nopen, djs, dss);
+      cj = cj + (k1j + dt * (double)djs) / 2.0;
+      css = css + (k1s + dt * (double)dss) / 2.0;
+      // ---- the transitions taken
+      if (mv_lcc) w = ch_set_lcc(w, lcc_move<R>(ls, u0, dt, lb, lf, lm, lt));
+      if (mv_y) w = ch_flip_y(w);
+      // the RyRs that move, lowest first, one exit walk per round: lanes moving
+      // different RyRs share a round instead of running one body per RyR index
+      // (the moves touch disjoint fields of w, so the order is immaterial)
+      unsigned mm = 0;
+#pragma unroll
+      for (int r = 0; r < kNryr; ++r) mm |= mv_r[r] ? (1u << r) : 0u;
+      while (mm) {
+        const int r = __builtin_ctz(mm);
+        mm &= mm - 1u;
+        int sr = st[0];
+        double ru = ur[0];
+#pragma unroll
+        for (int q = 1; q < kNryr; ++q) {
+          sr = (r == q) ? st[q] : sr;
+          ru = (r == q) ? ur[q] : ru;
+        }
+        Exits<R> e;
+        ryr_exits_sel(sr, k, a7, a8, rtot[sr - 1][tl], e);
+        w = ch_set_ryr(w, r, pick<R>(ru, dt, e));
+      }
+      if (mv_ito) w = ch_flip_ito2(w);
+      lopen = ch_lcc_open(w);
+      if (moved) nopen = ch_ryr_open(w);
+      steps += (c == 0);
     }
-    quad_end<R>(a, d, f, c, s);
-    steps = s.steps;
-    quad_acc(s.css, s.cj, s.w, s.lopen, s.nopen, c, a.avg, acc);
+    if (bad && c == 0) atomicOr(d.err, 1);
+    d.conc[(c + 1) * a.ld + f] = css;
+    d.chan[c * a.ld + f] = w;
+    if (c == 0) {
+      d.conc[f] = cj;
+      d.mtidx[f] = idx;
+      d.nreg[f] = nreg;
+      if (a.sorted && a.t1 > a.t0) d.cost[f] = steps;
+    }
+    acc[0] = css;
+    acc[1] = (c == 0) ? cj : 0.0;
+    acc[2] = lopen ? css : 0.0;
+    acc[3] = lopen ? 1.0 : 0.0;
+    acc[4] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
+    acc[5] = JRyRmax * ((double)nopen) * (cj - css);  // this cleft's release flux (calc_fru_avg_local.c)
+    if (a.avg) {  // calc_fru_avg_local.c:135-176, this cleft's terms (CaJSR ones in lane 0)
+      int nr = 0;
+#pragma unroll
+      for (int r = 0; r < kNryr; ++r) {
+        const int st = ch_ryr(w, r);
+        nr += (st == 1 || st == 2);
+      }
+      acc[6] = css;
+      acc[7] = (c == 0) ? cj : 0.0;
+      acc[8] = JRyRmax * ((double)nopen) * (cj - css);
+      acc[9] = (double)nopen;
+      acc[10] = (double)nr;
+      acc[11] = css * (1.0 + BSRtot / (KBSR + css) + BSLtot / (KBSL + css));
+      acc[12] = (c == 0) ? cj * (1.0 + CSQNtot / (KmCSQN + cj)) : 0.0;
+      acc[13] = (ch_lcc(w) <= 6) ? 1.0 : 0.0;
+      acc[14] = (ch_y(w) == 2) ? 1.0 : 0.0;
+      acc[15] = lopen ? 1.0 : 0.0;
+      acc[16] = (ch_ito2(w) == 2) ? 1.0 : 0.0;
+    }
   }
   return steps;
 }
@@ -1878,7 +2099,7 @@ __device__ __forceinline__ void count_steps(const Dev& d, unsigned steps) {
   if ((threadIdx.x & 63) == 0 && s) {
     atomicAdd(d.nsteps, (unsigned long long)s);
     atomicMax(d.maxsteps, m);
-    atomicAdd(d.slotsteps, 16ull * m);  // a wavefront of four-lane quads holds 16 FRU slots
+    atomicAdd(d.nsteps + 1, 16ull * m);  // a wavefront of four-lane quads holds 16 FRU slots
   }
 }
 
@@ -3143,7 +3364,7 @@ __device__ __forceinline__ void mc_refill(const SimArgs& base, const Dev& d, con
   if (lane == 0 && ss) {
     atomicAdd(d.nsteps, (unsigned long long)ss);
     atomicMax(d.maxsteps, sm);
-    atomicAdd(d.slotsteps, 16ull * passes);  // 16 FRU slots per pass of the wavefront
+    atomicAdd(d.nsteps + 1, 16ull * passes);  // 16 FRU slots per pass of the wavefront
   }
 }
 
@@ -3602,9 +3823,8 @@ int alloc_fru_state(grw_ctx* c) {
       (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, nslots * kSlot)) ||
       (rc = dalloc(c, &c->d.part2, (size_t)c->grid_s * kSlot)) ||
       (rc = dalloc(c, &c->d.ticket, (size_t)c->ncells)) || (rc = dalloc(c, &c->d.qhead, 1)) || (rc = dalloc(c, &c->d.out, 32)) ||
-      (rc = dalloc(c, &c->d.nsteps, 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
+      (rc = dalloc(c, &c->d.nsteps, 2)) || (rc = dalloc(c, &c->d.err, 1)) ||
       (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
-      (rc = dalloc(c, &c->d.slotsteps, 1)) ||
       (rc = dalloc(c, &c->d.tailticks, 1)) || (rc = dalloc(c, &c->d.perm, ld)) ||
       (rc = dalloc(c, &c->d.cost, ld)) || (rc = dalloc(c, &c->d.bkey, (size_t)c->grid_s)) ||
       (rc = dalloc(c, &c->d.border, (size_t)c->grid_s)))
@@ -4316,7 +4536,7 @@ int grw_get_stats(grw_ctx* c, grw_stats* st) {
   unsigned long long s = 0, cs = 0, tt = 0, ss = 0;
   HIPCHK(hipMemcpyAsync(&s, c->d.nsteps, sizeof s, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&cs, c->d.critsteps, sizeof cs, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(&ss, c->d.slotsteps, sizeof ss, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&ss, c->d.nsteps + 1, sizeof ss, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&tt, c->d.tailticks, sizeof tt, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   int khz = 0;
@@ -4332,9 +4552,8 @@ int grw_get_stats(grw_ctx* c, grw_stats* st) {
 
 int grw_reset_stats(grw_ctx* c) {
   if (!c) return fail(GRW_EINVAL, "null context");
-  HIPCHK(hipMemsetAsync(c->d.nsteps, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMemsetAsync(c->d.nsteps, 0, 2 * sizeof(unsigned long long), c->stream));
   HIPCHK(hipMemsetAsync(c->d.critsteps, 0, sizeof(unsigned long long), c->stream));
-  HIPCHK(hipMemsetAsync(c->d.slotsteps, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(hipMemsetAsync(c->d.tailticks, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->stats = grw_stats{};
@@ -4696,11 +4915,10 @@ int snapshot_take(grw_ctx* c, CtxSnapshot& s) {
                {c->d.conc_h, 5 * ld * sizeof(double)},
                {c->d.chan_h, 4 * ld * sizeof(uint32_t)},
                {c->d.mtidx_h, ld * sizeof(int32_t)},
-               {c->d.nsteps, sizeof(unsigned long long)},
+               {c->d.nsteps, 2 * sizeof(unsigned long long)},
                {c->d.err, sizeof(int)},
                {c->d.maxsteps, sizeof(unsigned int)},
                {c->d.critsteps, sizeof(unsigned long long)},
-               {c->d.slotsteps, sizeof(unsigned long long)},
                {c->d.tailticks, sizeof(unsigned long long)},
                {c->d.perm, ld * sizeof(int32_t)},
                {c->d.cost, ld * sizeof(uint32_t)},
