@@ -188,7 +188,7 @@ struct Dev {
   unsigned int* ticket;      // last-block election counters, one per cell (self-resetting)
   unsigned int* qhead;       // next work item of a queue-layout Monte Carlo launch
   double* out;               // [32]: 0-3 fluxes, 4-8 flux sums, 16-26 avg sums
-  unsigned long long* nsteps;        // [2]: FRU-steps, and FRU slots x passes of the wavefronts (lane use)
+  unsigned long long* nsteps;        // [0]: FRU-steps; [kLaneUse]: FRU slots x passes of the wavefronts
   int* err;
   unsigned int* maxsteps;           // largest FRU-step count of the current launch
   unsigned long long* critsteps;    // sum over launches of maxsteps
@@ -225,6 +225,9 @@ __device__ __forceinline__ int pick(double u, double dt, const Exits<R>& e) {
 // in thread 0 of the last block, with the totals in tot[].  Block b owns
 // d.part[b * kSlot, (b + 1) * kSlot); slot_off + NV must not exceed kSlot.
 constexpr int kSlot = 32;
+// The lane-use counter's place after the FRU-step counter: a 128-byte line of
+// its own, so that the two atomics every wavefront ends with do not meet on one line.
+constexpr int kLaneUse = 16;
 
 // The wavefront's xor butterfly v += v[lane ^ k] for k = 32, 16, 8, 4, 2, 1 in
 // that order, with each step's partner value moved by lane permutes instead of
@@ -600,7 +603,7 @@ __global__ void __launch_bounds__(256) k_simfru(const SimArgs a, const Dev d) {
     }
     if ((threadIdx.x & 63) == 0 && s) {
       atomicAdd(d.nsteps, s);
-      atomicAdd(d.nsteps + 1, 64ull * m);  // one FRU per lane
+      atomicAdd(d.nsteps + kLaneUse, 64ull * m);  // one FRU per lane
     }
   }
   if (REDUCE) {
@@ -2099,7 +2102,7 @@ __device__ __forceinline__ void count_steps(const Dev& d, unsigned steps) {
   if ((threadIdx.x & 63) == 0 && s) {
     atomicAdd(d.nsteps, (unsigned long long)s);
     atomicMax(d.maxsteps, m);
-    atomicAdd(d.nsteps + 1, 16ull * m);  // a wavefront of four-lane quads holds 16 FRU slots
+    atomicAdd(d.nsteps + kLaneUse, 16ull * m);  // a wavefront of four-lane quads holds 16 FRU slots
   }
 }
 
@@ -3364,7 +3367,7 @@ __device__ __forceinline__ void mc_refill(const SimArgs& base, const Dev& d, con
   if (lane == 0 && ss) {
     atomicAdd(d.nsteps, (unsigned long long)ss);
     atomicMax(d.maxsteps, sm);
-    atomicAdd(d.nsteps + 1, 16ull * passes);  // 16 FRU slots per pass of the wavefront
+    atomicAdd(d.nsteps + kLaneUse, 16ull * passes);  // 16 FRU slots per pass of the wavefront
   }
 }
 
@@ -3823,7 +3826,7 @@ int alloc_fru_state(grw_ctx* c) {
       (rc = dalloc(c, &c->d.mtidx_h, ld)) || (rc = dalloc(c, &c->d.part, nslots * kSlot)) ||
       (rc = dalloc(c, &c->d.part2, (size_t)c->grid_s * kSlot)) ||
       (rc = dalloc(c, &c->d.ticket, (size_t)c->ncells)) || (rc = dalloc(c, &c->d.qhead, 1)) || (rc = dalloc(c, &c->d.out, 32)) ||
-      (rc = dalloc(c, &c->d.nsteps, 2)) || (rc = dalloc(c, &c->d.err, 1)) ||
+      (rc = dalloc(c, &c->d.nsteps, kLaneUse + 1)) || (rc = dalloc(c, &c->d.err, 1)) ||
       (rc = dalloc(c, &c->d.maxsteps, 1)) || (rc = dalloc(c, &c->d.critsteps, 1)) ||
       (rc = dalloc(c, &c->d.tailticks, 1)) || (rc = dalloc(c, &c->d.perm, ld)) ||
       (rc = dalloc(c, &c->d.cost, ld)) || (rc = dalloc(c, &c->d.bkey, (size_t)c->grid_s)) ||
@@ -4536,7 +4539,7 @@ int grw_get_stats(grw_ctx* c, grw_stats* st) {
   unsigned long long s = 0, cs = 0, tt = 0, ss = 0;
   HIPCHK(hipMemcpyAsync(&s, c->d.nsteps, sizeof s, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&cs, c->d.critsteps, sizeof cs, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(&ss, c->d.nsteps + 1, sizeof ss, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&ss, c->d.nsteps + kLaneUse, sizeof ss, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&tt, c->d.tailticks, sizeof tt, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   int khz = 0;
@@ -4552,7 +4555,7 @@ int grw_get_stats(grw_ctx* c, grw_stats* st) {
 
 int grw_reset_stats(grw_ctx* c) {
   if (!c) return fail(GRW_EINVAL, "null context");
-  HIPCHK(hipMemsetAsync(c->d.nsteps, 0, 2 * sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMemsetAsync(c->d.nsteps, 0, (kLaneUse + 1) * sizeof(unsigned long long), c->stream));
   HIPCHK(hipMemsetAsync(c->d.critsteps, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(hipMemsetAsync(c->d.tailticks, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -4915,7 +4918,7 @@ int snapshot_take(grw_ctx* c, CtxSnapshot& s) {
                {c->d.conc_h, 5 * ld * sizeof(double)},
                {c->d.chan_h, 4 * ld * sizeof(uint32_t)},
                {c->d.mtidx_h, ld * sizeof(int32_t)},
-               {c->d.nsteps, 2 * sizeof(unsigned long long)},
+               {c->d.nsteps, (kLaneUse + 1) * sizeof(unsigned long long)},
                {c->d.err, sizeof(int)},
                {c->d.maxsteps, sizeof(unsigned int)},
                {c->d.critsteps, sizeof(unsigned long long)},
